@@ -1,0 +1,111 @@
+"""K1/K2 parity on the GPU: the divergence-form apply and the fused Chebyshev filter against the oracle.
+
+Tolerances: float64 kernels agree with the float64 oracle to 1e-13 of the largest output entry
+(summation order and FMA contraction differ; the arithmetic is the same); float32 kernels to 2e-5.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle.core import CuspFactor, CuspHamiltonian, Grid, weighted_divergence, staggered_coefficients
+from oracle.eigen import chebyshev_filter
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(shape, spacing, charges, positions, seed=0):
+    lengths = tuple((n - 1) * spacing for n in shape)
+    grid = Grid.from_box(lengths, spacing)
+    factor = CuspFactor(grid, charges, positions)
+    return grid, CuspFactoredRef(grid, factor)
+
+
+class CuspFactoredRef(CuspHamiltonian):
+    pass
+
+
+@pytest.mark.parametrize(
+    "shape,spacing,charges,positions",
+    [
+        ((13, 13, 13), 0.5, (1.0,), [[0.0, 0.0, 0.0]]),
+        ((17, 21, 19), 0.4, (2.0,), [[0.1, -0.2, 0.05]]),
+        ((33, 29, 37), 0.5, (1.0, 1.0), [[0.0, 0.0, -0.7], [0.0, 0.0, 0.7]]),
+        ((5, 3, 70), 0.3, (1.0,), [[0.0, 0.0, 0.0]]),
+    ],
+)
+@pytest.mark.parametrize("k", [1, 3])
+def test_apply_matches_oracle_f64(pkg, device, shape, spacing, charges, positions, k):
+    grid, ref = _case(shape, spacing, charges, positions)
+    geom = pkg.geometry_from_arrays(grid.shape, 8, grid.spacing, ref.face, ref.inverse_weight, ref.potential, device)
+    gen = torch.Generator().manual_seed(k)
+    phi = torch.rand((k, grid.n_points), generator=gen, dtype=torch.float64) * 2 - 1
+    out = pkg.ops.hdiv_apply(phi.to(device), geom).cpu().numpy()
+    expect = ref.apply(phi.numpy())
+    scale = np.abs(expect).max(axis=1, keepdims=True)
+    assert (np.abs(out - expect) / scale).max() < 1e-13
+
+
+@pytest.mark.parametrize("order", [2, 4, 6, 8])
+def test_orders_match_oracle_with_random_weights(pkg, device, order):
+    """Random positive face weights, ``invw`` and ``V``: every point is well conditioned, so the
+    comparison is per point."""
+    rng = np.random.default_rng(order)
+    shape = (11, 18, 23)
+    n = int(np.prod(shape))
+    faces = [rng.uniform(0.5, 1.5, size=tuple(s + (order if d == a else 0) for d, s in enumerate(shape))) for a in range(3)]
+    invw = rng.uniform(0.5, 2.0, size=n)
+    pot = rng.uniform(-1.0, 1.0, size=n)
+    h = 0.37
+    geom = pkg.geometry_from_arrays(shape, order, h, faces, invw, pot, device)
+    phi = rng.uniform(-1, 1, size=(2, n))
+    out = pkg.ops.hdiv_apply(torch.as_tensor(phi).to(device), geom).cpu().numpy()
+    box = phi.reshape(2, *shape)
+    expect = (weighted_divergence(box, faces, h, order).reshape(2, -1) * 0.5) * invw + phi * pot
+    assert np.abs(out - expect).max() / np.abs(expect).max() < 1e-13
+
+
+def test_apply_f32(pkg, device):
+    grid, ref = _case((21, 21, 21), 0.5, (1.0,), [[0.0, 0.0, 0.0]])
+    geom = pkg.geometry_from_arrays(grid.shape, 8, grid.spacing, ref.face, ref.inverse_weight, ref.potential, device,
+                                    dtype=torch.float32)
+    gen = torch.Generator().manual_seed(3)
+    phi = torch.rand((4, grid.n_points), generator=gen, dtype=torch.float64) * 2 - 1
+    out = pkg.ops.hdiv_apply(phi.to(device=device, dtype=torch.float32), geom).double().cpu().numpy()
+    expect = ref.apply(phi.numpy())
+    # float32 carries ~7 digits; the kinetic sum cancels about one of them at the cusp.
+    assert np.abs(out - expect).max() / np.abs(expect).max() < 2e-5
+
+
+def test_apply_is_bitwise_repeatable(pkg, device):
+    grid, ref = _case((19, 19, 19), 0.5, (2.0,), [[0.0, 0.0, 0.0]])
+    geom = pkg.geometry_from_arrays(grid.shape, 8, grid.spacing, ref.face, ref.inverse_weight, ref.potential, device)
+    phi = torch.randn((5, grid.n_points), dtype=torch.float64, generator=torch.Generator().manual_seed(1)).to(device)
+    a = pkg.ops.hdiv_apply(phi, geom)
+    b = pkg.ops.hdiv_apply(phi, geom)
+    assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 20])
+def test_filter_matches_oracle(pkg, device, degree):
+    grid, ref = _case((17, 17, 17), 0.5, (1.0,), [[0.0, 0.0, 0.0]])
+    geom = pkg.geometry_from_arrays(grid.shape, 8, grid.spacing, ref.face, ref.inverse_weight, ref.potential, device)
+    gen = torch.Generator().manual_seed(degree)
+    block = torch.rand((5, grid.n_points), generator=gen, dtype=torch.float64) * 2 - 1
+    lower, upper, smin = -0.3, 60.0, -0.55
+    out = pkg.chebyshev_filter_device(block.to(device), geom, degree, lower, upper, smin).cpu().numpy()
+    expect = chebyshev_filter(ref, block.numpy(), degree, lower, upper, smin)
+    assert np.abs(out - expect).max() / np.abs(expect).max() < 1e-12
+
+
+def test_invalid_arguments_raise(pkg, device):
+    grid, ref = _case((9, 9, 9), 0.5, (1.0,), [[0.0, 0.0, 0.0]])
+    geom = pkg.geometry_from_arrays(grid.shape, 8, grid.spacing, ref.face, ref.inverse_weight, ref.potential, device)
+    phi = torch.zeros((2, grid.n_points), dtype=torch.float64, device=device)
+    with pytest.raises(ValueError):
+        pkg.ops.hdiv_apply(phi[:, :-1], geom)
+    with pytest.raises(pkg._lib.HipLibraryError):
+        pkg._lib.call("cdft_hdiv_apply_f64", phi.data_ptr(), 0, 0, 0, 0, 0, phi.data_ptr(), 2, 9, 9, 9,
+                      0, 8, 0)

@@ -1,0 +1,159 @@
+"""The oracle pinned against the reference's own golden values, closed forms and brute force.
+
+Every value below is copied from the reference's data files (values only): ``tests/golden/values.json``,
+``docs/figures/convergence_ladders.json`` and ``reports/fingerprint_cpu_reference.json``. Each check
+is chosen so that a plausible slip (a wrong tap index, a dropped cell average, a transposed weight)
+moves it far outside its tolerance.
+"""
+
+from __future__ import annotations
+
+import json
+import math
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from oracle.core import (
+    CuspFactor, CuspHamiltonian, Grid, Measure, fd_coefficients, staggered_coefficients, weighted_divergence,
+)
+from oracle.eigen import chebyshev_coefficients, chebyshev_filter
+
+GOLDEN = json.loads((ROOT / "tests" / "golden" / "reference_values.json").read_text())
+
+
+def _small_operator(charges, spacing=0.5, length=6.0):
+    grid = Grid.from_box((length, length, length), spacing)
+    positions = [[0.0, 0.0, 0.0] for _ in charges]
+    if len(charges) == 2:
+        positions = [[0.0, 0.0, -1.0], [0.0, 0.0, 1.0]]
+    factor = CuspFactor(grid, charges, positions)
+    return grid, CuspHamiltonian(grid, factor)
+
+
+def test_staggered_weights_closed_forms():
+    assert staggered_coefficients(2) == (-1.0, 1.0)
+    assert tuple(round(c, 10) for c in staggered_coefficients(4)) == (0.0416666667, -1.125, 1.125, -0.0416666667)
+    assert fd_coefficients(2, 2) == (1.0, -2.0, 1.0)
+    assert fd_coefficients(1, 2) == (-0.5, 0.0, 0.5)
+    for order in (2, 4, 6, 8):
+        c = np.array(staggered_coefficients(order))
+        k = np.arange(-order // 2, order // 2) + 0.5
+        for m in range(order):
+            assert abs((c * k**m).sum() - (1.0 if m == 1 else 0.0)) < 1e-12
+
+
+@pytest.mark.parametrize("order", [2, 4, 6, 8])
+def test_transpose_is_exact_and_composite_is_spd(order):
+    """Assemble ``D^T w D`` column by column on a 1-D line: symmetric to round-off, PSD, and for
+    ``order = 2``, ``w = 1`` the 3-point Laplacian."""
+    size, h = 14, 0.37
+    rng = np.random.default_rng(11)
+    w = rng.uniform(0.5, 1.5, size=size + order)
+    ones = np.ones(size + order)
+    for weight in (w, ones):
+        mat = np.zeros((size, size))
+        for col in range(size):
+            e = np.zeros((size, 1, 1))
+            e[col] = 1.0
+            faces = [weight[:, None, None], np.zeros((size, 1 + order, 1)), np.zeros((size, 1, 1 + order))]
+            mat[:, col] = weighted_divergence(e, faces, h, order)[:, 0, 0]
+        assert np.abs(mat - mat.T).max() < 1e-12 * np.abs(mat).max()
+        assert np.linalg.eigvalsh(mat).min() > -1e-10 * np.abs(mat).max()
+    if order == 2:
+        lap = (2 * np.eye(size) - np.eye(size, k=1) - np.eye(size, k=-1)) / h**2
+        assert np.abs(mat - lap).max() < 1e-12 * np.abs(lap).max()
+
+
+@pytest.mark.parametrize("charge", [1.0, 2.0])
+def test_checkerboard_rayleigh_quotient_pins(charge):
+    grid, h = _small_operator((charge,))
+    alt = ((-1.0) ** grid.box_index().sum(-1))[None, :].astype(float)
+    alt = alt / h.measure.norm(alt)[0]
+    q = float(h.measure.cross(alt, h.apply(alt))[0, 0])
+    pin = GOLDEN["checkerboard.rayleigh.Z%g" % charge]
+    assert abs(q - pin) <= 1e-9 * abs(pin)
+
+
+@pytest.mark.parametrize("charges", [(1.0,), (2.0,), (1.0, 1.0)])
+def test_self_adjoint_in_weighted_measure(charges):
+    grid, h = _small_operator(charges)
+    rng = np.random.default_rng(7)
+    a, b = rng.standard_normal((2, 1, grid.n_points))
+    left = float(h.measure.cross(a, h.apply(b))[0, 0])
+    right = float(h.measure.cross(b, h.apply(a))[0, 0])
+    assert abs(left - right) / max(abs(left), abs(right)) < 1e-13
+
+
+@pytest.mark.parametrize("charge", [1.0, 2.0, 8.0])
+def test_single_nucleus_potential_is_constant(charge):
+    grid = Grid.from_box((6.0, 6.0, 6.0), 0.5)
+    factor = CuspFactor(grid, (charge,), [[0.0, 0.0, 0.0]])
+    assert np.abs(factor.transformed_potential + 0.5 * charge**2).max() < 1e-10 * charge**2
+
+
+def test_identity_factor_is_the_plain_laplacian():
+    """No nuclei: ``f = 1``, faces all one; the operator is ``-1/2`` times the composite second
+    difference, i.e. applied to a quadratic it returns the exact ``-1/2 * 6`` inside."""
+    grid = Grid.from_box((8.0, 8.0, 8.0), 0.5)
+    factor = CuspFactor(grid, (), np.zeros((0, 3)))
+    h = CuspHamiltonian(grid, factor)
+    pts = grid.points()
+    quad = (pts**2).sum(-1)[None, :]
+    out = h.kinetic_apply(quad).reshape(grid.shape)
+    interior = out[8:-8, 8:-8, 8:-8]
+    assert np.abs(interior - (-3.0)).max() < 1e-9  # kinetic part = -1/2 lap(r^2) = -3
+
+
+def test_chebyshev_recurrence_is_the_chebyshev_polynomial():
+    """On a diagonal operator the filter multiplies eigenvector i by ``T_d((l_i-c)/e) / T_d((s-c)/e)``."""
+    lam = np.linspace(-1.0, 5.0, 7)
+
+    class Diag:
+        def apply(self, x):
+            return x * lam
+
+    d, lower, upper, smin = 9, 0.5, 5.0, -1.0
+    x = np.ones((1, lam.size))
+    y = chebyshev_filter(Diag(), x, d, lower, upper, smin)
+    e, c = 0.5 * (upper - lower), 0.5 * (upper + lower)
+    t = np.polynomial.chebyshev.Chebyshev.basis(d)
+    expect = t((lam - c) / e) / t((smin - c) / e)
+    assert np.abs(y[0] - expect).max() < 1e-12 * np.abs(expect).max()
+    assert len(chebyshev_coefficients(d, lower, upper, smin)) == d
+
+
+def test_cusp_weight_closed_forms():
+    grid = Grid.from_box((16.0, 16.0, 16.0), 0.5)
+    one = CuspFactor(grid, (2.0,), [[0.0, 0.0, 0.0]])
+    assert one.weight_integral_exact() == pytest.approx(math.pi / 8.0)
+    two = CuspFactor(grid, (1.0, 1.0), [[0.0, 0.0, -1.0], [0.0, 0.0, 1.0]])
+    # The two-centre closed form tends to the united-atom value pi/8 as R -> 0 (checked numerically).
+    tiny = CuspFactor(grid, (1.0, 1.0), [[0.0, 0.0, -1e-4], [0.0, 0.0, 1e-4]])
+    assert tiny.weight_integral_exact() == pytest.approx(math.pi / 8.0, rel=1e-3)
+    assert 0.0 < two.weight_integral_exact() < math.pi / 8.0
+
+
+def test_bare_hydrogen_golden():
+    """Bare H, 33^3 at h = 1 (the derived single-centre grid), k = 5, degree 20, CheFSI from the
+    seeded normal start: eigenvalue, kinetic energy and iteration count of the reference."""
+    from oracle.physics import Numerics, Scenario, solve_noninteracting
+
+    r = solve_noninteracting(Scenario((1.0,), ((0.0, 0.0, 0.0),), 1.0), Numerics())
+    assert r["n_iterations"] == GOLDEN["fingerprint.h_atom.n_iterations"]
+    assert abs(r["eigenvalues"][0] - GOLDEN["eigenvalue.ground.h_atom"]) < 1e-9 * 0.5
+    assert abs(r["kinetic"] - GOLDEN["kinetic.h_atom"]) < 1e-9 * 0.5
+
+
+@pytest.mark.slow
+def test_helium_lda_ladder_rung_h05():
+    """He at VWN LDA on the 16-bohr box at h = 0.5 (33^3): the reference ladder's total energy and
+    SCF iteration count. This pins the quadrature, Poisson, LDA and SCF oracle together."""
+    from oracle.physics import Numerics, Scenario, solve_self_consistent
+
+    r = solve_self_consistent(Scenario((2.0,), ((0.0, 0.0, 0.0),), 2.0, "lda"),
+                              Numerics(box_lengths=(16.0, 16.0, 16.0), spacing=0.5))
+    pin = GOLDEN["ladder.he_lda.h0.5.total"]
+    assert abs(r["energies"]["total"] - pin) < 1e-11
+    assert r["scf_iterations"] == GOLDEN["ladder.he_lda.h0.5.scf_iterations"]
