@@ -29,6 +29,38 @@ SIGNATURES: dict[str, list] = {
     "cdft_cheb_step_f32": [P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
     "cdft_cheb_filter_f64": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
     "cdft_cheb_filter_f32": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
+    "cdft_wgram_f64": [P, I, P, I, P, L, D, I, P, L, P, P],
+    "cdft_rotate_f64": [P, I, I, I, P, P, L, P],
+    "cdft_lincomb_f64": [P, I, P, P, P, P, P, L, P],
+    "cdft_resnorm_f64": [P, P, P, P, I, L, P, L, P, P],
+    "cdft_chol_inv_f64": [P, I, P, P, P],
+    "cdft_eigh_f64": [P, I, P, P, P],
+    "cdft_orth_error_f64": [P, I, P, P],
+    "cdft_cusp_grid_fields_f64": [I, P, P, P, D, P, D, P, P, D, P, P, P, P, P],
+    "cdft_face_weights_f64": [I, P, P, P, D, P, I, P, P, P, P, P],
+    "cdft_node_fields_f64": [P, P, P, L, I, P, P, P, D, P, P, P, P, P, P],
+    "cdft_far_factor_f64": [P, D, P, I, P, P, P],
+    "cdft_node_stencils_f64": [P, L, P, D, P, P, P, P, P],
+    "cdft_interpolate_f64": [P, I, P, P, L, P, D, P, P, P],
+    "cdft_lift_f64": [P, I, L, P, P, P, P, P, D, P, P, P, P, P],
+    "cdft_gradient_f64": [P, I, P, D, P, P, I, P, P],
+    "cdft_hydrogenic_fields_f64": [P, L, P, D, P, I, P, P, P, P, P],
+    "cdft_lda_vwn_f64": [P, L, P, P, P],
+    "cdft_ewise_f64": [I, P, P, P, D, P, L, P],
+    "cdft_density_f64": [P, I, L, P, P, P, P, P],
+    "cdft_atom_combine_f64": [P, P, P, I, L, D, P, P],
+    "cdft_small_solve_f64": [P, P, P, I, P, P],
+    "cdft_reduce_f64": [I, P, P, P, L, D, I, P, L, P, I, P],
+    "cdft_poisson_create": [I, I, I, I, I, I, P],
+    "cdft_poisson_destroy": [P],
+    "cdft_poisson_kernel_f64": [P, D, D, P, P],
+    "cdft_poisson_solve_f64": [P, P, P, P, P, P, P, P],
+}
+
+#: Entry points that return a value rather than a status.
+VALUED = {
+    "cdft_reduction_workspace_doubles": (ctypes.c_long, [L]),
+    "cdft_reduce_workspace_doubles": (ctypes.c_long, [L]),
 }
 
 
@@ -58,12 +90,16 @@ def load() -> ctypes.CDLL:
             fn = getattr(lib, name)
             fn.restype = ctypes.c_int
             fn.argtypes = argtypes
+        for name, (restype, argtypes) in VALUED.items():
+            fn = getattr(lib, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
         _lib = lib
         return lib
 
 
 def exported_symbols() -> list[str]:
-    return ["cdft_last_error", *SIGNATURES]
+    return ["cdft_last_error", *SIGNATURES, *VALUED]
 
 
 def call(name: str, *args) -> None:

@@ -1,0 +1,402 @@
+// K3 / K4 and the small dense kernels of Rayleigh-Ritz and CheFSI, all float64 and deterministic.
+//
+//   wgram   : S_ij = scale * sum_p q_p a_ip b_jp   (block partials in a fixed order, second pass)
+//   rotate  : Y = R X  (R is kout x kin on the device, row-major; or R^T)
+//   lincomb : out = sum_t c_t x_t with c_t a host constant optionally times / over a device scalar
+//   resnorm : r_i = sqrt(max(sum_p q_p (Av_ip - e_i v_ip)^2, 0))
+//   chol_inv: T = L^{-1}, S = L L^T, flag on a non-positive pivot
+//   eigh    : cyclic Jacobi on a symmetrised k x k matrix, eigenvalues ascending
+//
+// Reductions never use floating-point atomics: every block writes its partial sums, and one block
+// sums the partials in block order, so a repeat is bitwise identical.
+#include "common.h"
+
+namespace cdft {
+
+constexpr int RB = 256;          // threads per reduction block
+constexpr int KMAX = 8;          // largest tile of a Gram product per launch
+constexpr int MAX_PARTIALS = 1024;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block-wide sum of `count` values per thread (count <= KMAX*KMAX); result valid in thread 0.
+template <int COUNT>
+__device__ __forceinline__ void block_sums(double (&acc)[COUNT], double* shared /* [4][COUNT] */) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < COUNT; ++i) {
+    double v = wave_sum(acc[i]);
+    if (lane == 0) shared[wave * COUNT + i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < COUNT; ++i) acc[i] = ((shared[i] + shared[COUNT + i]) + shared[2 * COUNT + i]) + shared[3 * COUNT + i];
+  }
+}
+
+__global__ __launch_bounds__(RB) void wgram_partial_kernel(const double* __restrict__ a, int ka,
+                                                           const double* __restrict__ b, int kb,
+                                                           const double* __restrict__ q, long n,
+                                                           double* __restrict__ partials) {
+  __shared__ double shared[4 * KMAX * KMAX];
+  double acc[KMAX * KMAX];
+#pragma unroll
+  for (int i = 0; i < KMAX * KMAX; ++i) acc[i] = 0.0;
+  const long stride = (long)gridDim.x * RB;
+  for (long p = (long)blockIdx.x * RB + threadIdx.x; p < n; p += stride) {
+    const double w = q ? q[p] : 1.0;
+    double bv[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) bv[j] = (j < kb) ? b[(long)j * n + p] : 0.0;
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i) {
+      if (i < ka) {
+        const double av = a[(long)i * n + p] * w;
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) acc[i * KMAX + j] += av * bv[j];
+      }
+    }
+  }
+  block_sums<KMAX * KMAX>(acc, shared);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i)
+#pragma unroll
+      for (int j = 0; j < KMAX; ++j)
+        if (i < ka && j < kb) partials[(long)blockIdx.x * KMAX * KMAX + i * KMAX + j] = acc[i * KMAX + j];
+  }
+}
+
+// One thread per output entry sums the partials in block order. post: 0 plain, 1 sqrt(max(x, 0)).
+__global__ void wgram_final_kernel(const double* __restrict__ partials, int nblocks, int ka, int kb,
+                                   double scale, int post, double* __restrict__ out, int ldo,
+                                   int row0, int col0) {
+  const int e = threadIdx.x;
+  if (e >= ka * kb) return;
+  const int i = e / kb, j = e % kb;
+  double s = 0.0;
+  for (int blk = 0; blk < nblocks; ++blk) s += partials[(long)blk * KMAX * KMAX + i * KMAX + j];
+  s *= scale;
+  if (post == 1) s = sqrt(s > 0.0 ? s : 0.0);
+  out[(row0 + i) * ldo + col0 + j] = s;
+}
+
+static int reduction_blocks(long n) {
+  long blocks = (n + 2047) / 2048;
+  if (blocks < 1) blocks = 1;
+  if (blocks > MAX_PARTIALS) blocks = MAX_PARTIALS;
+  return (int)blocks;
+}
+
+// ---- rotation ----
+
+constexpr int KROT = 16;  // largest subspace handled by the rotation and small dense kernels
+
+__global__ __launch_bounds__(256) void rotate_kernel(const double* __restrict__ r, int kin, int kout,
+                                                     int transpose, const double* __restrict__ x,
+                                                     double* __restrict__ y, long n) {
+  __shared__ double sr[KROT * KROT];
+  for (int e = threadIdx.x; e < kin * kout; e += blockDim.x) {
+    const int i = e / kin, j = e % kin;  // output i, input j
+    sr[i * KROT + j] = transpose ? r[j * kout + i] : r[i * kin + j];
+  }
+  __syncthreads();
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+    double xv[KROT];
+#pragma unroll
+    for (int j = 0; j < KROT; ++j) xv[j] = (j < kin) ? x[(long)j * n + p] : 0.0;
+    for (int i = 0; i < kout; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < KROT; ++j)
+        if (j < kin) s += sr[i * KROT + j] * xv[j];
+      y[(long)i * n + p] = s;
+    }
+  }
+}
+
+// ---- linear combination ----
+
+struct Terms {
+  const double* x[16];
+  double c[16];
+  int mode[16];  // 0: c; 1: c * dev[idx]; 2: c / dev[idx]
+  int idx[16];
+};
+
+__device__ __forceinline__ double term_coef(const Terms& t, int i, const double* dev) {
+  double c = t.c[i];
+  if (t.mode[i] == 1) c = c * dev[t.idx[i]];
+  if (t.mode[i] == 2) c = c / dev[t.idx[i]];
+  return c;
+}
+
+__global__ __launch_bounds__(256) void lincomb_kernel(double* __restrict__ out, int nterms, Terms t,
+                                                      const double* __restrict__ dev, long n) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+    double s = term_coef(t, 0, dev) * t.x[0][p];
+    for (int i = 1; i < nterms; ++i) s += term_coef(t, i, dev) * t.x[i][p];
+    out[p] = s;
+  }
+}
+
+// ---- residual norms ----
+
+__global__ __launch_bounds__(RB) void resnorm_partial_kernel(const double* __restrict__ av,
+                                                             const double* __restrict__ v,
+                                                             const double* __restrict__ evals,
+                                                             const double* __restrict__ q, int k,
+                                                             long n, double* __restrict__ partials) {
+  __shared__ double shared[4 * KMAX];
+  double acc[KMAX];
+  double e[KMAX];
+#pragma unroll
+  for (int i = 0; i < KMAX; ++i) {
+    acc[i] = 0.0;
+    e[i] = (i < k) ? evals[i] : 0.0;
+  }
+  const long stride = (long)gridDim.x * RB;
+  for (long p = (long)blockIdx.x * RB + threadIdx.x; p < n; p += stride) {
+    const double w = q ? q[p] : 1.0;
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i) {
+      if (i < k) {
+        const double r = av[(long)i * n + p] - e[i] * v[(long)i * n + p];
+        acc[i] += (r * r) * w;
+      }
+    }
+  }
+  block_sums<KMAX>(acc, shared);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i)
+      if (i < k) partials[(long)blockIdx.x * KMAX * KMAX + i * KMAX] = acc[i];
+  }
+}
+
+// ---- small dense kernels (one thread: k is the subspace size, <= KROT) ----
+
+__global__ void chol_inv_kernel(const double* __restrict__ s, int k, double* __restrict__ t,
+                                int* __restrict__ flag) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double l[KROT * KROT];
+  int ok = 1;
+  for (int i = 0; i < k * k; ++i) l[i] = 0.0;
+  for (int j = 0; j < k; ++j) {
+    double d = s[j * k + j];
+    for (int m = 0; m < j; ++m) d -= l[j * k + m] * l[j * k + m];
+    if (!(d > 0.0)) {
+      ok = 0;
+      break;
+    }
+    const double ljj = sqrt(d);
+    l[j * k + j] = ljj;
+    for (int i = j + 1; i < k; ++i) {
+      double v = s[i * k + j];
+      for (int m = 0; m < j; ++m) v -= l[i * k + m] * l[j * k + m];
+      l[i * k + j] = v / ljj;
+    }
+  }
+  // T = L^{-1}, forward substitution column by column of the identity.
+  for (int i = 0; i < k * k; ++i) t[i] = 0.0;
+  if (ok) {
+    for (int c = 0; c < k; ++c) {
+      for (int i = c; i < k; ++i) {
+        double v = (i == c) ? 1.0 : 0.0;
+        for (int m = c; m < i; ++m) v -= l[i * k + m] * t[m * k + c];
+        t[i * k + c] = v / l[i * k + i];
+      }
+    }
+  }
+  flag[0] = ok ? 0 : 1;
+}
+
+// Cyclic Jacobi on 0.5 (M + M^T). Writes eigenvalues ascending and C with eigenvectors in columns
+// (C[i*k + j] = component i of eigenvector j), each column's largest entry made positive.
+__global__ void eigh_kernel(const double* __restrict__ m_in, int k, double* __restrict__ evals,
+                            double* __restrict__ c_out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a[KROT * KROT], v[KROT * KROT];
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) {
+      a[i * k + j] = 0.5 * (m_in[i * k + j] + m_in[j * k + i]);
+      v[i * k + j] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < k; ++i) {
+      diag += a[i * k + i] * a[i * k + i];
+      for (int j = i + 1; j < k; ++j) off += a[i * k + j] * a[i * k + j];
+    }
+    if (off <= 1e-36 * diag || off == 0.0) break;
+    for (int p = 0; p < k - 1; ++p) {
+      for (int q = p + 1; q < k; ++q) {
+        const double apq = a[p * k + q];
+        if (apq == 0.0) continue;
+        const double app = a[p * k + p], aqq = a[q * k + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double cs = 1.0 / sqrt(tt * tt + 1.0), sn = tt * cs;
+        for (int r = 0; r < k; ++r) {
+          const double arp = a[r * k + p], arq = a[r * k + q];
+          a[r * k + p] = cs * arp - sn * arq;
+          a[r * k + q] = sn * arp + cs * arq;
+        }
+        for (int r = 0; r < k; ++r) {
+          const double apr = a[p * k + r], aqr = a[q * k + r];
+          a[p * k + r] = cs * apr - sn * aqr;
+          a[q * k + r] = sn * apr + cs * aqr;
+        }
+        a[p * k + q] = a[q * k + p] = 0.0;
+        for (int r = 0; r < k; ++r) {
+          const double vrp = v[r * k + p], vrq = v[r * k + q];
+          v[r * k + p] = cs * vrp - sn * vrq;
+          v[r * k + q] = sn * vrp + cs * vrq;
+        }
+      }
+    }
+  }
+  int order[KROT];
+  for (int i = 0; i < k; ++i) order[i] = i;
+  for (int i = 1; i < k; ++i) {  // insertion sort, stable
+    int o = order[i];
+    int j = i - 1;
+    while (j >= 0 && a[order[j] * k + order[j]] > a[o * k + o]) {
+      order[j + 1] = order[j];
+      --j;
+    }
+    order[j + 1] = o;
+  }
+  for (int jj = 0; jj < k; ++jj) {
+    const int j = order[jj];
+    evals[jj] = a[j * k + j];
+    int big = 0;
+    for (int i = 1; i < k; ++i)
+      if (fabs(v[i * k + j]) > fabs(v[big * k + j])) big = i;
+    const double sgn = v[big * k + j] < 0.0 ? -1.0 : 1.0;
+    for (int i = 0; i < k; ++i) c_out[i * k + jj] = sgn * v[i * k + j];
+  }
+}
+
+// max |M - I| for a k x k matrix (orthonormality error), written to out[0].
+__global__ void max_abs_minus_identity_kernel(const double* __restrict__ m, int k, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double worst = 0.0;
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) {
+      const double d = fabs(m[i * k + j] - (i == j ? 1.0 : 0.0));
+      worst = d > worst ? d : worst;
+    }
+  out[0] = worst;
+}
+
+}  // namespace cdft
+
+using namespace cdft;
+
+extern "C" int cdft_wgram_f64(const double* a, int ka, const double* b, int kb, const double* q,
+                              long n, double scale, int post, double* partials, long partials_len,
+                              double* out, void* stream) {
+  if (!a || !b || !out || !partials) return fail(CDFT_E_ARG, "wgram: null pointer argument");
+  if (ka < 1 || kb < 1 || n < 1) return fail(CDFT_E_ARG, "wgram: sizes must be positive");
+  if (post != 0 && post != 1) return fail(CDFT_E_ARG, "wgram: post must be 0 or 1");
+  const int blocks = reduction_blocks(n);
+  if (partials_len < (long)blocks * KMAX * KMAX)
+    return fail(CDFT_E_CAPACITY, "wgram: partials workspace holds %ld doubles, needs %ld", partials_len,
+                (long)blocks * KMAX * KMAX);
+  hipStream_t s = (hipStream_t)stream;
+  for (int i0 = 0; i0 < ka; i0 += KMAX) {
+    for (int j0 = 0; j0 < kb; j0 += KMAX) {
+      const int ti = ka - i0 < KMAX ? ka - i0 : KMAX;
+      const int tj = kb - j0 < KMAX ? kb - j0 : KMAX;
+      hipLaunchKernelGGL(wgram_partial_kernel, dim3(blocks), dim3(RB), 0, s, a + (long)i0 * n, ti,
+                         b + (long)j0 * n, tj, q, n, partials);
+      int rc = check_launch("wgram_partial");
+      if (rc) return rc;
+      hipLaunchKernelGGL(wgram_final_kernel, dim3(1), dim3(64), 0, s, partials, blocks, ti, tj, scale,
+                         post, out, kb, i0, j0);
+      rc = check_launch("wgram_final");
+      if (rc) return rc;
+    }
+  }
+  return CDFT_OK;
+}
+
+extern "C" long cdft_reduction_workspace_doubles(long n) {
+  return (long)reduction_blocks(n) * KMAX * KMAX;
+}
+
+extern "C" int cdft_rotate_f64(const double* r, int kin, int kout, int transpose, const double* x,
+                               double* y, long n, void* stream) {
+  if (!r || !x || !y) return fail(CDFT_E_ARG, "rotate: null pointer argument");
+  if (kin < 1 || kout < 1 || kin > KROT || kout > KROT || n < 1)
+    return fail(CDFT_E_ARG, "rotate: kin, kout must be in 1..%d (got %d, %d)", KROT, kin, kout);
+  if (x == y) return fail(CDFT_E_ARG, "rotate: y must not alias x");
+  long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(rotate_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, r, kin,
+                     kout, transpose, x, y, n);
+  return check_launch("rotate");
+}
+
+extern "C" int cdft_lincomb_f64(double* out, int nterms, const double* const* x, const double* c,
+                                const int* mode, const int* idx, const double* dev, long n,
+                                void* stream) {
+  if (!out || !x || !c || nterms < 1 || nterms > 16 || n < 1)
+    return fail(CDFT_E_ARG, "lincomb: bad arguments (nterms %d)", nterms);
+  Terms t;
+  for (int i = 0; i < nterms; ++i) {
+    if (!x[i]) return fail(CDFT_E_ARG, "lincomb: null term %d", i);
+    t.x[i] = x[i];
+    t.c[i] = c[i];
+    t.mode[i] = mode ? mode[i] : 0;
+    t.idx[i] = idx ? idx[i] : 0;
+    if (t.mode[i] != 0 && !dev) return fail(CDFT_E_ARG, "lincomb: device scalar requested without dev");
+  }
+  long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(lincomb_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, out,
+                     nterms, t, dev, n);
+  return check_launch("lincomb");
+}
+
+extern "C" int cdft_resnorm_f64(const double* av, const double* v, const double* evals,
+                                const double* q, int k, long n, double* partials, long partials_len,
+                                double* norms, void* stream) {
+  if (!av || !v || !evals || !norms || !partials) return fail(CDFT_E_ARG, "resnorm: null pointer argument");
+  if (k < 1 || k > KMAX || n < 1) return fail(CDFT_E_ARG, "resnorm: k must be in 1..%d", KMAX);
+  const int blocks = reduction_blocks(n);
+  if (partials_len < (long)blocks * KMAX * KMAX) return fail(CDFT_E_CAPACITY, "resnorm: partials too small");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(resnorm_partial_kernel, dim3(blocks), dim3(RB), 0, s, av, v, evals, q, k, n, partials);
+  int rc = check_launch("resnorm_partial");
+  if (rc) return rc;
+  // Reuse the Gram final pass: entries (i, 0) of each partial tile, one column.
+  hipLaunchKernelGGL(wgram_final_kernel, dim3(1), dim3(64), 0, s, partials, blocks, k, 1, 1.0, 1, norms, 1, 0, 0);
+  return check_launch("resnorm_final");
+}
+
+extern "C" int cdft_chol_inv_f64(const double* s, int k, double* t, int* flag, void* stream) {
+  if (!s || !t || !flag || k < 1 || k > KROT) return fail(CDFT_E_ARG, "chol_inv: bad arguments (k=%d)", k);
+  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, s, k, t, flag);
+  return check_launch("chol_inv");
+}
+
+extern "C" int cdft_eigh_f64(const double* m, int k, double* evals, double* c, void* stream) {
+  if (!m || !evals || !c || k < 1 || k > KROT) return fail(CDFT_E_ARG, "eigh: bad arguments (k=%d)", k);
+  hipLaunchKernelGGL(eigh_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, m, k, evals, c);
+  return check_launch("eigh");
+}
+
+extern "C" int cdft_orth_error_f64(const double* m, int k, double* out, void* stream) {
+  if (!m || !out || k < 1) return fail(CDFT_E_ARG, "orth_error: bad arguments");
+  hipLaunchKernelGGL(max_abs_minus_identity_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, m, k, out);
+  return check_launch("orth_error");
+}

@@ -9,8 +9,7 @@
 // output tiles (x slowest, z contiguous). Each thread owns one (y, z) column of 4 x-planes: the
 // x-faces come from a register pencil loaded straight from global memory (the x halo never enters
 // LDS), the y- and z-faces from an LDS plane tile with a +-(2P-1) halo. Weighted faces are computed
-// once per face and shared through LDS. The k vectors of a block are looped inside the kernel so a
-// tile's geometry stays resident in L1/L2 across the block.
+// once per face and shared through LDS.
 #include "common.h"
 
 namespace cdft {
@@ -23,6 +22,11 @@ struct Taps {
 };
 
 // Epilogue modes: 0 = A phi; 1 = (A y - c y) a; 2 = (A y - c y) a - b prev.
+//
+// One block = one 4x16x16 tile of one vector (blockIdx.y is the vector). Every global load of a
+// thread is independent and issued before the first barrier: its 4+2(2P-1) x-pencil, four y-halo
+// values (threads with ty < 2P-1... 14 rows x 16 columns x 4 planes) and four z-halo values, plus
+// the face weights. LDS holds the cross-shaped plane tiles (corners unused) and the weighted faces.
 template <typename T, int P, int MODE>
 __global__ __launch_bounds__(NT) void hdiv_kernel(
     const T* __restrict__ phi, const T* __restrict__ prev, T* __restrict__ out,
@@ -32,12 +36,14 @@ __global__ __launch_bounds__(NT) void hdiv_kernel(
     int tiles_x) {
   constexpr int R = 2 * P - 1;  // stencil reach: 2P-1 points each side
   constexpr int LY = TY + 2 * R, LZ = TZ + 2 * R;
+  static_assert(2 * R <= TY && 2 * R <= TZ, "halo rows must fit one pass of the block");
   __shared__ T sphi[TX][LY][LZ];
   __shared__ T syf[TX][TY + R][TZ];
   __shared__ T szf[TX][TY][TZ + R];
 
-  const unsigned nblocks = (unsigned)tiles_z * tiles_y * tiles_x;
-  const unsigned tile = xcd_tile(blockIdx.x, nblocks);
+  const unsigned ntiles = (unsigned)tiles_z * tiles_y * tiles_x;
+  const unsigned tile = xcd_tile(blockIdx.x, ntiles);
+  const int v = blockIdx.y;
   const int bz = tile % tiles_z;
   const int by = (tile / tiles_z) % tiles_y;
   const int bx = tile / (tiles_z * tiles_y);
@@ -49,91 +55,131 @@ __global__ __launch_bounds__(NT) void hdiv_kernel(
   const long plane = (long)n1 * n2;
   const long N = plane * n0;
   const int n1f = n1 + 2 * P, n2f = n2 + 2 * P;
+  const T* ph = phi + (long)v * N;
 
   T c[2 * P];
 #pragma unroll
   for (int i = 0; i < 2 * P; ++i) c[i] = (T)taps.c[i];
+
+  // ---- global loads (all independent) ----
+  T px[TX + 2 * R];
+#pragma unroll
+  for (int i = 0; i < TX + 2 * R; ++i) {
+    const int gx = X0 - R + i;
+    px[i] = (yz_in && gx >= 0 && gx < n0) ? ph[gx * plane + (long)y * n2 + z] : T(0);
+  }
+  // y halo: rows Y0-R .. Y0-1 (hy < R) and Y0+TY .. Y0+TY+R-1 (hy >= R), column z.
+  T hyv[TX];
+  const bool do_hy = ty < 2 * R;
+  const int hy_row = ty < R ? Y0 - R + ty : Y0 + TY + (ty - R);
+  const int hy_l = ty < R ? ty : TY + ty;  // local LDS row
+#pragma unroll
+  for (int p = 0; p < TX; ++p) {
+    const int gx = X0 + p;
+    hyv[p] = (do_hy && gx < n0 && z < n2 && hy_row >= 0 && hy_row < n1) ? ph[gx * plane + (long)hy_row * n2 + z] : T(0);
+  }
+  // z halo: columns Z0-R .. Z0-1 and Z0+TZ .. Z0+TZ+R-1, row y.
+  T hzv[TX];
+  const bool do_hz = tz < 2 * R;
+  const int hz_col = tz < R ? Z0 - R + tz : Z0 + TZ + (tz - R);
+  const int hz_l = tz < R ? tz : TZ + tz;
+#pragma unroll
+  for (int p = 0; p < TX; ++p) {
+    const int gx = X0 + p;
+    hzv[p] = (do_hz && gx < n0 && y < n1 && hz_col >= 0 && hz_col < n2) ? ph[gx * plane + (long)y * n2 + hz_col] : T(0);
+  }
+  // Face weights this thread applies: x faces of its pencil, y faces (rows fy = ty, ty+TY), z faces.
+  T wx[TX + R];
+#pragma unroll
+  for (int fx = 0; fx < TX + R; ++fx) {
+    const int a = X0 + fx;
+    wx[fx] = (yz_in && a < n0 + 2 * P) ? w0[((long)a * n1 + y) * n2 + z] : T(0);
+  }
+  T wy0[TX], wy1[TX], wz0[TX], wz1[TX];
+#pragma unroll
+  for (int p = 0; p < TX; ++p) {
+    const int gx = X0 + p;
+    const bool xin = gx < n0;
+    const int ay0 = Y0 + ty, ay1 = Y0 + ty + TY;
+    wy0[p] = (xin && z < n2 && ay0 < n1f) ? w1[((long)gx * n1f + ay0) * n2 + z] : T(0);
+    wy1[p] = (xin && z < n2 && ty < R && ay1 < n1f) ? w1[((long)gx * n1f + ay1) * n2 + z] : T(0);
+    const int az0 = Z0 + tz, az1 = Z0 + tz + TZ;
+    wz0[p] = (xin && y < n1 && az0 < n2f) ? w2[((long)gx * n1 + y) * n2f + az0] : T(0);
+    wz1[p] = (xin && y < n1 && tz < R && az1 < n2f) ? w2[((long)gx * n1 + y) * n2f + az1] : T(0);
+  }
+
+  // ---- stage the plane tiles ----
+#pragma unroll
+  for (int p = 0; p < TX; ++p) {
+    sphi[p][ty + R][tz + R] = px[R + p];
+    if (do_hy) sphi[p][hy_l][tz + R] = hyv[p];
+    if (do_hz) sphi[p][ty + R][hz_l] = hzv[p];
+  }
+  __syncthreads();
+
+  // ---- weighted faces ----
+#pragma unroll
+  for (int p = 0; p < TX; ++p) {
+    {
+      T f = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) f += c[i] * sphi[p][ty + i][tz + R];
+      syf[p][ty][tz] = f * wy0[p];
+    }
+    if (ty < R) {
+      T f = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) f += c[i] * sphi[p][ty + TY + i][tz + R];
+      syf[p][ty + TY][tz] = f * wy1[p];
+    }
+    {
+      T f = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) f += c[i] * sphi[p][ty + R][tz + i];
+      szf[p][ty][tz] = f * wz0[p];
+    }
+    if (tz < R) {
+      T f = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) f += c[i] * sphi[p][ty + R][tz + TZ + i];
+      szf[p][ty][tz + TZ] = f * wz1[p];
+    }
+  }
+  T gxf[TX + R];
+#pragma unroll
+  for (int fx = 0; fx < TX + R; ++fx) {
+    T f = 0;
+#pragma unroll
+    for (int i = 0; i < 2 * P; ++i) f += c[i] * px[fx + i];
+    gxf[fx] = f * wx[fx];
+  }
   T cc = 0, ca = 0, cb = 0;
   if (MODE >= 1) {
     cc = (T)table[3 * step];
     ca = (T)table[3 * step + 1];
     cb = (T)table[3 * step + 2];
   }
+  __syncthreads();
 
-  for (int v = 0; v < k; ++v) {
-    const T* ph = phi + (long)v * N;
-    __syncthreads();  // the previous vector is done with the LDS tiles
-    for (int idx = tid; idx < TX * LY * LZ; idx += NT) {
-      const int lz = idx % LZ;
-      const int t = idx / LZ;
-      const int ly = t % LY;
-      const int p = t / LY;
-      const int gx = X0 + p, gy = Y0 - R + ly, gz = Z0 - R + lz;
-      T val = 0;
-      if (gx < n0 && gy >= 0 && gy < n1 && gz >= 0 && gz < n2) val = ph[gx * plane + (long)gy * n2 + gz];
-      sphi[p][ly][lz] = val;
-    }
-    T px[TX + 2 * R];
+  if (!yz_in) return;
 #pragma unroll
-    for (int i = 0; i < TX + 2 * R; ++i) {
-      const int gx = X0 - R + i;
-      px[i] = (yz_in && gx >= 0 && gx < n0) ? ph[gx * plane + (long)y * n2 + z] : T(0);
-    }
-    __syncthreads();
-
+  for (int p = 0; p < TX; ++p) {
+    const int gx = X0 + p;
+    if (gx < n0) {
+      T ax = 0, ay = 0, az = 0;
 #pragma unroll
-    for (int p = 0; p < TX; ++p) {
-      const int gx = X0 + p;
-      for (int fy = ty; fy < TY + R; fy += TY) {
-        T f = 0;
+      for (int i = 0; i < 2 * P; ++i) ax += c[i] * gxf[p + R - i];
 #pragma unroll
-        for (int i = 0; i < 2 * P; ++i) f += c[i] * sphi[p][fy + i][tz + R];
-        const int a = Y0 + fy;  // face n = Y0 - P + fy sits at array index n + P
-        T w = 0;
-        if (gx < n0 && z < n2 && a < n1f) w = w1[((long)gx * n1f + a) * n2 + z];
-        syf[p][fy][tz] = f * w;
-      }
-      for (int fz = tz; fz < TZ + R; fz += TZ) {
-        T f = 0;
+      for (int i = 0; i < 2 * P; ++i) ay += c[i] * syf[p][ty + R - i][tz];
 #pragma unroll
-        for (int i = 0; i < 2 * P; ++i) f += c[i] * sphi[p][ty + R][fz + i];
-        const int a = Z0 + fz;
-        T w = 0;
-        if (gx < n0 && y < n1 && a < n2f) w = w2[((long)gx * n1 + y) * n2f + a];
-        szf[p][ty][fz] = f * w;
-      }
-    }
-    T gxf[TX + R];
-#pragma unroll
-    for (int fx = 0; fx < TX + R; ++fx) {
-      T f = 0;
-#pragma unroll
-      for (int i = 0; i < 2 * P; ++i) f += c[i] * px[fx + i];
-      const int a = X0 + fx;
-      gxf[fx] = (yz_in && a < n0 + 2 * P) ? f * w0[((long)a * n1 + y) * n2 + z] : T(0);
-    }
-    __syncthreads();
-
-    if (yz_in) {
-#pragma unroll
-      for (int p = 0; p < TX; ++p) {
-        const int gx = X0 + p;
-        if (gx >= n0) break;
-        T ax = 0, ay = 0, az = 0;
-#pragma unroll
-        for (int i = 0; i < 2 * P; ++i) ax += c[i] * gxf[p + R - i];
-#pragma unroll
-        for (int i = 0; i < 2 * P; ++i) ay += c[i] * syf[p][ty + R - i][tz];
-#pragma unroll
-        for (int i = 0; i < 2 * P; ++i) az += c[i] * szf[p][ty][tz + R - i];
-        const T lap = (ax + ay) + az;
-        const long idx = gx * plane + (long)y * n2 + z;
-        const T pv = px[R + p];
-        T val = lap * T(0.5) * invw[idx] + pv * pot[idx];
-        if (MODE == 1) val = (val - cc * pv) * ca;
-        if (MODE == 2) val = (val - cc * pv) * ca - cb * prev[(long)v * N + idx];
-        out[(long)v * N + idx] = val;
-      }
+      for (int i = 0; i < 2 * P; ++i) az += c[i] * szf[p][ty][tz + R - i];
+      const T lap = (ax + ay) + az;
+      const long idx = gx * plane + (long)y * n2 + z;
+      const T pv = px[R + p];
+      T val = lap * T(0.5) * invw[idx] + pv * pot[idx];
+      if (MODE == 1) val = (val - cc * pv) * ca;
+      if (MODE == 2) val = (val - cc * pv) * ca - cb * prev[(long)v * N + idx];
+      out[(long)v * N + idx] = val;
     }
   }
 }
@@ -159,7 +205,7 @@ static int launch_p(const T* phi, const T* prev, T* out, const T* w0, const T* w
   Taps<P> taps;
   for (int i = 0; i < 2 * P; ++i) taps.c[i] = coef[i];
   const Dims d = dims_for(n0, n1, n2);
-  hipLaunchKernelGGL((hdiv_kernel<T, P, MODE>), dim3(d.blocks), dim3(NT), 0, stream, phi, prev, out,
+  hipLaunchKernelGGL((hdiv_kernel<T, P, MODE>), dim3(d.blocks, k), dim3(NT), 0, stream, phi, prev, out,
                      w0, w1, w2, invw, pot, k, n0, n1, n2, taps, table, step, d.tiles_z, d.tiles_y,
                      d.tiles_x);
   return check_launch("hdiv_kernel");
@@ -185,7 +231,7 @@ static int validate(const void* phi, const void* out, const void* w0, const void
     return fail(CDFT_E_ARG, "hdiv: null pointer argument");
   if (k < 1 || n0 < 1 || n1 < 1 || n2 < 1)
     return fail(CDFT_E_ARG, "hdiv: sizes must be positive (k=%d, n=%d,%d,%d)", k, n0, n1, n2);
-  if ((long double)n0 * n1 * n2 * k > 9.0e18L) return fail(CDFT_E_ARG, "hdiv: block too large");
+  if ((long double)n0 * n1 * n2 * k > 9.0e18L || k > 65535) return fail(CDFT_E_ARG, "hdiv: block too large");
   if (order != 2 && order != 4 && order != 6 && order != 8)
     return fail(CDFT_E_ARG, "hdiv: unsupported stencil order %d (2, 4, 6 or 8)", order);
   if (phi == out) return fail(CDFT_E_ARG, "hdiv: out must not alias phi");

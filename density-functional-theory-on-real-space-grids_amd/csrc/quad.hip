@@ -1,0 +1,581 @@
+// Geometry and cusp-aware quadrature kernels (float64), built once per geometry and reused.
+//
+//   cusp_grid_fields : u, grad u, f^2 = exp(-2u) and W = -|grad u|^2/2 on the grid, with W and
+//                      grad u replaced by their 4^3 Gauss-Legendre cell averages within 2h of a
+//                      nucleus (reference src/cdft/operators/cusp.py:46-145)
+//   face_weights     : f^2 at the staggered faces of each axis (cusp.py:168-214), max u reported
+//   node_fields      : f^2, grad u, W and the partition weight P at the sphere nodes
+//                      (quadrature.py:226-303, 340-419)
+//   far_factor       : prod_a (1 - p_a) on the grid (quadrature.py:499-504)
+//   node_stencils    : base cell and the three 1-D degree-7 Lagrange weight rows of each node
+//                      (quadrature.py:56-93, 506-560)
+//   interpolate      : out[f][node] = sum_abc wx wy wz field[f][base + abc] -- matrix-free L
+//   lift             : out[f][g] = sum over nodes whose stencil covers g of x[f][node] * weight,
+//                      gathered per grid point through a cell-binned node list -- matrix-free L^T,
+//                      deterministic (no atomics)
+//   gradient         : order-2P central first derivatives with a zero halo (laplacian.py:111-144)
+#include "common.h"
+
+namespace cdft {
+
+constexpr int MAXAT = 8;
+
+struct Nuclei {
+  int n;
+  double z[MAXAT];
+  double px[MAXAT], py[MAXAT], pz[MAXAT];
+};
+
+struct GridDesc {
+  int n0, n1, n2;
+  double h, ox, oy, oz;
+};
+
+struct CellRule {  // the 4^3 Gauss-Legendre cell average
+  double off[4];   // node offsets (GL node / 2 * h)
+  double w[64];    // products wx*wy*wz in loop order (x outer, z inner)
+  double total;    // their sum, accumulated in loop order
+};
+
+struct Spheres {
+  int n;
+  double cx[MAXAT], cy[MAXAT], cz[MAXAT];
+  double radius[MAXAT], tcentre[MAXAT], twidth[MAXAT];
+  double cell_width;  // Becke erf width, already times sqrt(2)
+};
+
+__device__ __forceinline__ double dist3(double dx, double dy, double dz) {
+  return sqrt((dx * dx + dy * dy) + dz * dz);
+}
+
+__global__ void cusp_grid_fields_kernel(GridDesc g, Nuclei nu, double reach, CellRule rule,
+                                        double* __restrict__ u_out, double* __restrict__ gu,
+                                        double* __restrict__ weight, double* __restrict__ wpot) {
+  const long n = (long)g.n0 * g.n1 * g.n2;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i2 = (int)(p % g.n2);
+  const int i1 = (int)((p / g.n2) % g.n1);
+  const int i0 = (int)(p / ((long)g.n1 * g.n2));
+  const double x = g.ox + (double)i0 * g.h, y = g.oy + (double)i1 * g.h, z = g.oz + (double)i2 * g.h;
+  double u = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+  bool near = false;
+  for (int a = 0; a < nu.n; ++a) {
+    const double dx = x - nu.px[a], dy = y - nu.py[a], dz = z - nu.pz[a];
+    const double d = dist3(dx, dy, dz);
+    u += nu.z[a] * d;
+    if (d > 1.0e-12) {
+      const double safe = d > 1.0e-30 ? d : 1.0e-30;
+      gx += nu.z[a] * (dx / safe);
+      gy += nu.z[a] * (dy / safe);
+      gz += nu.z[a] * (dz / safe);
+    }
+    if (d <= reach) near = true;
+  }
+  double w = -0.5 * ((gx * gx + gy * gy) + gz * gz);
+  if (near && nu.n > 0) {
+    double ms = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+    int q = 0;
+    for (int a0 = 0; a0 < 4; ++a0)
+      for (int a1 = 0; a1 < 4; ++a1)
+        for (int a2 = 0; a2 < 4; ++a2, ++q) {
+          const double sx = x + rule.off[a0], sy = y + rule.off[a1], sz = z + rule.off[a2];
+          double hx = 0.0, hy = 0.0, hz = 0.0;
+          for (int a = 0; a < nu.n; ++a) {
+            const double dx = sx - nu.px[a], dy = sy - nu.py[a], dz = sz - nu.pz[a];
+            double d = dist3(dx, dy, dz);
+            d = d > 1.0e-300 ? d : 1.0e-300;
+            hx = hx + nu.z[a] * dx / d;
+            hy = hy + nu.z[a] * dy / d;
+            hz = hz + nu.z[a] * dz / d;
+          }
+          ms = ms + rule.w[q] * ((hx * hx + hy * hy) + hz * hz);
+          mx = mx + rule.w[q] * hx;
+          my = my + rule.w[q] * hy;
+          mz = mz + rule.w[q] * hz;
+        }
+    w = -0.5 * ms / rule.total;
+    gx = mx / rule.total;
+    gy = my / rule.total;
+    gz = mz / rule.total;
+  }
+  u_out[p] = u;
+  gu[p] = gx;
+  gu[n + p] = gy;
+  gu[2 * n + p] = gz;
+  weight[p] = exp(-2.0 * u);
+  wpot[p] = w;
+}
+
+__global__ void face_weights_kernel(GridDesc g, Nuclei nu, int axis, int order,
+                                    double* __restrict__ out, unsigned long long* __restrict__ umax) {
+  int s0 = g.n0, s1 = g.n1, s2 = g.n2;
+  if (axis == 0) s0 += order;
+  if (axis == 1) s1 += order;
+  if (axis == 2) s2 += order;
+  const long n = (long)s0 * s1 * s2;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i2 = (int)(p % s2);
+  const int i1 = (int)((p / s2) % s1);
+  const int i0 = (int)(p / ((long)s1 * s2));
+  const int half = order / 2;
+  double c0 = (double)i0, c1 = (double)i1, c2 = (double)i2;
+  if (axis == 0) c0 = c0 - half + 0.5;
+  if (axis == 1) c1 = c1 - half + 0.5;
+  if (axis == 2) c2 = c2 - half + 0.5;
+  const double x = g.ox + c0 * g.h, y = g.oy + c1 * g.h, z = g.oz + c2 * g.h;
+  double u = 0.0;
+  for (int a = 0; a < nu.n; ++a) {
+    const double dx = x - nu.px[a], dy = y - nu.py[a], dz = z - nu.pz[a];
+    u = u + nu.z[a] * sqrt((dx * dx + dy * dy) + dz * dz);
+  }
+  out[p] = exp(-2.0 * u);
+  atomicMax(umax, (unsigned long long)__double_as_longlong(u));  // u >= 0: integer order = float order
+}
+
+__device__ __forceinline__ double bump(const Spheres& s, int a, double x, double y, double z, double* dist) {
+  const double d = dist3(x - s.cx[a], y - s.cy[a], z - s.cz[a]);
+  *dist = d;
+  if (!(d < s.radius[a])) return 0.0;
+  return 0.5 * erfc((d - s.tcentre[a]) / s.twidth[a]);
+}
+
+__device__ double far_factor_at(const Spheres& s, double x, double y, double z) {
+  double far = 1.0, d;
+  for (int a = 0; a < s.n; ++a) far = far * (1.0 - bump(s, a, x, y, z, &d));
+  return far;
+}
+
+__global__ void far_factor_kernel(GridDesc g, Spheres s, double* __restrict__ far) {
+  const long n = (long)g.n0 * g.n1 * g.n2;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i2 = (int)(p % g.n2);
+  const int i1 = (int)((p / g.n2) % g.n1);
+  const int i0 = (int)(p / ((long)g.n1 * g.n2));
+  far[p] = far_factor_at(s, g.ox + (double)i0 * g.h, g.oy + (double)i1 * g.h, g.oz + (double)i2 * g.h);
+}
+
+// Partition P_a = Q (beta_a + sum_b beta_b [s_b >= R_b]) and the analytic fields at each node.
+__global__ void node_fields_kernel(const double* __restrict__ pts, const int* __restrict__ owner,
+                                   const double* __restrict__ base_w, long m, Nuclei nu, Spheres s,
+                                   double* __restrict__ weights, double* __restrict__ f2,
+                                   double* __restrict__ gu, double* __restrict__ wpot,
+                                   double* __restrict__ partition) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+  double near = 1.0 - far_factor_at(s, x, y, z);
+  double share;
+  const int own = owner[i];
+  if (s.n == 1) {
+    share = 1.0;
+  } else {
+    double dist[MAXAT], cells[MAXAT];
+    for (int a = 0; a < s.n; ++a) dist[a] = dist3(x - s.cx[a], y - s.cy[a], z - s.cz[a]);
+    double total = 0.0;
+    for (int a = 0; a < s.n; ++a) {
+      double c = 1.0;
+      for (int b = 0; b < s.n; ++b)
+        if (b != a) c = c * 0.5 * erfc((dist[a] - dist[b]) / s.cell_width);
+      cells[a] = c;
+    }
+    for (int a = 0; a < s.n; ++a) total += cells[a];
+    total = total > 1.0e-300 ? total : 1.0e-300;
+    for (int a = 0; a < s.n; ++a) cells[a] = cells[a] / total;
+    share = cells[own];
+    for (int b = 0; b < s.n; ++b) {
+      if (b == own) continue;
+      share = share + cells[b] * ((dist[b] >= s.radius[b]) ? 1.0 : 0.0);
+    }
+  }
+  const double part = share * near;
+  partition[i] = part;
+  weights[i] = base_w[i] * part;
+  double u = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+  for (int a = 0; a < nu.n; ++a) {
+    const double dx = x - nu.px[a], dy = y - nu.py[a], dz = z - nu.pz[a];
+    double d = dist3(dx, dy, dz);
+    d = d > 1.0e-300 ? d : 1.0e-300;
+    u = u + nu.z[a] * d;
+    gx = gx + nu.z[a] * (dx / d);
+    gy = gy + nu.z[a] * (dy / d);
+    gz = gz + nu.z[a] * (dz / d);
+  }
+  f2[i] = exp(-2.0 * u);
+  gu[i] = gx;
+  gu[m + i] = gy;
+  gu[2 * m + i] = gz;
+  wpot[i] = -0.5 * ((gx * gx + gy * gy) + gz * gz);
+}
+
+// Lagrange rows, degree 7 (8 points at offsets -3..4 around the base cell).
+__global__ void node_stencils_kernel(const double* __restrict__ pts, long m, GridDesc g,
+                                     int* __restrict__ base, double* __restrict__ lw,
+                                     int* __restrict__ bad) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double origin[3] = {g.ox, g.oy, g.oz};
+  const int dims[3] = {g.n0, g.n1, g.n2};
+  for (int ax = 0; ax < 3; ++ax) {
+    const double frac = (pts[3 * i + ax] - origin[ax]) / g.h;
+    const double b = floor(frac);
+    const double t = frac - b;
+    const int bi = (int)b;
+    base[3 * i + ax] = bi;
+    if (bi - 3 < 0 || bi + 4 >= dims[ax]) atomicOr(bad, 1);
+    double x[8];
+    for (int j = 0; j < 8; ++j) x[j] = t - (double)(j - 3);
+    for (int j = 0; j < 8; ++j) {
+      double w = 1.0;
+      for (int mm = 0; mm < 8; ++mm) {
+        if (mm == j) continue;
+        w = w * x[mm] / (double)(j - mm);
+      }
+      lw[(long)i * 24 + ax * 8 + j] = w;
+    }
+  }
+}
+
+// 8 lanes per node (lane a = x offset); nf <= 4 fields.
+__global__ __launch_bounds__(256) void interpolate_kernel(const double* __restrict__ field, int nf,
+                                                          long nfield, const int* __restrict__ base,
+                                                          const double* __restrict__ lw, long m,
+                                                          GridDesc g, double* __restrict__ out) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long node = gid >> 3;
+  const int a = (int)(gid & 7);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (node < m) {
+    const int b0 = base[3 * node] - 3 + a, b1 = base[3 * node + 1] - 3, b2 = base[3 * node + 2] - 3;
+    const double wx = lw[node * 24 + a];
+    const long plane = (long)g.n1 * g.n2;
+    for (int bb = 0; bb < 8; ++bb) {
+      const double wxy = wx * lw[node * 24 + 8 + bb];
+      const long row = (long)b0 * plane + (long)(b1 + bb) * g.n2 + b2;
+      for (int cc = 0; cc < 8; ++cc) {
+        const double w = wxy * lw[node * 24 + 16 + cc];
+        for (int f = 0; f < 4; ++f)
+          if (f < nf) acc[f] += w * field[f * nfield + row + cc];
+      }
+    }
+  }
+  for (int f = 0; f < 4; ++f) {
+    double v = acc[f];
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    acc[f] = v;
+  }
+  if (node < m && a == 0)
+    for (int f = 0; f < 4; ++f)
+      if (f < nf) out[f * m + node] = acc[f];
+}
+
+struct CellBox {
+  int lo0, lo1, lo2;  // smallest base cell
+  int c0, c1, c2;     // cell-box extents
+};
+
+__global__ __launch_bounds__(256) void lift_kernel(const double* __restrict__ x, int nf, long m,
+                                                   const int* __restrict__ cell_start,
+                                                   const int* __restrict__ cell_nodes,
+                                                   const int* __restrict__ base,
+                                                   const double* __restrict__ lw, GridDesc g,
+                                                   CellBox cb, const double* __restrict__ add,
+                                                   double* __restrict__ out) {
+  const long n = (long)g.n0 * g.n1 * g.n2;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i2 = (int)(p % g.n2);
+  const int i1 = (int)((p / g.n2) % g.n1);
+  const int i0 = (int)(p / ((long)g.n1 * g.n2));
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  // Cells whose stencil [b-3, b+4] covers this point: b in [i-4, i+3], clipped to the cell box.
+  const int l0 = max(i0 - 4, cb.lo0), h0 = min(i0 + 3, cb.lo0 + cb.c0 - 1);
+  const int l1 = max(i1 - 4, cb.lo1), h1 = min(i1 + 3, cb.lo1 + cb.c1 - 1);
+  const int l2 = max(i2 - 4, cb.lo2), h2 = min(i2 + 3, cb.lo2 + cb.c2 - 1);
+  for (int b0 = l0; b0 <= h0; ++b0)
+    for (int b1 = l1; b1 <= h1; ++b1)
+      for (int b2 = l2; b2 <= h2; ++b2) {
+        const int cell = ((b0 - cb.lo0) * cb.c1 + (b1 - cb.lo1)) * cb.c2 + (b2 - cb.lo2);
+        const int s = cell_start[cell], e = cell_start[cell + 1];
+        const int j0 = i0 - b0 + 3, j1 = i1 - b1 + 3, j2 = i2 - b2 + 3;
+        for (int t = s; t < e; ++t) {
+          const int node = cell_nodes[t];
+          const double w = (lw[(long)node * 24 + j0] * lw[(long)node * 24 + 8 + j1]) * lw[(long)node * 24 + 16 + j2];
+          for (int f = 0; f < 4; ++f)
+            if (f < nf) acc[f] += x[f * m + node] * w;
+        }
+      }
+  for (int f = 0; f < 4; ++f)
+    if (f < nf) out[f * n + p] = add ? acc[f] + add[f * n + p] : acc[f];
+}
+
+template <int P>
+__global__ void gradient_kernel(const double* __restrict__ field, int nf, GridDesc g,
+                                const double* __restrict__ coef /* 2P+1 central taps / h */,
+                                double* __restrict__ out) {
+  const long n = (long)g.n0 * g.n1 * g.n2;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i2 = (int)(p % g.n2);
+  const int i1 = (int)((p / g.n2) % g.n1);
+  const int i0 = (int)(p / ((long)g.n1 * g.n2));
+  const long plane = (long)g.n1 * g.n2;
+  const int idx[3] = {i0, i1, i2};
+  const int dims[3] = {g.n0, g.n1, g.n2};
+  const long stride[3] = {plane, (long)g.n2, 1};
+  for (int f = 0; f < nf; ++f) {
+    const double* src = field + (long)f * n;
+    for (int ax = 0; ax < 3; ++ax) {
+      double acc = 0.0;
+      bool first = true;
+      for (int j = 0; j < 2 * P + 1; ++j) {
+        const double c = coef[j];
+        if (c == 0.0) continue;
+        const int o = idx[ax] + j - P;
+        const double v = (o >= 0 && o < dims[ax]) ? src[p + (long)(j - P) * stride[ax]] : 0.0;
+        acc = first ? v * c : acc + v * c;
+        first = false;
+      }
+      out[((long)f * 3 + ax) * n + p] = acc;
+    }
+  }
+}
+
+// Unit cusp density exp(-2 Z s) and its closed-form potential (pi/Z^3)[1 - e^{-2Zs}(1 + Zs)]/s with
+// the small-s series below Zs = 1e-3 (reference quadrature.py:805-820, scf/step.py:128-165), per
+// nucleus, at the grid points (pts == NULL) or at a list of points.
+__global__ void hydrogenic_fields_kernel(const double* __restrict__ pts, long m, GridDesc g, Nuclei nu,
+                                         double* __restrict__ dens, double* __restrict__ v1s) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= m) return;
+  double x, y, z;
+  if (pts) {
+    x = pts[3 * p];
+    y = pts[3 * p + 1];
+    z = pts[3 * p + 2];
+  } else {
+    const int i2 = (int)(p % g.n2);
+    const int i1 = (int)((p / g.n2) % g.n1);
+    const int i0 = (int)(p / ((long)g.n1 * g.n2));
+    x = g.ox + (double)i0 * g.h;
+    y = g.oy + (double)i1 * g.h;
+    z = g.oz + (double)i2 * g.h;
+  }
+  for (int a = 0; a < nu.n; ++a) {
+    const double s = dist3(x - nu.px[a], y - nu.py[a], z - nu.pz[a]);
+    const double zz = nu.z[a];
+    if (dens) dens[(long)a * m + p] = exp(-2.0 * zz * s);
+    if (v1s) {
+      const double t = zz * s;
+      double v;
+      if (t < 1.0e-3) {
+        v = (M_PI / (zz * zz)) * (1.0 - (2.0 / 3.0) * (t * t) + (2.0 / 3.0) * (t * t * t) - 0.4 * (t * t * t * t) +
+                                  (8.0 / 45.0) * (t * t * t * t * t));
+      } else {
+        const double safe = s > 1.0e-300 ? s : 1.0e-300;
+        v = (M_PI / (zz * zz * zz)) * (1.0 - exp(-2.0 * t) * (1.0 + t)) / safe;
+      }
+      v1s[(long)a * m + p] = v;
+    }
+  }
+}
+
+static unsigned blocks_for(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+}  // namespace cdft
+
+using namespace cdft;
+
+static int check_nuclei(int n_atoms) {
+  if (n_atoms < 0 || n_atoms > MAXAT) return fail(CDFT_E_ARG, "at most %d nuclei (got %d)", MAXAT, n_atoms);
+  return CDFT_OK;
+}
+
+static Nuclei make_nuclei(int n_atoms, const double* charges, const double* positions) {
+  Nuclei nu;
+  nu.n = n_atoms;
+  for (int a = 0; a < n_atoms; ++a) {
+    nu.z[a] = charges[a];
+    nu.px[a] = positions[3 * a];
+    nu.py[a] = positions[3 * a + 1];
+    nu.pz[a] = positions[3 * a + 2];
+  }
+  return nu;
+}
+
+static GridDesc make_grid(const int* shape, double h, const double* origin) {
+  GridDesc g;
+  g.n0 = shape[0];
+  g.n1 = shape[1];
+  g.n2 = shape[2];
+  g.h = h;
+  g.ox = origin[0];
+  g.oy = origin[1];
+  g.oz = origin[2];
+  return g;
+}
+
+// charges, positions (n_atoms x 3), shape, origin, gl_offsets (4), gl_weights (64): host arrays.
+extern "C" int cdft_cusp_grid_fields_f64(int n_atoms, const double* charges, const double* positions,
+                                         const int* shape, double h, const double* origin,
+                                         double reach, const double* gl_offsets,
+                                         const double* gl_weights, double gl_total, double* u,
+                                         double* grad_u, double* weight, double* wpot, void* stream) {
+  int rc = check_nuclei(n_atoms);
+  if (rc) return rc;
+  if (!u || !grad_u || !weight || !wpot || !shape || !origin) return fail(CDFT_E_ARG, "cusp_grid_fields: null pointer");
+  GridDesc g = make_grid(shape, h, origin);
+  CellRule rule;
+  for (int i = 0; i < 4; ++i) rule.off[i] = gl_offsets[i];
+  for (int i = 0; i < 64; ++i) rule.w[i] = gl_weights[i];
+  rule.total = gl_total;
+  const long n = (long)g.n0 * g.n1 * g.n2;
+  hipLaunchKernelGGL(cusp_grid_fields_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, g,
+                     make_nuclei(n_atoms, charges, positions), reach, rule, u, grad_u, weight, wpot);
+  return check_launch("cusp_grid_fields");
+}
+
+// umax: device unsigned long long, zeroed by the caller; receives the bits of max u (>= 0).
+extern "C" int cdft_face_weights_f64(int n_atoms, const double* charges, const double* positions,
+                                     const int* shape, double h, const double* origin, int order,
+                                     double* w0, double* w1, double* w2, unsigned long long* umax,
+                                     void* stream) {
+  int rc = check_nuclei(n_atoms);
+  if (rc) return rc;
+  if (!w0 || !w1 || !w2 || !umax) return fail(CDFT_E_ARG, "face_weights: null pointer");
+  GridDesc g = make_grid(shape, h, origin);
+  Nuclei nu = make_nuclei(n_atoms, charges, positions);
+  double* outs[3] = {w0, w1, w2};
+  for (int axis = 0; axis < 3; ++axis) {
+    long n = (long)(g.n0 + (axis == 0 ? order : 0)) * (g.n1 + (axis == 1 ? order : 0)) * (g.n2 + (axis == 2 ? order : 0));
+    hipLaunchKernelGGL(face_weights_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, g, nu,
+                       axis, order, outs[axis], umax);
+    rc = check_launch("face_weights");
+    if (rc) return rc;
+  }
+  return CDFT_OK;
+}
+
+static Spheres make_spheres(int n, const double* sph, double cell_width) {
+  // sph: n rows of (cx, cy, cz, radius, taper_centre, taper_width)
+  Spheres s;
+  s.n = n;
+  for (int a = 0; a < n; ++a) {
+    s.cx[a] = sph[6 * a];
+    s.cy[a] = sph[6 * a + 1];
+    s.cz[a] = sph[6 * a + 2];
+    s.radius[a] = sph[6 * a + 3];
+    s.tcentre[a] = sph[6 * a + 4];
+    s.twidth[a] = sph[6 * a + 5];
+  }
+  s.cell_width = cell_width;
+  return s;
+}
+
+extern "C" int cdft_node_fields_f64(const double* pts, const int* owner, const double* base_w, long m,
+                                    int n_atoms, const double* charges, const double* positions,
+                                    const double* spheres, double cell_width, double* weights,
+                                    double* f2, double* grad_u, double* wpot, double* partition,
+                                    void* stream) {
+  int rc = check_nuclei(n_atoms);
+  if (rc) return rc;
+  if (!pts || !owner || !base_w || !weights || !f2 || !grad_u || !wpot || !partition || m < 1)
+    return fail(CDFT_E_ARG, "node_fields: bad arguments");
+  hipLaunchKernelGGL(node_fields_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, (hipStream_t)stream, pts, owner,
+                     base_w, m, make_nuclei(n_atoms, charges, positions), make_spheres(n_atoms, spheres, cell_width),
+                     weights, f2, grad_u, wpot, partition);
+  return check_launch("node_fields");
+}
+
+extern "C" int cdft_far_factor_f64(const int* shape, double h, const double* origin, int n_spheres,
+                                   const double* spheres, double* far, void* stream) {
+  int rc = check_nuclei(n_spheres);
+  if (rc) return rc;
+  if (!far) return fail(CDFT_E_ARG, "far_factor: null pointer");
+  GridDesc g = make_grid(shape, h, origin);
+  const long n = (long)g.n0 * g.n1 * g.n2;
+  hipLaunchKernelGGL(far_factor_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, g,
+                     make_spheres(n_spheres, spheres, 0.0), far);
+  return check_launch("far_factor");
+}
+
+extern "C" int cdft_node_stencils_f64(const double* pts, long m, const int* shape, double h,
+                                      const double* origin, int* base, double* lw, int* bad,
+                                      void* stream) {
+  if (!pts || !base || !lw || !bad || m < 1) return fail(CDFT_E_ARG, "node_stencils: bad arguments");
+  GridDesc g = make_grid(shape, h, origin);
+  hipLaunchKernelGGL(node_stencils_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, (hipStream_t)stream, pts, m, g,
+                     base, lw, bad);
+  return check_launch("node_stencils");
+}
+
+extern "C" int cdft_interpolate_f64(const double* field, int nf, const int* base, const double* lw,
+                                    long m, const int* shape, double h, const double* origin,
+                                    double* out, void* stream) {
+  if (!field || !base || !lw || !out || nf < 1 || nf > 4 || m < 1)
+    return fail(CDFT_E_ARG, "interpolate: bad arguments (nf=%d)", nf);
+  GridDesc g = make_grid(shape, h, origin);
+  const long n = (long)g.n0 * g.n1 * g.n2;
+  hipLaunchKernelGGL(interpolate_kernel, dim3(blocks_for(m * 8, 256)), dim3(256), 0, (hipStream_t)stream, field,
+                     nf, n, base, lw, m, g, out);
+  return check_launch("interpolate");
+}
+
+// cell_box (host): lo0, lo1, lo2, c0, c1, c2. add may be NULL.
+extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* cell_start,
+                             const int* cell_nodes, const int* base, const double* lw,
+                             const int* shape, double h, const double* origin, const int* cell_box,
+                             const double* add, double* out, void* stream) {
+  if (!x || !cell_start || !cell_nodes || !base || !lw || !out || !cell_box || nf < 1 || nf > 4)
+    return fail(CDFT_E_ARG, "lift: bad arguments (nf=%d)", nf);
+  GridDesc g = make_grid(shape, h, origin);
+  CellBox cb;
+  cb.lo0 = cell_box[0];
+  cb.lo1 = cell_box[1];
+  cb.lo2 = cell_box[2];
+  cb.c0 = cell_box[3];
+  cb.c1 = cell_box[4];
+  cb.c2 = cell_box[5];
+  const long n = (long)g.n0 * g.n1 * g.n2;
+  hipLaunchKernelGGL(lift_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, x, nf, m, cell_start,
+                     cell_nodes, base, lw, g, cb, add, out);
+  return check_launch("lift");
+}
+
+// coef: DEVICE array of the 2P+1 central first-derivative weights divided by h.
+extern "C" int cdft_gradient_f64(const double* field, int nf, const int* shape, double h,
+                                 const double* origin, const double* coef, int order, double* out,
+                                 void* stream) {
+  if (!field || !coef || !out || nf < 1) return fail(CDFT_E_ARG, "gradient: bad arguments");
+  GridDesc g = make_grid(shape, h, origin);
+  const long n = (long)g.n0 * g.n1 * g.n2;
+  const unsigned blocks = blocks_for(n, 256);
+  hipStream_t s = (hipStream_t)stream;
+  switch (order) {
+    case 2: hipLaunchKernelGGL(gradient_kernel<1>, dim3(blocks), dim3(256), 0, s, field, nf, g, coef, out); break;
+    case 4: hipLaunchKernelGGL(gradient_kernel<2>, dim3(blocks), dim3(256), 0, s, field, nf, g, coef, out); break;
+    case 6: hipLaunchKernelGGL(gradient_kernel<3>, dim3(blocks), dim3(256), 0, s, field, nf, g, coef, out); break;
+    case 8: hipLaunchKernelGGL(gradient_kernel<4>, dim3(blocks), dim3(256), 0, s, field, nf, g, coef, out); break;
+    default: return fail(CDFT_E_ARG, "gradient: unsupported order %d", order);
+  }
+  return check_launch("gradient");
+}
+
+// pts: device (m x 3) or NULL for the grid points (then m must be n0*n1*n2). dens/v1s: (na, m), either may be NULL.
+extern "C" int cdft_hydrogenic_fields_f64(const double* pts, long m, const int* shape, double h,
+                                          const double* origin, int n_atoms, const double* charges,
+                                          const double* positions, double* dens, double* v1s,
+                                          void* stream) {
+  int rc = check_nuclei(n_atoms);
+  if (rc) return rc;
+  if (m < 1 || (!dens && !v1s)) return fail(CDFT_E_ARG, "hydrogenic_fields: bad arguments");
+  GridDesc g = make_grid(shape, h, origin);
+  if (!pts && m != (long)g.n0 * g.n1 * g.n2) return fail(CDFT_E_ARG, "hydrogenic_fields: m must be the grid size");
+  hipLaunchKernelGGL(hydrogenic_fields_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, (hipStream_t)stream, pts, m, g,
+                     make_nuclei(n_atoms, charges, positions), dens, v1s);
+  return check_launch("hydrogenic_fields");
+}

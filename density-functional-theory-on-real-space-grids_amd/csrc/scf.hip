@@ -1,0 +1,259 @@
+// Pointwise and reduction kernels of the Kohn-Sham step (float64): VWN LDA, density, the Hartree
+// cusp split, the lumped-potential assembly, small solves and the SCF norms.
+//
+// LDA: Slater exchange + VWN5 correlation, spin-restricted, zeta = 0, with the density threshold
+// 1e-14 (reference src/cdft/xc/base.py:35,171-210, xc/lda.py:115-159,201-217). The potential is
+// the analytic derivative of the same energy density; at zeta = 0 the spin-interpolation terms and
+// their derivatives vanish identically.
+#include "common.h"
+
+namespace cdft {
+
+constexpr int RBS = 256;
+constexpr int SC_MAX_PARTIALS = 1024;
+
+__device__ __forceinline__ void lda_vwn(double n, double* e_out, double* v_out) {
+  const double half = 0.5 * n;
+  const double total = half + half;
+  if (total < 1.0e-14) {
+    *e_out = 0.0;
+    *v_out = 0.0;
+    return;
+  }
+  const double floor_ = 0.5 * 1.0e-14;
+  const double ns = half > floor_ ? half : floor_;
+  const double third = 1.0 / 3.0;
+  const double slater = 0.75 * pow(3.0 / M_PI, third);
+  const double factor = pow(2.0, third) * slater;
+  const double ns43 = pow(ns, 4.0 / 3.0);
+  const double ex = -factor * (ns43 + ns43);
+  const double vx = -factor * (4.0 / 3.0) * pow(ns, third);
+  const double nt = ns + ns;
+  const double rs = pow(3.0 / (4.0 * M_PI * nt), third);
+  const double x = sqrt(rs);
+  const double A = 0.0310907, x0 = -0.10498, b = 3.72744, c = 12.9352;
+  const double X = x * x + b * x + c;
+  const double X0 = x0 * x0 + b * x0 + c;
+  const double q = sqrt(4.0 * c - b * b);
+  const double at = atan(q / (2.0 * x + b));
+  const double para = A * (log(x * x / X) + (2.0 * b / q) * at -
+                           (b * x0 / X0) * (log((x - x0) * (x - x0) / X) + (2.0 * (b + 2.0 * x0) / q) * at));
+  const double dpara = A * (2.0 / x - (2.0 * x + 2.0 * b) / X -
+                            (b * x0 / X0) * (2.0 / (x - x0) - (2.0 * x + 2.0 * b + 2.0 * x0) / X));
+  *e_out = ex + nt * para;
+  *v_out = vx + (para - (x / 6.0) * dpara);
+}
+
+__global__ void lda_kernel(const double* __restrict__ n, long len, double* __restrict__ e,
+                           double* __restrict__ v) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= len) return;
+  double ee, vv;
+  lda_vwn(n[p], &ee, &vv);
+  e[p] = ee;
+  v[p] = vv;
+}
+
+// Elementwise expressions over up to three arrays and a host scalar.
+__global__ void ewise_kernel(int op, const double* __restrict__ a, const double* __restrict__ b,
+                             const double* __restrict__ c, double s, double* __restrict__ out, long len) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= len) return;
+  double r = 0.0;
+  switch (op) {
+    case 0: r = a[p] * b[p]; break;                 // a b
+    case 1: r = (a[p] * b[p]) * c[p]; break;        // (a b) c
+    case 2: r = s / a[p]; break;                    // s / a
+    case 3: r = a[p] / b[p]; break;                 // a / b
+    case 4: r = a[p] * s; break;                    // a s
+    case 5: r = (a[p] + b[p]) / c[p]; break;        // (a + b) / c
+    case 6: r = a[p] - b[p]; break;                 // a - b
+    case 7: r = (a[p] / s) * s; break;              // (a / s) s  (measure weight round trip)
+    case 8: r = s; break;                           // fill
+    case 9: r = a[p] + b[p]; break;                 // a + b
+    case 10: r = a[p] * ((b[p] * c[p] + b[len + p] * c[len + p]) + b[2 * len + p] * c[2 * len + p]); break;  // a (b . c)
+    default: r = 0.0;
+  }
+  out[p] = r;
+}
+
+// rho = sum_i occ_i phi_i^2 (occupied states only), n = rho f^2.
+__global__ void density_kernel(const double* __restrict__ phi, int k, long len, const double* __restrict__ f2,
+                               double o0, double o1, double o2, double o3, double* __restrict__ rho,
+                               double* __restrict__ n) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= len) return;
+  const double occ[4] = {o0, o1, o2, o3};
+  double r = 0.0;
+  for (int i = 0; i < k; ++i) {
+    const double v = phi[(long)i * len + p];
+    r = r + occ[i] * (v * v);
+  }
+  rho[p] = r;
+  if (n) n[p] = r * f2[p];
+}
+
+// out = base - sum_a c_a fields_a   (sign = -1) or base + sum_a c_a fields_a (sign = +1); the
+// coefficients c live on the device. Sum over atoms first, in atom order.
+__global__ void atom_combine_kernel(const double* __restrict__ base, const double* __restrict__ c,
+                                    const double* __restrict__ fields, int na, long len, double sign,
+                                    double* __restrict__ out) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= len) return;
+  double s = c[0] * fields[p];
+  for (int a = 1; a < na; ++a) s = s + c[a] * fields[(long)a * len + p];
+  out[p] = sign > 0.0 ? s + base[p] : base[p] - s;
+}
+
+// Dense solve A x = nucleus_f2 * rho_nuclei for the Hartree split coefficients (na <= 8).
+__global__ void small_solve_kernel(const double* __restrict__ a_in, const double* __restrict__ scale,
+                                   const double* __restrict__ b_in, int na, double* __restrict__ x) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a[64], b[8];
+  for (int i = 0; i < na * na; ++i) a[i] = a_in[i];
+  for (int i = 0; i < na; ++i) b[i] = scale[i] * b_in[i];
+  for (int col = 0; col < na; ++col) {  // partial pivoting
+    int piv = col;
+    for (int r = col + 1; r < na; ++r)
+      if (fabs(a[r * na + col]) > fabs(a[piv * na + col])) piv = r;
+    if (piv != col) {
+      for (int j = 0; j < na; ++j) {
+        const double t = a[col * na + j];
+        a[col * na + j] = a[piv * na + j];
+        a[piv * na + j] = t;
+      }
+      const double t = b[col];
+      b[col] = b[piv];
+      b[piv] = t;
+    }
+    for (int r = col + 1; r < na; ++r) {
+      const double f = a[r * na + col] / a[col * na + col];
+      for (int j = col; j < na; ++j) a[r * na + j] -= f * a[col * na + j];
+      b[r] -= f * b[col];
+    }
+  }
+  for (int i = na - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < na; ++j) s -= a[i * na + j] * x[j];
+    x[i] = s / a[i * na + i];
+  }
+}
+
+// Reductions with block partials (no atomics): mode 0: sum a*b*w (w, b optional),
+// mode 1: sum w*|a|, mode 2: max |a - b|.
+__global__ __launch_bounds__(RBS) void reduce_partial_kernel(int mode, const double* __restrict__ a,
+                                                             const double* __restrict__ b,
+                                                             const double* __restrict__ w, long len,
+                                                             double* __restrict__ partials) {
+  __shared__ double sh[RBS / 64];
+  double acc = 0.0;
+  const long stride = (long)gridDim.x * RBS;
+  for (long p = (long)blockIdx.x * RBS + threadIdx.x; p < len; p += stride) {
+    if (mode == 0) {
+      double t = a[p];
+      if (b) t = t * b[p];
+      if (w) t = t * w[p];
+      acc += t;
+    } else if (mode == 1) {
+      acc += w[p] * fabs(a[p]);
+    } else {
+      const double d = fabs(a[p] - b[p]);
+      acc = d > acc ? d : acc;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_down(acc, off, 64);
+    acc = (mode == 2) ? (o > acc ? o : acc) : acc + o;
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = sh[0];
+    for (int i = 1; i < RBS / 64; ++i) r = (mode == 2) ? (sh[i] > r ? sh[i] : r) : r + sh[i];
+    partials[blockIdx.x] = r;
+  }
+}
+
+// out[slot] = scale * (reduction of partials) (+ out[slot] if accumulate).
+__global__ void reduce_final_kernel(int mode, const double* __restrict__ partials, int nblocks,
+                                    double scale, int accumulate, double* __restrict__ out, int slot) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double r = partials[0];
+  for (int i = 1; i < nblocks; ++i) r = (mode == 2) ? (partials[i] > r ? partials[i] : r) : r + partials[i];
+  r = r * scale;
+  out[slot] = accumulate ? out[slot] + r : r;
+}
+
+static unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+
+static int red_blocks(long n) {
+  long b = (n + 2047) / 2048;
+  if (b < 1) b = 1;
+  if (b > SC_MAX_PARTIALS) b = SC_MAX_PARTIALS;
+  return (int)b;
+}
+
+}  // namespace cdft
+
+using namespace cdft;
+
+extern "C" int cdft_lda_vwn_f64(const double* n, long len, double* e, double* v, void* stream) {
+  if (!n || !e || !v || len < 1) return fail(CDFT_E_ARG, "lda_vwn: bad arguments");
+  hipLaunchKernelGGL(lda_kernel, dim3(nblk(len)), dim3(256), 0, (hipStream_t)stream, n, len, e, v);
+  return check_launch("lda_vwn");
+}
+
+extern "C" int cdft_ewise_f64(int op, const double* a, const double* b, const double* c, double s,
+                              double* out, long len, void* stream) {
+  if (!out || len < 1 || op < 0 || op > 10) return fail(CDFT_E_ARG, "ewise: bad arguments (op %d)", op);
+  const bool need_a = op != 8, need_b = (op == 0 || op == 1 || op == 3 || op == 5 || op == 6 || op == 9 || op == 10);
+  const bool need_c = (op == 1 || op == 5 || op == 10);
+  if ((need_a && !a) || (need_b && !b) || (need_c && !c)) return fail(CDFT_E_ARG, "ewise: missing operand (op %d)", op);
+  hipLaunchKernelGGL(ewise_kernel, dim3(nblk(len)), dim3(256), 0, (hipStream_t)stream, op, a, b, c, s, out, len);
+  return check_launch("ewise");
+}
+
+// occ: host array of k <= 4 occupations.
+extern "C" int cdft_density_f64(const double* phi, int k, long len, const double* f2, const double* occ,
+                                double* rho, double* n, void* stream) {
+  if (!phi || !occ || !rho || k < 1 || k > 4 || (n && !f2)) return fail(CDFT_E_ARG, "density: bad arguments");
+  double o[4] = {0, 0, 0, 0};
+  for (int i = 0; i < k; ++i) o[i] = occ[i];
+  hipLaunchKernelGGL(density_kernel, dim3(nblk(len)), dim3(256), 0, (hipStream_t)stream, phi, k, len, f2, o[0], o[1],
+                     o[2], o[3], rho, n);
+  return check_launch("density");
+}
+
+extern "C" int cdft_atom_combine_f64(const double* base, const double* c, const double* fields, int na,
+                                     long len, double sign, double* out, void* stream) {
+  if (!base || !c || !fields || !out || na < 1) return fail(CDFT_E_ARG, "atom_combine: bad arguments");
+  hipLaunchKernelGGL(atom_combine_kernel, dim3(nblk(len)), dim3(256), 0, (hipStream_t)stream, base, c, fields, na,
+                     len, sign, out);
+  return check_launch("atom_combine");
+}
+
+extern "C" int cdft_small_solve_f64(const double* a, const double* scale, const double* b, int na, double* x,
+                                    void* stream) {
+  if (!a || !scale || !b || !x || na < 1 || na > 8) return fail(CDFT_E_ARG, "small_solve: bad arguments");
+  hipLaunchKernelGGL(small_solve_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, a, scale, b, na, x);
+  return check_launch("small_solve");
+}
+
+extern "C" long cdft_reduce_workspace_doubles(long len) { return red_blocks(len); }
+
+// out[slot] = scale * reduce(mode, a, b, w) (+= if accumulate). mode 0: sum a b w; 1: sum w |a|;
+// 2: max |a - b|.
+extern "C" int cdft_reduce_f64(int mode, const double* a, const double* b, const double* w, long len,
+                               double scale, int accumulate, double* partials, long partials_len,
+                               double* out, int slot, void* stream) {
+  if (!a || !out || !partials || len < 1 || mode < 0 || mode > 2) return fail(CDFT_E_ARG, "reduce: bad arguments");
+  if ((mode == 1 && !w) || (mode == 2 && !b)) return fail(CDFT_E_ARG, "reduce: missing operand for mode %d", mode);
+  const int blocks = red_blocks(len);
+  if (partials_len < blocks) return fail(CDFT_E_CAPACITY, "reduce: partials workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(reduce_partial_kernel, dim3(blocks), dim3(RBS), 0, s, mode, a, b, w, len, partials);
+  int rc = check_launch("reduce_partial");
+  if (rc) return rc;
+  hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(64), 0, s, mode, partials, blocks, scale, accumulate, out, slot);
+  return check_launch("reduce_final");
+}

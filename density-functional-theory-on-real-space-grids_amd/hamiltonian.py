@@ -1,0 +1,151 @@
+"""The cusp-factorised Hamiltonian on the GPU, exposed only through its action on a block.
+
+Mirrors ``CuspFactoredHamiltonian`` (reference ``src/cdft/operators/hamiltonian.py:178-396``):
+``A phi = (1/2)(h^3/omega) sum_d D_d^T[f^2_face D_d phi] + (W~ + v~) phi``, self-adjoint in the
+measure ``q = omega``. ``apply`` is one K1 launch; the Lanczos bounds run on the device with all
+recurrence scalars kept in one device buffer and read once (reference ``_lanczos_bounds`` 29-89).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import ops
+from .ops import StencilGeometry, host_doubles
+from .stencil import staggered_coefficients
+
+
+def seeded_randn(shape, generator: torch.Generator | None, device) -> torch.Tensor:
+    """Standard-normal draw on the generator's own (CPU) device, then moved (reference
+    ``precision.py:246-263``), so the GPU solve starts from the reference's vectors."""
+    size = (shape,) if isinstance(shape, int) else tuple(shape)
+    draw = torch.randn(size, generator=generator, dtype=torch.float64)
+    return draw.to(device)
+
+
+class DeviceMeasure:
+    """``<a, b> = sum_p q_p a_p b_p`` on the device (reference ``eigen/measure.py:18-96``)."""
+
+    def __init__(self, grid, q: torch.Tensor):
+        self.grid = grid
+        self.q = q
+
+    def gram(self, block: torch.Tensor, out=None) -> torch.Tensor:
+        return ops.wgram(block, None, self.q, out=out)
+
+    def cross(self, left: torch.Tensor, right: torch.Tensor, out=None) -> torch.Tensor:
+        return ops.wgram(left, right, self.q, out=out)
+
+    def inner_into(self, a: torch.Tensor, b: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        return ops.wgram(a.reshape(1, -1), b.reshape(1, -1), self.q, out=out.view(1, 1))
+
+    def norm_into(self, a: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        return ops.wgram(a.reshape(1, -1), None, self.q, out=out.view(1, 1), sqrt=True)
+
+    def norms(self, block: torch.Tensor) -> torch.Tensor:
+        zeros = torch.zeros(block.shape[0], dtype=torch.float64, device=block.device)
+        return ops.resnorm(block, block, zeros, self.q)
+
+
+class CuspFactoredHamiltonian:
+    """``A = f^-1 H f`` in staggered divergence form with the cusp-aware measure."""
+
+    uses_divergence_form = True
+
+    def __init__(self, grid, factor, quadrature, extra_potential: torch.Tensor | None = None):
+        self.grid = grid
+        self.factor = factor
+        self.quadrature = quadrature
+        bare, invw, q = quadrature.bare_operator_terms()
+        self.bare_potential = bare
+        self.inverse_weight = invw
+        if extra_potential is None:
+            potential = bare
+        else:
+            potential = ops.ewise(ops.EW_ADD, grid.empty(), bare, extra_potential)
+        self.potential = potential
+        order = grid.fd_order
+        faces = factor.face_weights(order)
+        taps = host_doubles([c * (1.0 / grid.spacing) for c in staggered_coefficients(order)])
+        self.geometry = StencilGeometry(grid.shape, order, faces[0], faces[1], faces[2], invw, potential, taps)
+        self.measure = DeviceMeasure(grid, q)
+        self.apply_count = 0
+        self._kinetic_geometry = None
+
+    def apply(self, phi: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        self.apply_count += 1
+        return ops.hdiv_apply(phi, self.geometry, out)
+
+    def geometry_in(self, dtype: torch.dtype) -> StencilGeometry:
+        """The stencil geometry cast to ``dtype`` (cached): the float32 filter reads float32 face
+        weights, ``invw`` and ``V``, halving its geometry traffic."""
+        cache = self.__dict__.setdefault("_cast", {})
+        if dtype not in cache:
+            cache[dtype] = self.geometry.to(dtype)
+        return cache[dtype]
+
+    def note_applications(self, count: int) -> None:
+        self.apply_count += int(count)
+
+    def kinetic_apply(self, phi: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """``(1/2)(h^3/omega) sum_d D_d^T[f^2 D_d phi]`` alone (zero diagonal potential)."""
+        if self._kinetic_geometry is None:
+            self._kinetic_geometry = self.geometry.with_potential(torch.zeros_like(self.potential))
+        return ops.hdiv_apply(phi, self._kinetic_geometry, out)
+
+    def spectral_bounds(self, steps: int = 12, safety: float = 1.02, generator=None) -> tuple[float, float]:
+        """Rigorous Lanczos enclosure in the operator's measure, widened by ``safety``."""
+        g = self.grid
+        start = seeded_randn(g.n_points, generator, g.device)
+        return lanczos_bounds(self, start, steps, safety)
+
+
+def lanczos_bounds(hamiltonian, start: torch.Tensor, steps: int, safety: float) -> tuple[float, float]:
+    """Short Lanczos run with one reorthogonalisation pass against the previous two vectors; the
+    alphas and betas stay on the device and are read in one transfer (reference
+    ``operators/hamiltonian.py:29-89``)."""
+    m = hamiltonian.measure
+    dev = start.device
+    n = start.numel()
+    scal = torch.zeros(2 * steps + 3, dtype=torch.float64, device=dev)
+    a_slot = lambda i: i  # noqa: E731
+    b_slot = lambda i: steps + i  # noqa: E731
+    s1, s2, s0 = 2 * steps, 2 * steps + 1, 2 * steps + 2
+    m.norm_into(start, scal[s0:s0 + 1])
+    v = ops.lincomb(torch.empty(n, dtype=torch.float64, device=dev), [(start, 1.0, 2, s0)], scal)
+    v_prev = torch.zeros_like(v)
+    w = torch.empty_like(v)
+    for i in range(steps):
+        hamiltonian.apply(v.view(1, -1), w.view(1, -1))
+        m.inner_into(v, w, scal[a_slot(i):a_slot(i) + 1])
+        terms = [(w, 1.0), (v, -1.0, 1, a_slot(i))]
+        if i > 0:
+            terms.append((v_prev, -1.0, 1, b_slot(i - 1)))
+        ops.lincomb(w, terms, scal)
+        m.inner_into(w, v, scal[s1:s1 + 1])
+        m.inner_into(w, v_prev, scal[s2:s2 + 1])
+        ops.lincomb(w, [(w, 1.0), (v, -1.0, 1, s1), (v_prev, -1.0, 1, s2)], scal)
+        m.norm_into(w, scal[b_slot(i):b_slot(i) + 1])
+        v_new = ops.lincomb(torch.empty_like(v), [(w, 1.0, 2, b_slot(i))], scal)
+        v_prev, v = v, v_new
+    values = scal.cpu().numpy()
+    alphas = list(values[:steps])
+    betas = list(values[steps:2 * steps])
+    for index, value in enumerate(betas):
+        if value < 1e-12:
+            del alphas[index + 1:], betas[index + 1:]
+            break
+    size = len(alphas)
+    tri = np.zeros((size, size))
+    for i in range(size):
+        tri[i, i] = alphas[i]
+        if i + 1 < size:
+            tri[i, i + 1] = tri[i + 1, i] = betas[i]
+    ritz = np.linalg.eigvalsh(tri)  # a 12 x 12 tridiagonal on the host, as the reference does
+    residual = betas[-1] if betas else 0.0
+    lo = float(ritz[0]) - residual
+    hi = float(ritz[-1]) + residual
+    centre = 0.5 * (hi + lo)
+    half = 0.5 * (hi - lo) * safety
+    return centre - half, centre + half

@@ -140,3 +140,170 @@ def cheb_filter(x: torch.Tensor, geom: StencilGeometry, table: torch.Tensor, deg
               geom.potential.data_ptr(), k, n0, n1, n2, ctypes.addressof(geom.taps), geom.order, table.data_ptr(),
               int(degree), stream_of(x))
     return out
+
+
+# ------------------------------------------------------------------------------------------------
+# Reductions, rotations and small dense kernels (float64)
+# ------------------------------------------------------------------------------------------------
+
+_workspaces: dict[tuple, torch.Tensor] = {}
+
+
+def partials_for(n: int, device: torch.device) -> torch.Tensor:
+    """The block-partials workspace of the deterministic reductions for length ``n`` (cached per
+    device and size class; allocated once through torch)."""
+    need = int(_lib.load().cdft_reduction_workspace_doubles(int(n)))
+    key = ("partials", str(device))
+    ws = _workspaces.get(key)
+    if ws is None or ws.numel() < need:
+        ws = torch.empty(max(need, 64 * 64), dtype=torch.float64, device=device)
+        _workspaces[key] = ws
+    return ws
+
+
+def wgram(a: torch.Tensor, b: torch.Tensor | None = None, q: torch.Tensor | None = None, scale: float = 1.0,
+          out: torch.Tensor | None = None, sqrt: bool = False) -> torch.Tensor:
+    """``scale * sum_p q_p a_ip b_jp`` as a ``(ka, kb)`` float64 device matrix (K3)."""
+    a2 = a.reshape(-1, a.shape[-1])
+    b2 = a2 if b is None else b.reshape(-1, b.shape[-1])
+    n = a2.shape[1]
+    if b2.shape[1] != n or a2.dtype != torch.float64 or b2.dtype != torch.float64:
+        raise ValueError("wgram: blocks must be float64 with equal point counts")
+    if not (a2.is_contiguous() and b2.is_contiguous()):
+        raise ValueError("wgram: blocks must be contiguous")
+    if q is not None:
+        require(q, (n,), torch.float64, a2.device, "q")
+    ka, kb = a2.shape[0], b2.shape[0]
+    if out is None:
+        out = torch.empty((ka, kb), dtype=torch.float64, device=a2.device)
+    require(out, (ka, kb), torch.float64, a2.device, "out")
+    ws = partials_for(n, a2.device)
+    _lib.call("cdft_wgram_f64", a2.data_ptr(), ka, b2.data_ptr(), kb, _ptr(q), n, float(scale), 1 if sqrt else 0,
+              ws.data_ptr(), ws.numel(), out.data_ptr(), stream_of(a2))
+    return out
+
+
+def rotate(r: torch.Tensor, x: torch.Tensor, transpose: bool = False, out: torch.Tensor | None = None) -> torch.Tensor:
+    """``R x`` (or ``R^T x``) for a small float64 device matrix ``R`` and a block ``x`` (K4)."""
+    kin = x.shape[0]
+    kout = r.shape[1] if transpose else r.shape[0]
+    if (r.shape[0] if transpose else r.shape[1]) != kin:
+        raise ValueError(f"rotate: R {tuple(r.shape)} does not match a block of {kin} vectors")
+    n = x.shape[1]
+    if out is None:
+        out = torch.empty((kout, n), dtype=torch.float64, device=x.device)
+    require(out, (kout, n), torch.float64, x.device, "out")
+    require(x, (kin, n), torch.float64, x.device, "x")
+    rr = r.contiguous()
+    _lib.call("cdft_rotate_f64", rr.data_ptr(), kin, kout, 1 if transpose else 0, x.data_ptr(), out.data_ptr(), n,
+              stream_of(x))
+    return out
+
+
+def lincomb(out: torch.Tensor, terms: list, dev: torch.Tensor | None = None) -> torch.Tensor:
+    """``out = sum_t c_t x_t``; each term is ``(x, c)`` or ``(x, c, mode, idx)`` with mode 1 multiplying
+    and mode 2 dividing by the device scalar ``dev[idx]``. ``out`` may alias a term."""
+    n = out.numel()
+    m = len(terms)
+    ptrs = (ctypes.c_void_p * m)()
+    coefs = (ctypes.c_double * m)()
+    modes = (ctypes.c_int * m)()
+    idxs = (ctypes.c_int * m)()
+    for i, term in enumerate(terms):
+        x, c = term[0], term[1]
+        if x.numel() != n or x.dtype != torch.float64 or not x.is_contiguous():
+            raise ValueError("lincomb: every term must be a contiguous float64 array of the output's size")
+        ptrs[i] = x.data_ptr()
+        coefs[i] = float(c)
+        modes[i] = term[2] if len(term) > 2 else 0
+        idxs[i] = term[3] if len(term) > 3 else 0
+    _lib.call("cdft_lincomb_f64", out.data_ptr(), m, ctypes.addressof(ptrs), ctypes.addressof(coefs),
+              ctypes.addressof(modes), ctypes.addressof(idxs), _ptr(dev), n, stream_of(out))
+    return out
+
+
+def resnorm(av: torch.Tensor, v: torch.Tensor, evals: torch.Tensor, q: torch.Tensor | None,
+            out: torch.Tensor | None = None) -> torch.Tensor:
+    """``sqrt(sum q (Av - e v)^2)`` per vector."""
+    k, n = v.shape
+    out = torch.empty(k, dtype=torch.float64, device=v.device) if out is None else out
+    ws = partials_for(n, v.device)
+    _lib.call("cdft_resnorm_f64", av.data_ptr(), v.data_ptr(), evals.data_ptr(), _ptr(q), k, n, ws.data_ptr(),
+              ws.numel(), out.data_ptr(), stream_of(v))
+    return out
+
+
+def chol_inv(s: torch.Tensor, t: torch.Tensor, flag: torch.Tensor) -> None:
+    k = s.shape[0]
+    _lib.call("cdft_chol_inv_f64", s.data_ptr(), k, t.data_ptr(), flag.data_ptr(), stream_of(s))
+
+
+def eigh(m: torch.Tensor, evals: torch.Tensor, vecs: torch.Tensor) -> None:
+    k = m.shape[0]
+    _lib.call("cdft_eigh_f64", m.data_ptr(), k, evals.data_ptr(), vecs.data_ptr(), stream_of(m))
+
+
+def orth_error(m: torch.Tensor, out: torch.Tensor) -> None:
+    _lib.call("cdft_orth_error_f64", m.data_ptr(), m.shape[0], out.data_ptr(), stream_of(m))
+
+
+# ------------------------------------------------------------------------------------------------
+# Pointwise kernels and scalar reductions of the Kohn-Sham step (float64)
+# ------------------------------------------------------------------------------------------------
+
+EW_MUL, EW_MUL3, EW_SDIV, EW_DIV, EW_SCALE, EW_SUMDIV, EW_SUB, EW_ROUNDTRIP, EW_FILL, EW_ADD, EW_DOT3 = range(11)
+
+
+def ewise(op: int, out: torch.Tensor, a: torch.Tensor | None = None, b: torch.Tensor | None = None,
+          c: torch.Tensor | None = None, s: float = 0.0) -> torch.Tensor:
+    """Pointwise expression (op codes above); ``EW_DOT3`` is ``a * (b . c)`` with ``b, c`` of shape
+    ``(3, len)``."""
+    for i, t in enumerate((a, b, c)):
+        size = out.numel() * (3 if (op == EW_DOT3 and i > 0) else 1)
+        if t is not None and (t.numel() != size or t.dtype != torch.float64 or not t.is_contiguous()):
+            raise ValueError("ewise: operands must be contiguous float64 arrays of the output's size")
+    _lib.call("cdft_ewise_f64", op, _ptr(a), _ptr(b), _ptr(c), float(s), out.data_ptr(), out.numel(), stream_of(out))
+    return out
+
+
+def lda_vwn(n: torch.Tensor, e: torch.Tensor | None = None, v: torch.Tensor | None = None):
+    e = torch.empty_like(n) if e is None else e
+    v = torch.empty_like(n) if v is None else v
+    _lib.call("cdft_lda_vwn_f64", n.data_ptr(), n.numel(), e.data_ptr(), v.data_ptr(), stream_of(n))
+    return e, v
+
+
+def density(phi: torch.Tensor, occupations, f2: torch.Tensor | None, rho: torch.Tensor, n: torch.Tensor | None) -> None:
+    k = phi.shape[0]
+    occ = host_doubles(list(occupations)[:k])
+    _lib.call("cdft_density_f64", phi.data_ptr(), k, phi.shape[1], _ptr(f2), ctypes.addressof(occ), rho.data_ptr(),
+              _ptr(n), stream_of(phi))
+
+
+def atom_combine(base: torch.Tensor, c: torch.Tensor, fields: torch.Tensor, sign: float, out: torch.Tensor) -> torch.Tensor:
+    _lib.call("cdft_atom_combine_f64", base.data_ptr(), c.data_ptr(), fields.data_ptr(), fields.shape[0],
+              base.numel(), float(sign), out.data_ptr(), stream_of(out))
+    return out
+
+
+def small_solve(a: torch.Tensor, scale: torch.Tensor, b: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+    _lib.call("cdft_small_solve_f64", a.data_ptr(), scale.data_ptr(), b.data_ptr(), a.shape[0], x.data_ptr(),
+              stream_of(x))
+    return x
+
+
+RED_DOT, RED_L1, RED_MAXDIFF = 0, 1, 2
+
+
+def reduce(mode: int, a: torch.Tensor, out: torch.Tensor, slot: int = 0, b: torch.Tensor | None = None,
+           w: torch.Tensor | None = None, scale: float = 1.0, accumulate: bool = False) -> torch.Tensor:
+    """``out[slot] = scale * reduction`` (``+=`` with ``accumulate``): 0 = sum a b w, 1 = sum w |a|,
+    2 = max |a - b|. Deterministic two-pass, no atomics."""
+    n = a.numel()
+    need = int(_lib.load().cdft_reduce_workspace_doubles(n))
+    ws = partials_for(max(n, 1), a.device)
+    if ws.numel() < need:
+        raise RuntimeError("reduction workspace too small")
+    _lib.call("cdft_reduce_f64", mode, a.data_ptr(), _ptr(b), _ptr(w), n, float(scale), 1 if accumulate else 0,
+              ws.data_ptr(), ws.numel(), out.data_ptr(), int(slot), stream_of(a))
+    return out

@@ -1,0 +1,70 @@
+"""Open-boundary Hartree solver: the Coulomb-cutoff kernel on a zero-padded box, rocFFT on the GPU.
+
+Mirrors ``CoulombCutoffPoisson`` (reference ``src/cdft/operators/poisson.py:27-120``): padded shape
+``fft_friendly(ceil(pad * n))`` per axis, cutoff radius ``R = min(P_d h)/2``, kernel
+``K(G) = 4 pi (1 - cos G R)/G^2`` with ``K(0) = 2 pi R^2``. The plan and the Green's-function table
+are built once; a solve is pad-scatter, D2Z, multiply, Z2D, gather (K5).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import torch
+
+from . import _lib
+from .ops import stream_of
+
+
+def fft_friendly(n: int) -> int:
+    """Next integer whose only prime factors are 2, 3 and 5."""
+    c = n
+    while True:
+        r = c
+        for p in (2, 3, 5):
+            while r % p == 0:
+                r //= p
+        if r == 1:
+            return c
+        c += 1
+
+
+class CoulombCutoffPoisson:
+    def __init__(self, grid, pad_factor: float = 2.5):
+        if pad_factor < 2.0:
+            raise ValueError("pad_factor must be at least 2.0 for the convolution to be linear")
+        self.grid = grid
+        self.padded_shape = tuple(fft_friendly(int(math.ceil(pad_factor * n))) for n in grid.shape)
+        self.cutoff_radius = 0.5 * min(n * grid.spacing for n in self.padded_shape)
+        p0, p1, p2 = self.padded_shape
+        handle = ctypes.c_void_p()
+        n0, n1, n2 = grid.shape
+        _lib.call("cdft_poisson_create", n0, n1, n2, p0, p1, p2, ctypes.addressof(handle))
+        self._handle = handle
+        dev = grid.device
+        self.padded_in = torch.zeros((p0, p1, p2), dtype=torch.float64, device=dev)
+        self.padded_out = torch.empty((p0, p1, p2), dtype=torch.float64, device=dev)
+        self.spectrum = torch.empty((p0, p1, p2 // 2 + 1, 2), dtype=torch.float64, device=dev)
+        self.kernel = torch.empty((p0, p1, p2 // 2 + 1), dtype=torch.float64, device=dev)
+        _lib.call("cdft_poisson_kernel_f64", handle, grid.spacing, self.cutoff_radius, self.kernel.data_ptr(),
+                  stream_of(self.kernel))
+
+    def solve(self, density: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """``v_H`` on the grid for a total density ``(N,)``."""
+        if density.dim() != 1 or density.numel() != self.grid.n_points:
+            raise ValueError("expected a total density of shape (n_points,)")
+        out = torch.empty_like(density) if out is None else out
+        _lib.call("cdft_poisson_solve_f64", self._handle, density.data_ptr(), self.padded_in.data_ptr(),
+                  self.spectrum.data_ptr(), self.padded_out.data_ptr(), self.kernel.data_ptr(), out.data_ptr(),
+                  stream_of(density))
+        return out
+
+    def __del__(self):
+        handle = getattr(self, "_handle", None)
+        if handle is not None and handle.value:
+            try:
+                _lib.call("cdft_poisson_destroy", handle)
+            except Exception:  # noqa: BLE001 - interpreter shutdown
+                pass
+            self._handle = None

@@ -1,0 +1,489 @@
+"""The Kohn-Sham map and the SCF / non-interacting drivers, on the GPU.
+
+Mirrors ``KohnShamStep`` (reference ``src/cdft/scf/step.py:111-476``), ``solve_self_consistent``
+(``scf/loop.py:255-550`` with ``BoundReusePolicy`` 94-170 and ``initial_density`` 173-193),
+``PulayMixer`` (``scf/mixing.py:86-162``), ``aufbau_occupations`` (``scf/occupations.py:16-48``)
+and ``solve_noninteracting`` (``scf/noninteracting.py:118-330``). Every grid- or node-sized array
+lives on the device and is produced by a HIP kernel; the host keeps the control flow and reads the
+iteration's scalars once, as the reference does.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib, ops
+from .eigen import ChebyshevFilteredSubspace, EigenConfig, SpectralBoundsHint
+from .geometry import CuspFactor, CuspQuadrature, UniformGrid
+from .hamiltonian import CuspFactoredHamiltonian
+from .ops import stream_of
+from .poisson import CoulombCutoffPoisson
+
+FILTER_STEPS_PER_ITERATION = 4
+LANCZOS_REUSE_BACKSTOP = 8
+LANCZOS_REUSE_MAX_WIDENING = 0.05
+BOUND_REUSE_RITZ_FRACTION = 0.9
+
+
+def aufbau(eigenvalues, n_electrons: float, capacity: float = 2.0) -> np.ndarray:
+    """Fill from the bottom, fractional frontier allowed (reference ``scf/occupations.py:16-70``)."""
+    occ = np.zeros(len(eigenvalues))
+    remaining = float(n_electrons)
+    for i in range(len(occ)):
+        if remaining <= 1e-14:
+            break
+        take = min(capacity, remaining)
+        occ[i] = take
+        remaining -= take
+    if remaining > 1e-10:
+        raise ValueError(f"{n_electrons} electrons do not fit in {len(occ)} states of capacity {capacity}")
+    return occ
+
+
+def nuclear_repulsion(charges, positions) -> float:
+    pos = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
+    e = 0.0
+    for i in range(len(charges)):
+        for j in range(i + 1, len(charges)):
+            e += charges[i] * charges[j] / float(np.linalg.norm(pos[i] - pos[j]))
+    return e
+
+
+class KohnShamStep:
+    """One application of the Kohn-Sham map on the cusp-factorised path, spin-restricted LDA."""
+
+    def __init__(self, scenario, numerics, grid: UniformGrid, factor: CuspFactor, quadrature: CuspQuadrature):
+        self.scenario = scenario
+        self.numerics = numerics
+        self.grid = grid
+        self.factor = factor
+        self.q = quadrature
+        self.charges = tuple(float(z) for z in scenario.charges)
+        self.positions = np.asarray(scenario.positions, dtype=np.float64).reshape(-1, 3)
+        self.n_electrons = float(scenario.n_electrons)
+        self.n_states = max(1, int(-(-self.n_electrons // 2.0)))
+        self.ion_ion = nuclear_repulsion(self.charges, self.positions)
+        self.poisson = CoulombCutoffPoisson(grid, numerics.poisson_pad_factor)
+        self.bare = CuspFactoredHamiltonian(grid, factor, quadrature)
+        dev = grid.device
+        na = len(self.charges)
+        dist = [[float(np.linalg.norm(self.positions[a] - self.positions[b])) for b in range(na)] for a in range(na)]
+        overlap = [[math.exp(-2.0 * zb * dist[a][b]) for b, zb in enumerate(self.charges)] for a in range(na)]
+        self.overlap = torch.tensor(overlap, dtype=torch.float64, device=dev)
+        np.linalg.solve(np.array(overlap), np.ones(na))  # validated once, as the reference does
+        nucleus_f2 = [math.exp(-2.0 * sum(zb * dist[a][b] for b, zb in enumerate(self.charges) if b != a)) for a in range(na)]
+        self.nucleus_f2 = torch.tensor(nucleus_f2, dtype=torch.float64, device=dev)
+        n, m = grid.n_points, quadrature.n_nodes
+        self.cusp_grid = torch.empty((na, n), dtype=torch.float64, device=dev)
+        self.v1s_grid = torch.empty((na, n), dtype=torch.float64, device=dev)
+        self.v1s_node = torch.empty((na, m), dtype=torch.float64, device=dev)
+        s = stream_of(self.cusp_grid)
+        _lib.call("cdft_hydrogenic_fields_f64", None, n, grid.c_shape, grid.spacing, grid.c_origin, na,
+                  factor.c_charges, factor.c_positions, self.cusp_grid.data_ptr(), self.v1s_grid.data_ptr(), s)
+        _lib.call("cdft_hydrogenic_fields_f64", quadrature.points.data_ptr(), m, grid.c_shape, grid.spacing,
+                  grid.c_origin, na, factor.c_charges, factor.c_positions, None, self.v1s_node.data_ptr(), s)
+        self.f2 = factor.weight
+        self._alloc()
+
+    def _alloc(self):
+        g, m = self.grid, self.q.n_nodes
+        f = dict(dtype=torch.float64, device=g.device)
+        self.rho = torch.empty(g.n_points, **f)
+        self.n_smooth = torch.empty(g.n_points, **f)
+        self.v_smooth = torch.empty(g.n_points, **f)
+        self.v_h_grid = torch.empty(g.n_points, **f)
+        self.interp_in = torch.empty((2, g.n_points), **f)
+        self.interp = torch.empty((2, m), **f)
+        self.v_h_node = torch.empty(m, **f)
+        self.n_node = torch.empty(m, **f)
+        self.e_node = torch.empty(m, **f)
+        self.v_node = torch.empty(m, **f)
+        self.e_grid = torch.empty(g.n_points, **f)
+        self.v_grid = torch.empty(g.n_points, **f)
+        self.node_x = torch.empty((2, m), **f)
+        self.grid_add = torch.empty((2, g.n_points), **f)
+        self.tmp_node = torch.empty(m, **f)
+        self.tmp_grid = torch.empty(g.n_points, **f)
+        self.coefficients = torch.empty(len(self.charges), **f)
+        self.rho_nuclei = torch.empty(len(self.charges), **f)
+
+    # ---- potentials ----
+
+    def potentials(self, density: torch.Tensor) -> dict:
+        """Hartree (cusp split + FFT) and LDA lumped weights (reference ``scf/step.py:224-313``).
+
+        Returns device tensors; the three energies sit in ``scalars = [E_H, E_xc, int n v_xc]``.
+        """
+        q, g = self.q, self.grid
+        n_grid = density.reshape(-1, g.n_points)[0]
+        ops.ewise(ops.EW_DIV, self.rho, n_grid, self.f2)
+        self.rho_nuclei.copy_(q.interpolate_at(self.positions, self.rho))
+        ops.small_solve(self.overlap, self.nucleus_f2, self.rho_nuclei, self.coefficients)
+        ops.atom_combine(n_grid, self.coefficients, self.cusp_grid, -1.0, self.n_smooth)
+        self.poisson.solve(self.n_smooth, self.v_smooth)
+        ops.atom_combine(self.v_smooth, self.coefficients, self.v1s_grid, +1.0, self.v_h_grid)
+        self.interp_in[0].copy_(self.v_smooth)
+        self.interp_in[1].copy_(self.rho)
+        q.interpolate(self.interp_in, out=self.interp)
+        ops.atom_combine(self.interp[0], self.coefficients, self.v1s_node, +1.0, self.v_h_node)
+        ops.ewise(ops.EW_MUL, self.n_node, q.f2, self.interp[1])
+        ops.lda_vwn(n_grid, self.e_grid, self.v_grid)
+        ops.lda_vwn(self.n_node, self.e_node, self.v_node)
+        # node parts W P (f2 v), grid parts far_w (f2 v): Hartree in row 0, XC in row 1
+        ops.ewise(ops.EW_MUL, self.tmp_node, q.f2, self.v_h_node)
+        ops.ewise(ops.EW_MUL, self.node_x[0], q.weights, self.tmp_node)
+        ops.ewise(ops.EW_MUL, self.tmp_node, q.f2, self.v_node)
+        ops.ewise(ops.EW_MUL, self.node_x[1], q.weights, self.tmp_node)
+        ops.ewise(ops.EW_MUL, self.tmp_grid, self.f2, self.v_h_grid)
+        ops.ewise(ops.EW_MUL, self.grid_add[0], q.far_weights, self.tmp_grid)
+        ops.ewise(ops.EW_MUL, self.tmp_grid, self.f2, self.v_grid)
+        ops.ewise(ops.EW_MUL, self.grid_add[1], q.far_weights, self.tmp_grid)
+        lumped = q.lift(self.node_x, add=self.grid_add)
+        hartree_weights, xc_weights = lumped[0], lumped[1]
+        scalars = torch.empty(3, dtype=torch.float64, device=g.device)
+        ops.reduce(ops.RED_DOT, hartree_weights, scalars, 0, b=self.rho, scale=0.5)
+        ops.reduce(ops.RED_DOT, q.weights, scalars, 1, b=self.e_node)
+        ops.reduce(ops.RED_DOT, q.far_weights, scalars, 1, b=self.e_grid, accumulate=True)
+        ops.reduce(ops.RED_DOT, xc_weights, scalars, 2, b=self.rho)
+        effective = ops.ewise(ops.EW_SUMDIV, g.empty(), hartree_weights, xc_weights, q.mass_weights)
+        return {"effective": effective, "scalars": scalars, "v_hartree": self.v_h_grid.clone(),
+                "v_xc": self.v_grid.clone(), "coefficients": self.coefficients.clone()}
+
+    def hamiltonian(self, pot: dict) -> CuspFactoredHamiltonian:
+        return CuspFactoredHamiltonian(self.grid, self.factor, self.q, extra_potential=pot["effective"])
+
+    def diagonalise(self, solver: ChebyshevFilteredSubspace, h, initial=None, max_iterations=None, bounds=None):
+        gen = torch.Generator(device="cpu").manual_seed(self.numerics.seed)
+        saved = solver.config.max_iterations
+        if max_iterations is not None:
+            solver.config.max_iterations = max_iterations
+        try:
+            return solver.solve(h, self.n_states, initial=initial, generator=gen, bounds=bounds)
+        finally:
+            solver.config.max_iterations = saved
+
+    def output_density(self, eigen, occ) -> tuple[torch.Tensor, torch.Tensor]:
+        g = self.grid
+        rho = g.empty()
+        n = g.empty(1)
+        ops.density(eigen.eigenvectors, occ, self.f2, rho, n[0])
+        return n, rho
+
+    # ---- energies ----
+
+    def energy_scalars(self, phi: torch.Tensor, occ, rho_out: torch.Tensor, effective_in: torch.Tensor) -> torch.Tensor:
+        """Device buffer ``[kinetic form (k x k, flattened) | cross | potential | dc_out]`` (reference
+        ``operators/hamiltonian.py:344-377``, ``scf/step.py:398-439``)."""
+        g, q = self.grid, self.q
+        k = phi.shape[0]
+        kin = self.bare.kinetic_apply(phi)
+        out = torch.empty(k * k + 3, dtype=torch.float64, device=g.device)
+        self.bare.measure.cross(phi, kin, out=out[: k * k].view(k, k))
+        grad = g.gradient(rho_out)[0]  # (3, N)
+        stacked = torch.empty((4, g.n_points), dtype=torch.float64, device=g.device)
+        stacked[0].copy_(rho_out)
+        stacked[1:].copy_(grad)
+        nodes = q.interpolate(stacked)  # rho, d/dx, d/dy, d/dz at the nodes
+        m = q.n_nodes
+        cross_node = ops.ewise(ops.EW_DOT3, torch.empty(m, dtype=torch.float64, device=g.device), q.f2, nodes[1:], q.grad_u)
+        cross_grid = ops.ewise(ops.EW_DOT3, g.empty(), self.factor.weight, grad, self.factor.grad_u)
+        ops.reduce(ops.RED_DOT, q.weights, out, k * k, b=cross_node, scale=0.5)
+        ops.reduce(ops.RED_DOT, q.far_weights, out, k * k, b=cross_grid, scale=0.5, accumulate=True)
+        pot_node = ops.ewise(ops.EW_MUL3, torch.empty(m, dtype=torch.float64, device=g.device), q.f2, nodes[0],
+                             q.transformed_potential)
+        pot_grid = ops.ewise(ops.EW_MUL3, g.empty(), self.factor.weight, rho_out, self.factor.transformed_potential)
+        ops.reduce(ops.RED_DOT, q.weights, out, k * k + 1, b=pot_node)
+        ops.reduce(ops.RED_DOT, q.far_weights, out, k * k + 1, b=pot_grid, accumulate=True)
+        if effective_in is not None:
+            ops.reduce(ops.RED_DOT, effective_in, out, k * k + 2, b=q.mass_weights, w=rho_out)
+        return out
+
+
+def energies_from_host(k, occ, evals, host_scalars, e_in, e_out, ion_ion):
+    """Energy breakdown from host floats (reference ``scf/step.py:398-461``). ``e_in``/``e_out`` are
+    ``[E_H, E_xc, int n v_xc]`` of the input and output densities (``e_out`` None: Harris-Foulkes)."""
+    form = np.asarray(host_scalars[: k * k]).reshape(k, k)
+    first = float((occ[:k] * np.diag(form)).sum())
+    cross, potential, dc_out = host_scalars[k * k], host_scalars[k * k + 1], host_scalars[k * k + 2]
+    kinetic = first - cross - potential
+    band = float((occ * evals).sum())
+    kpe = band - dc_out
+    harris = band - e_in[0] - e_in[2] + e_in[1] + ion_ion
+    if e_out is None:
+        total, hartree, xc = harris, e_in[0], e_in[1]
+    else:
+        hartree, xc = e_out[0], e_out[1]
+        total = kpe + hartree + xc + ion_ion
+    external = total - kinetic - hartree - xc - ion_ion
+    return {"total": total, "kinetic": kinetic, "external": external, "hartree": hartree, "xc": xc,
+            "harris_foulkes": harris, "band": band, "external_direct": cross + 2.0 * potential,
+            "kinetic_plus_external": kpe, "ion_ion": ion_ion}
+
+
+class PulayMixer:
+    """Periodic Pulay/DIIS with Tikhonov regularisation (reference ``scf/mixing.py:86-162``); the
+    residual products are device reductions read in one transfer, the small system is solved on the
+    host, the combination is one device kernel."""
+
+    def __init__(self, alpha, history, period, inner_weights: torch.Tensor):
+        self.alpha, self.history, self.period = alpha, history, period
+        self.w = inner_weights
+        self.inputs, self.residuals = [], []
+        self.name = f"periodic_pulay(alpha={alpha:g}, history={history}, period={period})"
+
+    def mix(self, n_in: torch.Tensor, n_out: torch.Tensor, iteration: int) -> torch.Tensor:
+        residual = torch.empty_like(n_in)
+        ops.ewise(ops.EW_SUB, residual.view(-1), n_out.reshape(-1), n_in.reshape(-1))
+        self.inputs.append(n_in.clone())
+        self.residuals.append(residual)
+        if len(self.inputs) > self.history:
+            self.inputs.pop(0)
+            self.residuals.pop(0)
+        m = len(self.inputs)
+        out = torch.empty_like(n_in)
+        if m < 2 or iteration % self.period != 0:
+            ops.lincomb(out.view(-1), [(n_in.reshape(-1), 1.0), (residual.reshape(-1), self.alpha)])
+            return out
+        pairs = [(i, j) for i in range(m) for j in range(i, m)]
+        vals = torch.empty(len(pairs), dtype=torch.float64, device=n_in.device)
+        for slot, (i, j) in enumerate(pairs):
+            ops.reduce(ops.RED_DOT, self.residuals[i].reshape(-1), vals, slot, b=self.residuals[j].reshape(-1), w=self.w)
+        host = vals.cpu().numpy()
+        gram = np.zeros((m, m))
+        for (i, j), v in zip(pairs, host):
+            gram[i, j] = gram[j, i] = v
+        scale = float(gram.diagonal().max()) or 1.0
+        gram = gram + 1.0e-12 * scale * np.eye(m)
+        system = np.zeros((m + 1, m + 1))
+        system[:m, :m] = gram
+        system[:m, m] = -1.0
+        system[m, :m] = 1.0
+        rhs = np.zeros(m + 1)
+        rhs[m] = 1.0
+        coeffs = np.linalg.solve(system, rhs)[:m]
+        terms = []
+        for c, si, sr in zip(coeffs, self.inputs, self.residuals):
+            terms.append((si.reshape(-1), float(c)))
+            terms.append((sr.reshape(-1), float(c) * self.alpha))
+        ops.lincomb(out.view(-1), terms)
+        return out
+
+
+class LinearMixer:
+    def __init__(self, alpha):
+        self.alpha = alpha
+        self.name = f"linear(alpha={alpha:g})"
+
+    def mix(self, n_in, n_out, iteration):
+        residual = torch.empty_like(n_in)
+        ops.ewise(ops.EW_SUB, residual.view(-1), n_out.reshape(-1), n_in.reshape(-1))
+        out = torch.empty_like(n_in)
+        ops.lincomb(out.view(-1), [(n_in.reshape(-1), 1.0), (residual.view(-1), self.alpha)])
+        return out
+
+
+class BoundReusePolicy:
+    """When an SCF iteration may filter with the previous iteration's bounds (reference
+    ``scf/loop.py:94-170``)."""
+
+    def __init__(self):
+        self.bounds = None
+        self.since = 0
+        self.widening = 0.0
+        self.top_fraction = float("nan")
+        self.history: list[str] = []
+
+    def decide(self, residual_rose: bool, rung_changed: bool):
+        if self.bounds is None:
+            return "no carried interval"
+        if rung_changed:
+            return "fallback rung taken"
+        if residual_rose:
+            return "SCF residual rose"
+        if self.since >= LANCZOS_REUSE_BACKSTOP:
+            return f"backstop ({LANCZOS_REUSE_BACKSTOP} iterations)"
+        lo, hi = self.bounds
+        if self.widening > LANCZOS_REUSE_MAX_WIDENING * (hi - lo):
+            return "accumulated shift"
+        if not self.top_fraction < BOUND_REUSE_RITZ_FRACTION:
+            return "top Ritz fraction near the upper bound"
+        return None
+
+    def observe(self, reason, info):
+        self.bounds = tuple(info["spectral_bounds"])
+        self.top_fraction = float(info["top_ritz_fraction"])
+        if info["bounds_source"] == "reused":
+            self.since += 1
+            self.widening += float(info["bounds_shift"])
+            self.history.append("reused")
+        else:
+            self.since = 0
+            self.widening = 0.0
+            self.history.append(f"lanczos: {reason}")
+
+
+def build_geometry(scenario, numerics, device, anchor=None):
+    grid = UniformGrid.from_box(numerics.box_lengths, numerics.spacing, device, anchor=anchor, fd_order=numerics.fd_order)
+    factor = CuspFactor(grid, scenario.charges, scenario.positions)
+    quadrature = CuspQuadrature(grid, factor)
+    return grid, factor, quadrature
+
+
+def initial_density(step: KohnShamStep) -> torch.Tensor:
+    """Hydrogenic superposition normalised in the cusp-aware measure (reference ``scf/loop.py:173-193``)."""
+    g = step.grid
+    coefs = torch.tensor([z**3 / math.pi for z in step.charges], dtype=torch.float64, device=g.device)
+    total = ops.atom_combine(torch.zeros(g.n_points, dtype=torch.float64, device=g.device), coefs, step.cusp_grid, +1.0,
+                             g.empty())
+    rho = ops.ewise(ops.EW_DIV, g.empty(), total, step.f2)
+    charge = torch.empty(1, dtype=torch.float64, device=g.device)
+    ops.reduce(ops.RED_DOT, step.q.mass_weights, charge, 0, b=rho)
+    scale = step.n_electrons / float(charge.item())  # setup: one read
+    return ops.ewise(ops.EW_SCALE, g.empty(), total, s=scale).reshape(1, -1)
+
+
+def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None, filter_dtype=torch.float64,
+                          refine_steps: int = 0):
+    """The SCF driver (reference ``scf/loop.py:255-460``). ``filter_dtype=float32`` runs the Chebyshev
+    filter applies in float32 (everything else float64) and ``refine_steps`` float64 filter passes
+    close the final diagonalisation."""
+    started = time.perf_counter()
+    grid, factor, q = geometry if geometry is not None else build_geometry(scenario, numerics, device, anchor)
+    step = KohnShamStep(scenario, numerics, grid, factor, q)
+    g = grid
+    inner_weights = ops.ewise(ops.EW_DIV, g.empty(), q.mass_weights, ops.ewise(ops.EW_MUL, g.empty(), step.f2, step.f2))
+    solver = ChebyshevFilteredSubspace(grid, numerics.eigen, ritz_stop=False, filter_dtype=filter_dtype)
+    n_in = initial_density(step)
+    mixer = PulayMixer(numerics.alpha, numerics.history, numerics.pulay_period, inner_weights)
+    ladder = [
+        lambda: PulayMixer(numerics.alpha, numerics.history, numerics.pulay_period, inner_weights),
+        lambda: PulayMixer(0.5 * numerics.alpha, numerics.history, numerics.pulay_period, inner_weights),
+        lambda: LinearMixer(0.5 * numerics.alpha),
+    ]
+    rung = 0
+    energies, residual_history, evals_history, events = [], [], [], []
+    worsened = 0
+    previous_residual = float("inf")
+    initial_block = None
+    policy = BoundReusePolicy()
+    previous_effective = None
+    rung_changed = False
+    converged = False
+    applications = []
+    iteration = 0
+    norm_buf = torch.empty(3, dtype=torch.float64, device=g.device)
+    tmp = g.empty()
+    for iteration in range(1, numerics.max_scf + 1):
+        pin = step.potentials(n_in)
+        h = step.hamiltonian(pin)
+        steps = None if initial_block is None else FILTER_STEPS_PER_ITERATION
+        reason = policy.decide(worsened > 0, rung_changed)
+        hint = None
+        if reason is None and previous_effective is not None:
+            ops.reduce(ops.RED_MAXDIFF, pin["effective"], norm_buf, 2, b=previous_effective)
+            hint = SpectralBoundsHint(policy.bounds[0], policy.bounds[1], float(norm_buf[2].item()))
+        elif reason is None:
+            reason = "no previous potential"
+        eigen = step.diagonalise(solver, h, initial=initial_block, max_iterations=steps, bounds=hint)
+        policy.observe(reason, eigen.info)
+        previous_effective = pin["effective"]
+        rung_changed = False
+        applications.append(h.apply_count)
+        initial_block = eigen.eigenvectors
+        occ = aufbau(eigen.eigenvalues, step.n_electrons)
+        n_out, rho_out = step.output_density(eigen, occ)
+        pout = step.potentials(n_out)
+        escal = step.energy_scalars(eigen.eigenvectors, occ, rho_out, pin["effective"])
+        residual = ops.ewise(ops.EW_SUB, g.empty(), n_out[0], n_in[0])
+        ops.ewise(ops.EW_DIV, tmp, residual, step.f2)
+        ops.reduce(ops.RED_L1, tmp, norm_buf, 0, w=q.mass_weights)
+        n_next = mixer.mix(n_in, n_out, iteration)
+        ops.ewise(ops.EW_SUB, tmp, n_next[0], n_in[0])
+        ops.ewise(ops.EW_DIV, tmp, tmp, step.f2)
+        ops.reduce(ops.RED_L1, tmp, norm_buf, 1, w=q.mass_weights)
+        host = torch.cat([escal, pin["scalars"], pout["scalars"], norm_buf[:2]]).cpu().numpy()  # one read
+        k = eigen.eigenvectors.shape[0]
+        ne = k * k + 3
+        e_in, e_out = host[ne:ne + 3], host[ne + 3:ne + 6]
+        terms = energies_from_host(k, occ, eigen.eigenvalues, host[:ne], e_in, e_out, step.ion_ion)
+        res_norm, change = float(host[ne + 6]), float(host[ne + 7])
+        energies.append(terms["total"])
+        residual_history.append(res_norm)
+        evals_history.append(eigen.eigenvalues.copy())
+        de = abs(energies[-1] - energies[-2]) if len(energies) > 1 else float("inf")
+        if iteration >= numerics.min_scf and de < numerics.energy_tol and res_norm < numerics.density_tol:
+            converged = True
+            break
+        worsened = worsened + 1 if res_norm > previous_residual else 0
+        previous_residual = res_norm
+        if worsened >= 3 and rung + 1 < len(ladder):
+            rung += 1
+            mixer = ladder[rung]()
+            worsened = 0
+            rung_changed = True
+            events.append(f"iteration {iteration}: residual grew three times, rung {rung}")
+            n_next = torch.empty_like(n_in)
+            ops.lincomb(n_next.view(-1), [(n_in.reshape(-1), 1.0), (residual, 0.5 * numerics.alpha)])
+        n_in = n_next
+    h_final = step.hamiltonian(pin)
+    final_solver = solver
+    if filter_dtype != torch.float64:
+        final_solver = ChebyshevFilteredSubspace(grid, numerics.eigen, ritz_stop=False, filter_dtype=torch.float64)
+    if refine_steps > 0 and filter_dtype != torch.float64:
+        eigen = step.diagonalise(solver, h_final, initial=eigen.eigenvectors)
+        eigen = step.diagonalise(final_solver, h_final, initial=eigen.eigenvectors, max_iterations=refine_steps)
+    else:
+        eigen = step.diagonalise(final_solver, h_final, initial=eigen.eigenvectors)
+    occ = aufbau(eigen.eigenvalues, step.n_electrons)
+    n_out, rho_out = step.output_density(eigen, occ)
+    pout = step.potentials(n_out)
+    escal = step.energy_scalars(eigen.eigenvectors, occ, rho_out, pin["effective"])
+    host = torch.cat([escal, pin["scalars"], pout["scalars"]]).cpu().numpy()
+    k = eigen.eigenvectors.shape[0]
+    ne = k * k + 3
+    terms = energies_from_host(k, occ, eigen.eigenvalues, host[:ne], host[ne:ne + 3], host[ne + 3:ne + 6], step.ion_ion)
+    return {
+        "energies": terms,
+        "eigenvalues": eigen.eigenvalues,
+        "eigenvectors": eigen.eigenvectors,
+        "density": n_out,
+        "scf_iterations": iteration,
+        "converged": converged and eigen.converged,
+        "energy_history": energies,
+        "residual_history": residual_history,
+        "bound_history": policy.history,
+        "applications": applications,
+        "fallback_events": events,
+        "wall_seconds": time.perf_counter() - started,
+        "step": step,
+    }
+
+
+def solve_noninteracting(scenario, numerics, device, anchor=None, geometry=None):
+    """Fixed-potential eigenproblem (reference ``scf/noninteracting.py:118-330``): CheFSI with the
+    weak-form Ritz stop, then ``T_s`` and ``E_ext = band - T``."""
+    grid, factor, q = geometry if geometry is not None else build_geometry(scenario, numerics, device, anchor)
+    h = CuspFactoredHamiltonian(grid, factor, q)
+    n_states = max(1, int(-(-float(scenario.n_electrons) // 2.0)))
+    gen = torch.Generator(device="cpu").manual_seed(numerics.seed)
+    solver = ChebyshevFilteredSubspace(grid, numerics.eigen, ritz_stop=True)
+    result = solver.solve(h, n_states, generator=gen)
+    occ = aufbau(result.eigenvalues, scenario.n_electrons)
+    step = KohnShamStep.__new__(KohnShamStep)
+    step.grid, step.factor, step.q, step.bare = grid, factor, q, h
+    rho = grid.empty()
+    ops.density(result.eigenvectors, occ, None, rho, None)
+    escal = KohnShamStep.energy_scalars(step, result.eigenvectors, occ, rho, None).cpu().numpy()
+    k = result.eigenvectors.shape[0]
+    form = escal[: k * k].reshape(k, k)
+    kinetic = float((occ[:k] * np.diag(form)).sum()) - escal[k * k] - escal[k * k + 1]
+    band = float((occ * result.eigenvalues).sum())
+    ion = nuclear_repulsion(scenario.charges, scenario.positions)
+    return {"eigenvalues": result.eigenvalues, "eigenvectors": result.eigenvectors, "n_iterations": result.n_iterations,
+            "converged": result.converged, "kinetic": kinetic, "band": band, "total": band + ion, "info": result.info,
+            "hamiltonian": h}

@@ -64,6 +64,48 @@ int cdft_cheb_filter_f32(const float* x, float* out, float* work1, float* work2,
                          int k, int n0, int n1, int n2, const double* coef, int order,
                          const double* table, int degree, void* stream);
 
+/* ---- K3 / K4: float64 reductions and rotations of Rayleigh-Ritz (deterministic, no atomics) ---- */
+
+/* Doubles of block-partials workspace the reductions below need for length n. */
+long cdft_reduction_workspace_doubles(long n);
+
+/* out[i*kb + j] = scale * sum_p q[p] a[i*n + p] b[j*n + p] (q may be NULL for unit weights);
+ * post = 1 stores sqrt(max(x, 0)). Replaces Measure.inner/norm/gram/cross
+ * (src/cdft/eigen/measure.py:72-96) and orthonormality_error_tensor (eigen/rayleigh_ritz.py:44-48). */
+int cdft_wgram_f64(const double* a, int ka, const double* b, int kb, const double* q, long n,
+                   double scale, int post, double* partials, long partials_len, double* out,
+                   void* stream);
+
+/* y[i] = sum_j R[i][j] x[j] (R kout x kin row-major on the device; transpose = 1 uses R^T),
+ * kin, kout <= 16. Replaces the rotations `rotation.T @ basis` and the triangular solve of
+ * orthonormalise (src/cdft/eigen/rayleigh_ritz.py:51-102, precision.py:292-308). */
+int cdft_rotate_f64(const double* r, int kin, int kout, int transpose, const double* x, double* y,
+                    long n, void* stream);
+
+/* out = sum_t c_t x_t (x: host array of nterms device pointers; c, mode, idx: host arrays; mode 1
+ * multiplies c_t by dev[idx_t], mode 2 divides). nterms <= 16; out may alias a term. Replaces the
+ * Lanczos updates (src/cdft/operators/hamiltonian.py:55-65) and the Pulay combination
+ * (src/cdft/scf/mixing.py:158-162). */
+int cdft_lincomb_f64(double* out, int nterms, const double* const* x, const double* c,
+                     const int* mode, const int* idx, const double* dev, long n, void* stream);
+
+/* norms[i] = sqrt(max(sum_p q (av_i - evals_i v_i)^2, 0)), k <= 8. Replaces residual_norms
+ * (src/cdft/eigen/rayleigh_ritz.py:105-115). */
+int cdft_resnorm_f64(const double* av, const double* v, const double* evals, const double* q, int k,
+                     long n, double* partials, long partials_len, double* norms, void* stream);
+
+/* T = L^{-1} with S = L L^T (k <= 16); flag[0] = 1 if S is not positive definite. Replaces the
+ * Cholesky step of orthonormalise (src/cdft/eigen/rayleigh_ritz.py:63-72). */
+int cdft_chol_inv_f64(const double* s, int k, double* t, int* flag, void* stream);
+
+/* Symmetric eigendecomposition of 0.5 (M + M^T) (k <= 16) by cyclic Jacobi: evals ascending,
+ * eigenvectors in the columns of c. Replaces torch.linalg.eigh of the Rayleigh-Ritz matrix
+ * (src/cdft/eigen/rayleigh_ritz.py:96-99). */
+int cdft_eigh_f64(const double* m, int k, double* evals, double* c, void* stream);
+
+/* out[0] = max |M - I| over a k x k matrix (src/cdft/eigen/rayleigh_ritz.py:44-48). */
+int cdft_orth_error_f64(const double* m, int k, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,91 @@
+"""K3/K4 and the small dense kernels on the GPU against numpy float64.
+
+Tolerances: reductions and rotations 1e-13 relative (summation order differs); Jacobi eigenvalues
+1e-13 relative to the spectral radius, eigenvectors up to sign.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("ka,kb,n", [(1, 1, 1), (5, 5, 274625), (3, 7, 1000), (8, 8, 4097), (11, 9, 5000)])
+def test_wgram_matches_numpy(pkg, device, ka, kb, n):
+    rng = np.random.default_rng(ka * 100 + kb)
+    a = rng.standard_normal((ka, n))
+    b = rng.standard_normal((kb, n))
+    q = rng.uniform(0.1, 2.0, n)
+    got = pkg.ops.wgram(torch.as_tensor(a).to(device), torch.as_tensor(b).to(device), torch.as_tensor(q).to(device),
+                        scale=0.5).cpu().numpy()
+    expect = 0.5 * (a * q) @ b.T
+    assert np.abs(got - expect).max() <= 1e-13 * np.abs(expect).max() * max(1.0, np.sqrt(n) / 100)
+    again = pkg.ops.wgram(torch.as_tensor(a).to(device), torch.as_tensor(b).to(device), torch.as_tensor(q).to(device),
+                          scale=0.5).cpu().numpy()
+    assert np.array_equal(got, again)
+
+
+def test_rotate_and_lincomb(pkg, device):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((5, 3001))
+    r = rng.standard_normal((4, 5))
+    xd = torch.as_tensor(x).to(device)
+    got = pkg.ops.rotate(torch.as_tensor(r).to(device), xd).cpu().numpy()
+    assert np.abs(got - r @ x).max() < 1e-13 * np.abs(r @ x).max()
+    rt = rng.standard_normal((5, 3))
+    got = pkg.ops.rotate(torch.as_tensor(rt).to(device), xd, transpose=True).cpu().numpy()
+    assert np.abs(got - rt.T @ x).max() < 1e-13 * np.abs(rt.T @ x).max()
+    dev = torch.tensor([2.0, 4.0], dtype=torch.float64, device=device)
+    out = torch.empty(3001, dtype=torch.float64, device=device)
+    pkg.ops.lincomb(out, [(xd[0], 1.0), (xd[1], -0.5, 1, 0), (xd[2], 3.0, 2, 1)], dev)
+    expect = x[0] - 0.5 * 2.0 * x[1] + 3.0 / 4.0 * x[2]
+    assert np.abs(out.cpu().numpy() - expect).max() < 1e-14 * np.abs(expect).max()
+
+
+def test_resnorm(pkg, device):
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal((5, 7777))
+    av = rng.standard_normal((5, 7777))
+    e = rng.standard_normal(5)
+    q = rng.uniform(0.5, 1.5, 7777)
+    got = pkg.ops.resnorm(torch.as_tensor(av).to(device), torch.as_tensor(v).to(device),
+                          torch.as_tensor(e).to(device), torch.as_tensor(q).to(device)).cpu().numpy()
+    expect = np.sqrt(((av - e[:, None] * v) ** 2 * q).sum(axis=1))
+    assert np.abs(got - expect).max() < 1e-13 * expect.max()
+
+
+@pytest.mark.parametrize("k", [1, 2, 5, 8, 16])
+def test_cholesky_inverse_and_jacobi(pkg, device, k):
+    rng = np.random.default_rng(k)
+    a = rng.standard_normal((k, k + 3))
+    s = a @ a.T
+    sd = torch.as_tensor(s).to(device)
+    t = torch.empty_like(sd)
+    flag = torch.zeros(1, dtype=torch.int32, device=device)
+    pkg.ops.chol_inv(sd, t, flag)
+    tn = t.cpu().numpy()
+    assert int(flag.item()) == 0
+    assert np.abs(tn @ s @ tn.T - np.eye(k)).max() < 1e-12
+    assert np.abs(np.triu(tn, 1)).max() == 0.0
+    m = rng.standard_normal((k, k))
+    m = m + m.T
+    evals = torch.empty(k, dtype=torch.float64, device=device)
+    vecs = torch.empty((k, k), dtype=torch.float64, device=device)
+    pkg.ops.eigh(torch.as_tensor(m).to(device), evals, vecs)
+    w, u = np.linalg.eigh(m)
+    assert np.abs(evals.cpu().numpy() - w).max() < 1e-13 * np.abs(w).max()
+    got = vecs.cpu().numpy()
+    for j in range(k):
+        assert min(np.abs(got[:, j] - u[:, j]).max(), np.abs(got[:, j] + u[:, j]).max()) < 1e-10
+    assert np.abs(got.T @ got - np.eye(k)).max() < 1e-13
+
+
+def test_cholesky_flags_indefinite(pkg, device):
+    s = torch.tensor([[1.0, 2.0], [2.0, 1.0]], dtype=torch.float64, device=device)
+    t = torch.empty_like(s)
+    flag = torch.zeros(1, dtype=torch.int32, device=device)
+    pkg.ops.chol_inv(s, t, flag)
+    assert int(flag.item()) == 1
