@@ -39,55 +39,63 @@ __device__ __forceinline__ void block_sums(double (&acc)[COUNT], double* shared 
   }
 }
 
+// KA x KB accumulators per thread (compile-time; runtime ka <= KA, kb <= KB guard the loads).
+template <int KA, int KB>
 __global__ __launch_bounds__(RB) void wgram_partial_kernel(const double* __restrict__ a, int ka,
                                                            const double* __restrict__ b, int kb,
                                                            const double* __restrict__ q, long n,
                                                            double* __restrict__ partials) {
-  __shared__ double shared[4 * KMAX * KMAX];
-  double acc[KMAX * KMAX];
+  __shared__ double shared[4 * KA * KB];
+  double acc[KA * KB];
 #pragma unroll
-  for (int i = 0; i < KMAX * KMAX; ++i) acc[i] = 0.0;
+  for (int i = 0; i < KA * KB; ++i) acc[i] = 0.0;
   const long stride = (long)gridDim.x * RB;
   for (long p = (long)blockIdx.x * RB + threadIdx.x; p < n; p += stride) {
     const double w = q ? q[p] : 1.0;
-    double bv[KMAX];
+    double bv[KB];
 #pragma unroll
-    for (int j = 0; j < KMAX; ++j) bv[j] = (j < kb) ? b[(long)j * n + p] : 0.0;
+    for (int j = 0; j < KB; ++j) bv[j] = (j < kb) ? b[(long)j * n + p] : 0.0;
 #pragma unroll
-    for (int i = 0; i < KMAX; ++i) {
+    for (int i = 0; i < KA; ++i) {
       if (i < ka) {
         const double av = a[(long)i * n + p] * w;
 #pragma unroll
-        for (int j = 0; j < KMAX; ++j) acc[i * KMAX + j] += av * bv[j];
+        for (int j = 0; j < KB; ++j) acc[i * KB + j] += av * bv[j];
       }
     }
   }
-  block_sums<KMAX * KMAX>(acc, shared);
+  block_sums<KA * KB>(acc, shared);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < KMAX; ++i)
+    for (int i = 0; i < KA; ++i)
 #pragma unroll
-      for (int j = 0; j < KMAX; ++j)
-        if (i < ka && j < kb) partials[(long)blockIdx.x * KMAX * KMAX + i * KMAX + j] = acc[i * KMAX + j];
+      for (int j = 0; j < KB; ++j)
+        if (i < ka && j < kb) partials[(long)blockIdx.x * KMAX * KMAX + i * KMAX + j] = acc[i * KB + j];
   }
 }
 
-// One thread per output entry sums the partials in block order. post: 0 plain, 1 sqrt(max(x, 0)).
-__global__ void wgram_final_kernel(const double* __restrict__ partials, int nblocks, int ka, int kb,
-                                   double scale, int post, double* __restrict__ out, int ldo,
-                                   int row0, int col0) {
-  const int e = threadIdx.x;
+// One wavefront per output entry: lane l sums partials l, l+64, ... in order, then a fixed
+// butterfly. post: 0 plain, 1 sqrt(max(x, 0)).
+__global__ __launch_bounds__(64) void wgram_final_kernel(const double* __restrict__ partials, int nblocks,
+                                                         int ka, int kb, double scale, int post,
+                                                         double* __restrict__ out, int ldo, int row0,
+                                                         int col0) {
+  const int e = blockIdx.x;
   if (e >= ka * kb) return;
   const int i = e / kb, j = e % kb;
   double s = 0.0;
-  for (int blk = 0; blk < nblocks; ++blk) s += partials[(long)blk * KMAX * KMAX + i * KMAX + j];
-  s *= scale;
-  if (post == 1) s = sqrt(s > 0.0 ? s : 0.0);
-  out[(row0 + i) * ldo + col0 + j] = s;
+  for (int blk = threadIdx.x; blk < nblocks; blk += 64) s += partials[(long)blk * KMAX * KMAX + i * KMAX + j];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (threadIdx.x == 0) {
+    s *= scale;
+    if (post == 1) s = sqrt(s > 0.0 ? s : 0.0);
+    out[(row0 + i) * ldo + col0 + j] = s;
+  }
 }
 
 static int reduction_blocks(long n) {
-  long blocks = (n + 2047) / 2048;
+  long blocks = (n + 1023) / 1024;
   if (blocks < 1) blocks = 1;
   if (blocks > MAX_PARTIALS) blocks = MAX_PARTIALS;
   return (int)blocks;
@@ -285,6 +293,134 @@ __global__ void eigh_kernel(const double* __restrict__ m_in, int k, double* __re
   }
 }
 
+// The same two kernels for a compile-time K <= 8: every index is a constant, so the matrices live
+// in registers (the generic versions above keep them in per-lane scratch).
+template <int K>
+__global__ void chol_inv_small_kernel(const double* __restrict__ s, double* __restrict__ t, int* __restrict__ flag) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double l[K][K], ti[K][K];
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < K; ++i)
+#pragma unroll
+    for (int j = 0; j < K; ++j) l[i][j] = ti[i][j] = 0.0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    double d = s[j * K + j];
+#pragma unroll
+    for (int m = 0; m < j; ++m) d -= l[j][m] * l[j][m];
+    ok = ok && (d > 0.0);
+    const double ljj = sqrt(d > 0.0 ? d : 1.0);
+    l[j][j] = ljj;
+#pragma unroll
+    for (int i = j + 1; i < K; ++i) {
+      double v = s[i * K + j];
+#pragma unroll
+      for (int m = 0; m < j; ++m) v -= l[i][m] * l[j][m];
+      l[i][j] = v / ljj;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < K; ++c)
+#pragma unroll
+    for (int i = c; i < K; ++i) {
+      double v = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int m = c; m < i; ++m) v -= l[i][m] * ti[m][c];
+      ti[i][c] = v / l[i][i];
+    }
+#pragma unroll
+  for (int i = 0; i < K; ++i)
+#pragma unroll
+    for (int j = 0; j < K; ++j) t[i * K + j] = ok ? ti[i][j] : 0.0;
+  flag[0] = ok ? 0 : 1;
+}
+
+template <int K>
+__global__ void eigh_small_kernel(const double* __restrict__ m_in, double* __restrict__ evals, double* __restrict__ c_out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a[K][K], v[K][K];
+#pragma unroll
+  for (int i = 0; i < K; ++i)
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      a[i][j] = 0.5 * (m_in[i * K + j] + m_in[j * K + i]);
+      v[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, diag = 0.0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      diag += a[i][i] * a[i][i];
+#pragma unroll
+      for (int j = i + 1; j < K; ++j) off += a[i][j] * a[i][j];
+    }
+    if (off <= 1e-36 * diag || off == 0.0) break;
+#pragma unroll
+    for (int p = 0; p < K - 1; ++p) {
+#pragma unroll
+      for (int q = p + 1; q < K; ++q) {
+        const double apq = a[p][q];
+        if (apq != 0.0) {
+          const double app = a[p][p], aqq = a[q][q];
+          const double theta = (aqq - app) / (2.0 * apq);
+          const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+          const double cs = 1.0 / sqrt(tt * tt + 1.0), sn = tt * cs;
+#pragma unroll
+          for (int r = 0; r < K; ++r) {
+            const double arp = a[r][p], arq = a[r][q];
+            a[r][p] = cs * arp - sn * arq;
+            a[r][q] = sn * arp + cs * arq;
+          }
+#pragma unroll
+          for (int r = 0; r < K; ++r) {
+            const double apr = a[p][r], aqr = a[q][r];
+            a[p][r] = cs * apr - sn * aqr;
+            a[q][r] = sn * apr + cs * aqr;
+          }
+          a[p][q] = a[q][p] = 0.0;
+#pragma unroll
+          for (int r = 0; r < K; ++r) {
+            const double vrp = v[r][p], vrq = v[r][q];
+            v[r][p] = cs * vrp - sn * vrq;
+            v[r][q] = sn * vrp + cs * vrq;
+          }
+        }
+      }
+    }
+  }
+  double e[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) e[i] = a[i][i];
+  // Stable ascending sort of (e, columns of v) by an unrolled exchange network.
+#pragma unroll
+  for (int pass = 0; pass < K; ++pass)
+#pragma unroll
+    for (int j = 0; j + 1 < K; ++j)
+      if (e[j] > e[j + 1]) {
+        const double te = e[j];
+        e[j] = e[j + 1];
+        e[j + 1] = te;
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+          const double tv = v[r][j];
+          v[r][j] = v[r][j + 1];
+          v[r][j + 1] = tv;
+        }
+      }
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    evals[j] = e[j];
+    double big = v[0][j];
+#pragma unroll
+    for (int i = 1; i < K; ++i)
+      if (fabs(v[i][j]) > fabs(big)) big = v[i][j];
+    const double sgn = big < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) c_out[i * K + j] = sgn * v[i][j];
+  }
+}
+
 // max |M - I| for a k x k matrix (orthonormality error), written to out[0].
 __global__ void max_abs_minus_identity_kernel(const double* __restrict__ m, int k, double* __restrict__ out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -316,11 +452,25 @@ extern "C" int cdft_wgram_f64(const double* a, int ka, const double* b, int kb, 
     for (int j0 = 0; j0 < kb; j0 += KMAX) {
       const int ti = ka - i0 < KMAX ? ka - i0 : KMAX;
       const int tj = kb - j0 < KMAX ? kb - j0 : KMAX;
-      hipLaunchKernelGGL(wgram_partial_kernel, dim3(blocks), dim3(RB), 0, s, a + (long)i0 * n, ti,
-                         b + (long)j0 * n, tj, q, n, partials);
+      const double* ap = a + (long)i0 * n;
+      const double* bp = b + (long)j0 * n;
+      if (ti == tj && ti <= 8) {
+        switch (ti) {
+          case 1: hipLaunchKernelGGL((wgram_partial_kernel<1, 1>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          case 2: hipLaunchKernelGGL((wgram_partial_kernel<2, 2>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          case 3: hipLaunchKernelGGL((wgram_partial_kernel<3, 3>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          case 4: hipLaunchKernelGGL((wgram_partial_kernel<4, 4>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          case 5: hipLaunchKernelGGL((wgram_partial_kernel<5, 5>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          case 6: hipLaunchKernelGGL((wgram_partial_kernel<6, 6>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          case 7: hipLaunchKernelGGL((wgram_partial_kernel<7, 7>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          default: hipLaunchKernelGGL((wgram_partial_kernel<8, 8>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+        }
+      } else {
+        hipLaunchKernelGGL((wgram_partial_kernel<KMAX, KMAX>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials);
+      }
       int rc = check_launch("wgram_partial");
       if (rc) return rc;
-      hipLaunchKernelGGL(wgram_final_kernel, dim3(1), dim3(64), 0, s, partials, blocks, ti, tj, scale,
+      hipLaunchKernelGGL(wgram_final_kernel, dim3(ti * tj), dim3(64), 0, s, partials, blocks, ti, tj, scale,
                          post, out, kb, i0, j0);
       rc = check_launch("wgram_final");
       if (rc) return rc;
@@ -379,19 +529,41 @@ extern "C" int cdft_resnorm_f64(const double* av, const double* v, const double*
   int rc = check_launch("resnorm_partial");
   if (rc) return rc;
   // Reuse the Gram final pass: entries (i, 0) of each partial tile, one column.
-  hipLaunchKernelGGL(wgram_final_kernel, dim3(1), dim3(64), 0, s, partials, blocks, k, 1, 1.0, 1, norms, 1, 0, 0);
+  hipLaunchKernelGGL(wgram_final_kernel, dim3(k), dim3(64), 0, s, partials, blocks, k, 1, 1.0, 1, norms, 1, 0, 0);
   return check_launch("resnorm_final");
 }
 
 extern "C" int cdft_chol_inv_f64(const double* s, int k, double* t, int* flag, void* stream) {
   if (!s || !t || !flag || k < 1 || k > KROT) return fail(CDFT_E_ARG, "chol_inv: bad arguments (k=%d)", k);
-  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, s, k, t, flag);
+  hipStream_t st = (hipStream_t)stream;
+  switch (k) {
+    case 1: hipLaunchKernelGGL(chol_inv_small_kernel<1>, dim3(1), dim3(64), 0, st, s, t, flag); break;
+    case 2: hipLaunchKernelGGL(chol_inv_small_kernel<2>, dim3(1), dim3(64), 0, st, s, t, flag); break;
+    case 3: hipLaunchKernelGGL(chol_inv_small_kernel<3>, dim3(1), dim3(64), 0, st, s, t, flag); break;
+    case 4: hipLaunchKernelGGL(chol_inv_small_kernel<4>, dim3(1), dim3(64), 0, st, s, t, flag); break;
+    case 5: hipLaunchKernelGGL(chol_inv_small_kernel<5>, dim3(1), dim3(64), 0, st, s, t, flag); break;
+    case 6: hipLaunchKernelGGL(chol_inv_small_kernel<6>, dim3(1), dim3(64), 0, st, s, t, flag); break;
+    case 7: hipLaunchKernelGGL(chol_inv_small_kernel<7>, dim3(1), dim3(64), 0, st, s, t, flag); break;
+    case 8: hipLaunchKernelGGL(chol_inv_small_kernel<8>, dim3(1), dim3(64), 0, st, s, t, flag); break;
+    default: hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(64), 0, st, s, k, t, flag);
+  }
   return check_launch("chol_inv");
 }
 
 extern "C" int cdft_eigh_f64(const double* m, int k, double* evals, double* c, void* stream) {
   if (!m || !evals || !c || k < 1 || k > KROT) return fail(CDFT_E_ARG, "eigh: bad arguments (k=%d)", k);
-  hipLaunchKernelGGL(eigh_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, m, k, evals, c);
+  hipStream_t st = (hipStream_t)stream;
+  switch (k) {
+    case 1: hipLaunchKernelGGL(eigh_small_kernel<1>, dim3(1), dim3(64), 0, st, m, evals, c); break;
+    case 2: hipLaunchKernelGGL(eigh_small_kernel<2>, dim3(1), dim3(64), 0, st, m, evals, c); break;
+    case 3: hipLaunchKernelGGL(eigh_small_kernel<3>, dim3(1), dim3(64), 0, st, m, evals, c); break;
+    case 4: hipLaunchKernelGGL(eigh_small_kernel<4>, dim3(1), dim3(64), 0, st, m, evals, c); break;
+    case 5: hipLaunchKernelGGL(eigh_small_kernel<5>, dim3(1), dim3(64), 0, st, m, evals, c); break;
+    case 6: hipLaunchKernelGGL(eigh_small_kernel<6>, dim3(1), dim3(64), 0, st, m, evals, c); break;
+    case 7: hipLaunchKernelGGL(eigh_small_kernel<7>, dim3(1), dim3(64), 0, st, m, evals, c); break;
+    case 8: hipLaunchKernelGGL(eigh_small_kernel<8>, dim3(1), dim3(64), 0, st, m, evals, c); break;
+    default: hipLaunchKernelGGL(eigh_kernel, dim3(1), dim3(64), 0, st, m, k, evals, c);
+  }
   return check_launch("eigh");
 }
 

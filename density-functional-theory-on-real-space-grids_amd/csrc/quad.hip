@@ -11,8 +11,8 @@
 //                      (quadrature.py:56-93, 506-560)
 //   interpolate      : out[f][node] = sum_abc wx wy wz field[f][base + abc] -- matrix-free L
 //   lift             : out[f][g] = sum over nodes whose stencil covers g of x[f][node] * weight,
-//                      gathered per grid point through a cell-binned node list -- matrix-free L^T,
-//                      deterministic (no atomics)
+//                      gathered per grid point from nodes numbered in base-cell order (a CSR of
+//                      cell starts) -- matrix-free L^T, deterministic (no atomics)
 //   gradient         : order-2P central first derivatives with a zero halo (laplacian.py:111-144)
 #include "common.h"
 
@@ -278,39 +278,58 @@ struct CellBox {
   int c0, c1, c2;     // cell-box extents
 };
 
-__global__ __launch_bounds__(256) void lift_kernel(const double* __restrict__ x, int nf, long m,
-                                                   const int* __restrict__ cell_start,
-                                                   const int* __restrict__ cell_nodes,
-                                                   const int* __restrict__ base,
-                                                   const double* __restrict__ lw, GridDesc g,
-                                                   CellBox cb, const double* __restrict__ add,
-                                                   double* __restrict__ out) {
-  const long n = (long)g.n0 * g.n1 * g.n2;
+// out[f][p] = add[f][p] (or 0) for every grid point; the region kernel then adds the node sums.
+__global__ void lift_fill_kernel(const double* __restrict__ add, int nf, long n, double* __restrict__ out) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const int i2 = (int)(p % g.n2);
-  const int i1 = (int)((p / g.n2) % g.n1);
-  const int i0 = (int)(p / ((long)g.n1 * g.n2));
+  for (int f = 0; f < nf; ++f) out[f * n + p] = add ? add[f * n + p] : 0.0;
+}
+
+// One wavefront per grid point of the region the nodes can reach. Lane (d0, d1) = (lane / 8,
+// lane % 8) owns the cells (i0-4+d0, i1-4+d1, *) whose z-cells [i2-4, i2+3] are consecutive in the
+// cell-sorted node list, so each lane walks one contiguous node range. Lane sums are combined by a
+// fixed butterfly, so the result is deterministic.
+__global__ __launch_bounds__(256) void lift_region_kernel(const double* __restrict__ x, int nf, long m,
+                                                          const int* __restrict__ cell_start,
+                                                          const int* __restrict__ base,
+                                                          const double* __restrict__ lw, GridDesc g,
+                                                          CellBox cb, int r0, int r1, int r2, int e0,
+                                                          int e1, int e2, double* __restrict__ out) {
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const long nregion = (long)e0 * e1 * e2;
+  if (wave >= nregion) return;  // uniform per wave
+  const int i2 = r2 + (int)(wave % e2);
+  const int i1 = r1 + (int)((wave / e2) % e1);
+  const int i0 = r0 + (int)(wave / ((long)e1 * e2));
+  const int b0 = i0 - 4 + (lane >> 3), b1 = i1 - 4 + (lane & 7);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  // Cells whose stencil [b-3, b+4] covers this point: b in [i-4, i+3], clipped to the cell box.
-  const int l0 = max(i0 - 4, cb.lo0), h0 = min(i0 + 3, cb.lo0 + cb.c0 - 1);
-  const int l1 = max(i1 - 4, cb.lo1), h1 = min(i1 + 3, cb.lo1 + cb.c1 - 1);
-  const int l2 = max(i2 - 4, cb.lo2), h2 = min(i2 + 3, cb.lo2 + cb.c2 - 1);
-  for (int b0 = l0; b0 <= h0; ++b0)
-    for (int b1 = l1; b1 <= h1; ++b1)
-      for (int b2 = l2; b2 <= h2; ++b2) {
-        const int cell = ((b0 - cb.lo0) * cb.c1 + (b1 - cb.lo1)) * cb.c2 + (b2 - cb.lo2);
-        const int s = cell_start[cell], e = cell_start[cell + 1];
-        const int j0 = i0 - b0 + 3, j1 = i1 - b1 + 3, j2 = i2 - b2 + 3;
-        for (int t = s; t < e; ++t) {
-          const int node = cell_nodes[t];
-          const double w = (lw[(long)node * 24 + j0] * lw[(long)node * 24 + 8 + j1]) * lw[(long)node * 24 + 16 + j2];
-          for (int f = 0; f < 4; ++f)
-            if (f < nf) acc[f] += x[f * m + node] * w;
-        }
+  if (b0 >= cb.lo0 && b0 < cb.lo0 + cb.c0 && b1 >= cb.lo1 && b1 < cb.lo1 + cb.c1) {
+    const int zl = max(i2 - 4, cb.lo2), zh = min(i2 + 3, cb.lo2 + cb.c2 - 1);
+    if (zl <= zh) {
+      const int row = ((b0 - cb.lo0) * cb.c1 + (b1 - cb.lo1)) * cb.c2 - cb.lo2;
+      const int s = cell_start[row + zl], e = cell_start[row + zh + 1];
+      const int j0 = i0 - b0 + 3, j1 = i1 - b1 + 3;
+      for (int node = s; node < e; ++node) {  // nodes are numbered in base-cell order
+        const int j2 = i2 - base[3 * node + 2] + 3;
+        const double* l = lw + (long)node * 24;
+        const double w = (l[j0] * l[8 + j1]) * l[16 + j2];
+        for (int f = 0; f < 4; ++f)
+          if (f < nf) acc[f] += x[f * m + node] * w;
       }
-  for (int f = 0; f < 4; ++f)
-    if (f < nf) out[f * n + p] = add ? acc[f] + add[f * n + p] : acc[f];
+    }
+  }
+  for (int f = 0; f < 4; ++f) {
+    double v = acc[f];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    acc[f] = v;
+  }
+  if (lane == 0) {
+    const long n = (long)g.n0 * g.n1 * g.n2;
+    const long p = ((long)i0 * g.n1 + i1) * g.n2 + i2;
+    for (int f = 0; f < nf; ++f) out[f * n + p] = acc[f] + out[f * n + p];
+  }
 }
 
 template <int P>
@@ -527,10 +546,10 @@ extern "C" int cdft_interpolate_f64(const double* field, int nf, const int* base
 
 // cell_box (host): lo0, lo1, lo2, c0, c1, c2. add may be NULL.
 extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* cell_start,
-                             const int* cell_nodes, const int* base, const double* lw,
+                             const int* base, const double* lw,
                              const int* shape, double h, const double* origin, const int* cell_box,
                              const double* add, double* out, void* stream) {
-  if (!x || !cell_start || !cell_nodes || !base || !lw || !out || !cell_box || nf < 1 || nf > 4)
+  if (!x || !cell_start || !base || !lw || !out || !cell_box || nf < 1 || nf > 4)
     return fail(CDFT_E_ARG, "lift: bad arguments (nf=%d)", nf);
   GridDesc g = make_grid(shape, h, origin);
   CellBox cb;
@@ -541,9 +560,20 @@ extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* cell_st
   cb.c1 = cell_box[4];
   cb.c2 = cell_box[5];
   const long n = (long)g.n0 * g.n1 * g.n2;
-  hipLaunchKernelGGL(lift_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, x, nf, m, cell_start,
-                     cell_nodes, base, lw, g, cb, add, out);
-  return check_launch("lift");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(lift_fill_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, add, nf, n, out);
+  int rc = check_launch("lift_fill");
+  if (rc) return rc;
+  // Grid points reachable from a node whose base cell lies in the cell box: [lo-3, lo+c-1+4].
+  const int r0 = max(cb.lo0 - 3, 0), r1 = max(cb.lo1 - 3, 0), r2 = max(cb.lo2 - 3, 0);
+  const int h0 = min(cb.lo0 + cb.c0 + 3, g.n0 - 1), h1 = min(cb.lo1 + cb.c1 + 3, g.n1 - 1),
+            h2 = min(cb.lo2 + cb.c2 + 3, g.n2 - 1);
+  const int e0 = h0 - r0 + 1, e1 = h1 - r1 + 1, e2 = h2 - r2 + 1;
+  if (e0 <= 0 || e1 <= 0 || e2 <= 0) return CDFT_OK;
+  const long waves = (long)e0 * e1 * e2;
+  hipLaunchKernelGGL(lift_region_kernel, dim3(blocks_for(waves * 64, 256)), dim3(256), 0, s, x, nf, m, cell_start,
+                     base, lw, g, cb, r0, r1, r2, e0, e1, e2, out);
+  return check_launch("lift_region");
 }
 
 // coef: DEVICE array of the 2P+1 central first-derivative weights divided by h.

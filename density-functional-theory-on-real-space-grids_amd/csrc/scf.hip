@@ -175,19 +175,27 @@ __global__ __launch_bounds__(RBS) void reduce_partial_kernel(int mode, const dou
 }
 
 // out[slot] = scale * (reduction of partials) (+ out[slot] if accumulate).
-__global__ void reduce_final_kernel(int mode, const double* __restrict__ partials, int nblocks,
-                                    double scale, int accumulate, double* __restrict__ out, int slot) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double r = partials[0];
-  for (int i = 1; i < nblocks; ++i) r = (mode == 2) ? (partials[i] > r ? partials[i] : r) : r + partials[i];
-  r = r * scale;
-  out[slot] = accumulate ? out[slot] + r : r;
+// One wavefront: lane l combines partials l, l+64, ... in order, then a fixed butterfly.
+__global__ __launch_bounds__(64) void reduce_final_kernel(int mode, const double* __restrict__ partials,
+                                                          int nblocks, double scale, int accumulate,
+                                                          double* __restrict__ out, int slot) {
+  double r = (mode == 2) ? 0.0 : 0.0;
+  for (int i = threadIdx.x; i < nblocks; i += 64) r = (mode == 2) ? (partials[i] > r ? partials[i] : r) : r + partials[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(r, off, 64);
+    r = (mode == 2) ? (o > r ? o : r) : r + o;
+  }
+  if (threadIdx.x == 0) {
+    r = r * scale;
+    out[slot] = accumulate ? out[slot] + r : r;
+  }
 }
 
 static unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
 static int red_blocks(long n) {
-  long b = (n + 2047) / 2048;
+  long b = (n + 1023) / 1024;
   if (b < 1) b = 1;
   if (b > SC_MAX_PARTIALS) b = SC_MAX_PARTIALS;
   return (int)b;

@@ -128,12 +128,20 @@ def _fraction(value, low, high):
 class ChebyshevFilteredSubspace:
     """CheFSI (reference ``eigen/chefsi.py:408-585``) on the device."""
 
-    def __init__(self, grid, config: EigenConfig, ritz_stop: bool = True, filter_dtype=torch.float64):
+    #: Mixed precision: filter in float32 while the worst residual is above this...
+    switch_residual: float = 1.0e-5
+    #: ...and always finish an iteration-limited solve with this many float64 filter passes.
+    fp64_tail: int = 1
+
+    def __init__(self, grid, config: EigenConfig, ritz_stop: bool = True, filter_dtype=torch.float64,
+                 filter_hook=None):
         self.grid = grid
         self.config = config
         self.ritz_stop = ritz_stop
         self.filter_dtype = filter_dtype
+        self.filter_hook = filter_hook
         self.history: list[dict] = []
+        self.filter_precisions: list[str] = []
         self._work: dict[int, RitzWork] = {}
 
     def work_for(self, k: int) -> RitzWork:
@@ -141,17 +149,23 @@ class ChebyshevFilteredSubspace:
             self._work[k] = RitzWork(k, self.grid.n_points, self.grid.device)
         return self._work[k]
 
-    def filter(self, hamiltonian, block: torch.Tensor, out: torch.Tensor, work: RitzWork, lower, upper, smin):
+    def filter(self, hamiltonian, block: torch.Tensor, out: torch.Tensor, work: RitzWork, lower, upper, smin,
+               dtype=torch.float64):
         rows = chebyshev_coefficients(self.config.chebyshev_degree, lower, upper, smin)
         table = torch.tensor(rows, dtype=torch.float64).pin_memory().to(block.device, non_blocking=True)
-        geom = hamiltonian.geometry
-        if self.filter_dtype == torch.float64:
-            ops.cheb_filter(block, geom, table, len(rows), out=out, work=(work.work1, work.work2))
+        if self.filter_hook is not None:
+            self.filter_hook("start", self, block if dtype == torch.float64 else block.to(dtype), len(rows))
+        if dtype == torch.float64:
+            ops.cheb_filter(block, hamiltonian.geometry, table, len(rows), out=out, work=(work.work1, work.work2))
+            self.filter_precisions.append("f64")
         else:
-            g32 = hamiltonian.geometry_in(self.filter_dtype)
-            x32 = block.to(self.filter_dtype)
+            g32 = hamiltonian.geometry_in(dtype)
+            x32 = block.to(dtype)
             y32 = ops.cheb_filter(x32, g32, table, len(rows))
             out.copy_(y32)
+            self.filter_precisions.append("f32")
+        if self.filter_hook is not None:
+            self.filter_hook("end", self, block if dtype == torch.float64 else x32, len(rows))
         hamiltonian.note_applications(len(rows))
         return out
 
@@ -181,6 +195,7 @@ class ChebyshevFilteredSubspace:
         rayleigh_ritz(hamiltonian, block, work)
         residual_norms(hamiltonian, work)
         evals, norms, _ = _read(work)
+        worst = float(norms[:n_states].max())
         previous = evals.copy()
         stable = 0
         stop_reason = "iteration limit"
@@ -206,7 +221,11 @@ class ChebyshevFilteredSubspace:
             lower = min(max(lower, spectrum_min + 1e-8), spectrum_max - 1e-8)
             # The Ritz vectors of the previous step are the filter input.
             src = work.vectors
-            self.filter(hamiltonian, src, work.filtered, work, lower, spectrum_max, spectrum_min)
+            dtype = torch.float64
+            if self.filter_dtype != torch.float64 and worst > self.switch_residual and \
+                    iteration <= cfg.max_iterations - self.fp64_tail:
+                dtype = self.filter_dtype
+            self.filter(hamiltonian, src, work.filtered, work, lower, spectrum_max, spectrum_min, dtype)
             rayleigh_ritz(hamiltonian, work.filtered, work)
             residual_norms(hamiltonian, work)
             evals, norms, orth = _read(work)

@@ -270,11 +270,22 @@ class CuspQuadrature:
         self.spheres = spheres
         self._spheres_c = host_doubles([v for s in spheres for v in s])
         points = np.concatenate(pts_list)
+        owners = np.concatenate(owner_list)
+        bare_weights = np.concatenate(w_list)
+        # Number the nodes in the order of their interpolation base cell (stable), so the nodes a
+        # grid point gathers from in the transpose are contiguous in memory. Only the order of the
+        # node sums changes.
+        base_key = np.floor((points - origin) / h).astype(np.int64)
+        lo_key = base_key.min(axis=0)
+        ext_key = base_key.max(axis=0) - lo_key + 1
+        key = ((base_key[:, 0] - lo_key[0]) * ext_key[1] + (base_key[:, 1] - lo_key[1])) * ext_key[2] + (base_key[:, 2] - lo_key[2])
+        perm = np.argsort(key, kind="stable")
+        points, owners, bare_weights = points[perm], owners[perm], bare_weights[perm]
         self.n_nodes = m = points.shape[0]
         as_dev = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)  # noqa: E731
         self.points = as_dev(points)
-        owner = as_dev(np.concatenate(owner_list), torch.int32)
-        base_w = as_dev(np.concatenate(w_list))
+        owner = as_dev(owners.astype(np.int32), torch.int32)
+        base_w = as_dev(bare_weights)
         self.weights = torch.empty(m, dtype=torch.float64, device=dev)
         self.f2 = torch.empty(m, dtype=torch.float64, device=dev)
         self.grad_u = torch.empty((3, m), dtype=torch.float64, device=dev)
@@ -299,11 +310,13 @@ class CuspQuadrature:
         ext = base_host.max(axis=0) - lo + 1
         cell = ((base_host[:, 0] - lo[0]) * ext[1] + (base_host[:, 1] - lo[1])) * ext[2] + (base_host[:, 2] - lo[2])
         order = np.argsort(cell, kind="stable").astype(np.int32)
+        if not np.array_equal(order, np.arange(m, dtype=np.int32)):
+            raise RuntimeError("quadrature nodes are not in base-cell order")
         counts = np.bincount(cell, minlength=int(ext.prod()))
         start = np.zeros(counts.size + 1, dtype=np.int32)
         np.cumsum(counts, out=start[1:])
-        self.cell_nodes = as_dev(order, torch.int32)
         self.cell_start = as_dev(start, torch.int32)
+        self.node_order = perm  # GPU node i is node perm[i] of the sphere-by-sphere enumeration
         self._cell_box = _ints([lo[0], lo[1], lo[2], ext[0], ext[1], ext[2]])
         # Far part of the rule.
         self.far = grid.empty()
@@ -339,7 +352,7 @@ class CuspQuadrature:
         nf = x.shape[0]
         out = torch.empty((nf, self.grid.n_points), dtype=torch.float64, device=self.grid.device) if out is None else out
         _lib.call("cdft_lift_f64", x.data_ptr(), nf, self.n_nodes, self.cell_start.data_ptr(),
-                  self.cell_nodes.data_ptr(), self.base.data_ptr(), self.lw.data_ptr(), self.grid.c_shape,
+                  self.base.data_ptr(), self.lw.data_ptr(), self.grid.c_shape,
                   self.grid.spacing, self.grid.c_origin, ctypes.addressof(self._cell_box), _ptr_or_none(add),
                   out.data_ptr(), stream_of(x))
         return out.reshape(-1) if node_values.dim() == 1 else out

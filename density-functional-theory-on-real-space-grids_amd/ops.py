@@ -14,7 +14,13 @@ import torch
 from . import _lib
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream_of(tensor: torch.Tensor) -> int:
+    """The raw ``hipStream_t`` torch currently issues work on for the tensor's device."""
+    if _raw_stream is not None:
+        return _raw_stream(tensor.device.index if tensor.device.index is not None else torch.cuda.current_device())
     return torch.cuda.current_stream(tensor.device).cuda_stream
 
 

@@ -348,16 +348,18 @@ def initial_density(step: KohnShamStep) -> torch.Tensor:
 
 
 def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None, filter_dtype=torch.float64,
-                          refine_steps: int = 0):
+                          filter_hook=None):
     """The SCF driver (reference ``scf/loop.py:255-460``). ``filter_dtype=float32`` runs the Chebyshev
-    filter applies in float32 (everything else float64) and ``refine_steps`` float64 filter passes
-    close the final diagonalisation."""
+    filter in float32 while the eigen-residual is above the solver's switch and finishes every solve
+    with float64 passes; Rayleigh-Ritz, orthonormalisation, densities, Poisson, XC and energies stay
+    float64."""
     started = time.perf_counter()
     grid, factor, q = geometry if geometry is not None else build_geometry(scenario, numerics, device, anchor)
     step = KohnShamStep(scenario, numerics, grid, factor, q)
     g = grid
     inner_weights = ops.ewise(ops.EW_DIV, g.empty(), q.mass_weights, ops.ewise(ops.EW_MUL, g.empty(), step.f2, step.f2))
-    solver = ChebyshevFilteredSubspace(grid, numerics.eigen, ritz_stop=False, filter_dtype=filter_dtype)
+    solver = ChebyshevFilteredSubspace(grid, numerics.eigen, ritz_stop=False, filter_dtype=filter_dtype,
+                                       filter_hook=filter_hook)
     n_in = initial_density(step)
     mixer = PulayMixer(numerics.alpha, numerics.history, numerics.pulay_period, inner_weights)
     ladder = [
@@ -431,14 +433,7 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
             ops.lincomb(n_next.view(-1), [(n_in.reshape(-1), 1.0), (residual, 0.5 * numerics.alpha)])
         n_in = n_next
     h_final = step.hamiltonian(pin)
-    final_solver = solver
-    if filter_dtype != torch.float64:
-        final_solver = ChebyshevFilteredSubspace(grid, numerics.eigen, ritz_stop=False, filter_dtype=torch.float64)
-    if refine_steps > 0 and filter_dtype != torch.float64:
-        eigen = step.diagonalise(solver, h_final, initial=eigen.eigenvectors)
-        eigen = step.diagonalise(final_solver, h_final, initial=eigen.eigenvectors, max_iterations=refine_steps)
-    else:
-        eigen = step.diagonalise(final_solver, h_final, initial=eigen.eigenvectors)
+    eigen = step.diagonalise(solver, h_final, initial=eigen.eigenvectors)
     occ = aufbau(eigen.eigenvalues, step.n_electrons)
     n_out, rho_out = step.output_density(eigen, occ)
     pout = step.potentials(n_out)
@@ -459,6 +454,7 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
         "bound_history": policy.history,
         "applications": applications,
         "fallback_events": events,
+        "filter_precisions": list(solver.filter_precisions),
         "wall_seconds": time.perf_counter() - started,
         "step": step,
     }
