@@ -207,6 +207,7 @@ def main(argv=None):
                                "degree 20, periodic Pulay, float64, geometry build included",
                    "grid": list(shape), "block": 5},
         "energy_total": r["energies"]["total"],
+        "hip_graphs": r["graphs"],
         "energy_reference_cpu": -2.8348195294207272,
         "scf_iterations": r["scf_iterations"],
         "block_applies": int(sum(r["applications"])),

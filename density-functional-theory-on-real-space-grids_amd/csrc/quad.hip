@@ -10,9 +10,9 @@
 //   node_stencils    : base cell and the three 1-D degree-7 Lagrange weight rows of each node
 //                      (quadrature.py:56-93, 506-560)
 //   interpolate      : out[f][node] = sum_abc wx wy wz field[f][base + abc] -- matrix-free L
-//   lift             : out[f][g] = sum over nodes whose stencil covers g of x[f][node] * weight,
-//                      gathered per grid point from nodes numbered in base-cell order (a CSR of
-//                      cell starts) -- matrix-free L^T, deterministic (no atomics)
+//   lift             : out[f][g] = sum over nodes whose stencil covers g of x[f][node] * weight --
+//                      matrix-free L^T in two separable stages (per z-column of cells, then a
+//                      64-term combine per grid point), deterministic (no atomics)
 //   gradient         : order-2P central first derivatives with a zero halo (laplacian.py:111-144)
 #include "common.h"
 
@@ -278,58 +278,117 @@ struct CellBox {
   int c0, c1, c2;     // cell-box extents
 };
 
-// out[f][p] = add[f][p] (or 0) for every grid point; the region kernel then adds the node sums.
-__global__ void lift_fill_kernel(const double* __restrict__ add, int nf, long n, double* __restrict__ out) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  for (int f = 0; f < nf; ++f) out[f * n + p] = add ? add[f * n + p] : 0.0;
-}
-
-// One wavefront per grid point of the region the nodes can reach. Lane (d0, d1) = (lane / 8,
-// lane % 8) owns the cells (i0-4+d0, i1-4+d1, *) whose z-cells [i2-4, i2+3] are consecutive in the
-// cell-sorted node list, so each lane walks one contiguous node range. Lane sums are combined by a
-// fixed butterfly, so the result is deterministic.
-__global__ __launch_bounds__(256) void lift_region_kernel(const double* __restrict__ x, int nf, long m,
+// Stage 1 of the transpose, one block of 64 threads per z-column (b0, b1) of the cell box. Thread
+// t = 8 j0 + j1 accumulates Z[f][col][t][iz] = sum over the column's nodes of
+// x[f][n] * (wx_n[j0] * wy_n[j1]) * wz_n[iz - (b2_n - lo2)], iz indexing grid z = lo2 - 3 + iz. The
+// column's nodes are contiguous and sorted by b2 (nodes are numbered in base-cell order), so each
+// thread keeps a sliding register window of the 8 z-values a node touches and retires the lowest
+// one whenever b2 advances: no dynamic register indexing, no atomics, node data broadcast to the
+// block.
+__global__ __launch_bounds__(64) void lift_columns_kernel(const double* __restrict__ x, int nf, long m,
                                                           const int* __restrict__ cell_start,
                                                           const int* __restrict__ base,
-                                                          const double* __restrict__ lw, GridDesc g,
-                                                          CellBox cb, int r0, int r1, int r2, int e0,
-                                                          int e1, int e2, double* __restrict__ out) {
-  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  const long nregion = (long)e0 * e1 * e2;
-  if (wave >= nregion) return;  // uniform per wave
-  const int i2 = r2 + (int)(wave % e2);
-  const int i1 = r1 + (int)((wave / e2) % e1);
-  const int i0 = r0 + (int)(wave / ((long)e1 * e2));
-  const int b0 = i0 - 4 + (lane >> 3), b1 = i1 - 4 + (lane & 7);
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  if (b0 >= cb.lo0 && b0 < cb.lo0 + cb.c0 && b1 >= cb.lo1 && b1 < cb.lo1 + cb.c1) {
-    const int zl = max(i2 - 4, cb.lo2), zh = min(i2 + 3, cb.lo2 + cb.c2 - 1);
-    if (zl <= zh) {
-      const int row = ((b0 - cb.lo0) * cb.c1 + (b1 - cb.lo1)) * cb.c2 - cb.lo2;
-      const int s = cell_start[row + zl], e = cell_start[row + zh + 1];
-      const int j0 = i0 - b0 + 3, j1 = i1 - b1 + 3;
-      for (int node = s; node < e; ++node) {  // nodes are numbered in base-cell order
-        const int j2 = i2 - base[3 * node + 2] + 3;
-        const double* l = lw + (long)node * 24;
-        const double w = (l[j0] * l[8 + j1]) * l[16 + j2];
-        for (int f = 0; f < 4; ++f)
-          if (f < nf) acc[f] += x[f * m + node] * w;
+                                                          const double* __restrict__ lw, CellBox cb,
+                                                          int ez, double* __restrict__ z) {
+  const int col = blockIdx.x;
+  const int t = threadIdx.x;
+  const int j0 = t >> 3, j1 = t & 7;
+  const long ncols = (long)cb.c0 * cb.c1;
+  const int s = cell_start[(long)col * cb.c2], e = cell_start[(long)(col + 1) * cb.c2];
+  double acc[4][8];
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[f][k] = 0.0;
+  int cur = 0;
+  double* zc = z + ((long)col * 64 + t) * ez;
+  const long fstride = ncols * 64 * ez;
+  // Node records are staged through LDS 64 at a time (one per thread, independent loads), so the
+  // serial walk reads LDS broadcasts instead of waiting on a global load per node.
+  __shared__ double s_lw[64][25];
+  __shared__ double s_x[4][64];
+  __shared__ int s_b2[64];
+  for (int chunk = s; chunk < e; chunk += 64) {
+    const int cnt = min(64, e - chunk);
+    __syncthreads();
+    const double* lc = lw + (long)chunk * 24;
+#pragma unroll
+    for (int q = 0; q < 24; ++q) {
+      const int idx = q * 64 + t;  // coalesced over the chunk's contiguous 24-double records
+      const int r = idx / 24;
+      if (r < cnt) s_lw[r][idx - 24 * r] = lc[idx];
+    }
+    if (t < cnt) {
+      const int node = chunk + t;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) s_x[f][t] = (f < nf) ? x[f * m + node] : 0.0;
+      s_b2[t] = base[3 * node + 2] - cb.lo2;
+    }
+    __syncthreads();
+    for (int u = 0; u < cnt; ++u) {
+      const int b2 = s_b2[u];
+      while (cur < b2) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          if (f < nf) zc[f * fstride + cur] = acc[f][0];
+#pragma unroll
+          for (int k = 0; k < 7; ++k) acc[f][k] = acc[f][k + 1];
+          acc[f][7] = 0.0;
+        }
+        ++cur;
+      }
+      const double wxy = s_lw[u][j0] * s_lw[u][8 + j1];
+      double xv[4];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) xv[f] = s_x[f][u];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const double w = wxy * s_lw[u][16 + k];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) acc[f][k] += xv[f] * w;
       }
     }
   }
-  for (int f = 0; f < 4; ++f) {
-    double v = acc[f];
+  while (cur < ez) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    acc[f] = v;
+    for (int f = 0; f < 4; ++f) {
+      if (f < nf) zc[f * fstride + cur] = acc[f][0];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) acc[f][k] = acc[f][k + 1];
+      acc[f][7] = 0.0;
+    }
+    ++cur;
   }
-  if (lane == 0) {
-    const long n = (long)g.n0 * g.n1 * g.n2;
-    const long p = ((long)i0 * g.n1 + i1) * g.n2 + i2;
-    for (int f = 0; f < nf; ++f) out[f * n + p] = acc[f] + out[f * n + p];
+}
+
+// Stage 2: out[f][p] = add[f][p] + sum_{j0, j1} Z[f][col(i0 - j0 + 3, i1 - j1 + 3)][8 j0 + j1][iz]
+// (j0 outer, j1 inner), or add / 0 outside the region the nodes reach.
+__global__ void lift_combine_kernel(const double* __restrict__ z, int nf, CellBox cb, int ez, GridDesc g,
+                                    const double* __restrict__ add, double* __restrict__ out) {
+  const long n = (long)g.n0 * g.n1 * g.n2;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i2 = (int)(p % g.n2);
+  const int i1 = (int)((p / g.n2) % g.n1);
+  const int i0 = (int)(p / ((long)g.n1 * g.n2));
+  const int iz = i2 - (cb.lo2 - 3);
+  const long ncols = (long)cb.c0 * cb.c1;
+  const long fstride = ncols * 64 * ez;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (iz >= 0 && iz < ez) {
+    for (int j0 = 0; j0 < 8; ++j0) {
+      const int b0 = i0 - j0 + 3 - cb.lo0;
+      if (b0 < 0 || b0 >= cb.c0) continue;
+      for (int j1 = 0; j1 < 8; ++j1) {
+        const int b1 = i1 - j1 + 3 - cb.lo1;
+        if (b1 < 0 || b1 >= cb.c1) continue;
+        const long idx = (((long)b0 * cb.c1 + b1) * 64 + j0 * 8 + j1) * ez + iz;
+        for (int f = 0; f < 4; ++f)
+          if (f < nf) acc[f] += z[f * fstride + idx];
+      }
+    }
   }
+  for (int f = 0; f < nf; ++f) out[f * n + p] = add ? acc[f] + add[f * n + p] : acc[f];
 }
 
 template <int P>
@@ -545,11 +604,19 @@ extern "C" int cdft_interpolate_f64(const double* field, int nf, const int* base
 }
 
 // cell_box (host): lo0, lo1, lo2, c0, c1, c2. add may be NULL.
-extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* cell_start,
-                             const int* base, const double* lw,
-                             const int* shape, double h, const double* origin, const int* cell_box,
-                             const double* add, double* out, void* stream) {
-  if (!x || !cell_start || !base || !lw || !out || !cell_box || nf < 1 || nf > 4)
+// Workspace doubles the transpose needs for a cell box of extents (c0, c1, c2) and nf fields.
+extern "C" long cdft_lift_workspace_doubles(const int* cell_box, int nf) {
+  if (!cell_box) return 0;
+  return (long)nf * cell_box[3] * cell_box[4] * 64 * (cell_box[5] + 7);
+}
+
+// cell_box (host): lo0, lo1, lo2, c0, c1, c2; cell_start: c0*c1*c2 + 1 node offsets (nodes numbered
+// in base-cell order); work: cdft_lift_workspace_doubles doubles. add may be NULL.
+extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* cell_start, const int* base,
+                             const double* lw, const int* shape, double h, const double* origin,
+                             const int* cell_box, const double* add, double* work, long work_len,
+                             double* out, void* stream) {
+  if (!x || !cell_start || !base || !lw || !out || !cell_box || !work || nf < 1 || nf > 4)
     return fail(CDFT_E_ARG, "lift: bad arguments (nf=%d)", nf);
   GridDesc g = make_grid(shape, h, origin);
   CellBox cb;
@@ -559,21 +626,18 @@ extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* cell_st
   cb.c0 = cell_box[3];
   cb.c1 = cell_box[4];
   cb.c2 = cell_box[5];
+  const int ez = cb.c2 + 7;
+  if (work_len < cdft_lift_workspace_doubles(cell_box, nf))
+    return fail(CDFT_E_CAPACITY, "lift: workspace holds %ld doubles, needs %ld", work_len,
+                cdft_lift_workspace_doubles(cell_box, nf));
   const long n = (long)g.n0 * g.n1 * g.n2;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(lift_fill_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, add, nf, n, out);
-  int rc = check_launch("lift_fill");
+  hipLaunchKernelGGL(lift_columns_kernel, dim3((unsigned)(cb.c0 * cb.c1)), dim3(64), 0, s, x, nf, m, cell_start,
+                     base, lw, cb, ez, work);
+  int rc = check_launch("lift_columns");
   if (rc) return rc;
-  // Grid points reachable from a node whose base cell lies in the cell box: [lo-3, lo+c-1+4].
-  const int r0 = max(cb.lo0 - 3, 0), r1 = max(cb.lo1 - 3, 0), r2 = max(cb.lo2 - 3, 0);
-  const int h0 = min(cb.lo0 + cb.c0 + 3, g.n0 - 1), h1 = min(cb.lo1 + cb.c1 + 3, g.n1 - 1),
-            h2 = min(cb.lo2 + cb.c2 + 3, g.n2 - 1);
-  const int e0 = h0 - r0 + 1, e1 = h1 - r1 + 1, e2 = h2 - r2 + 1;
-  if (e0 <= 0 || e1 <= 0 || e2 <= 0) return CDFT_OK;
-  const long waves = (long)e0 * e1 * e2;
-  hipLaunchKernelGGL(lift_region_kernel, dim3(blocks_for(waves * 64, 256)), dim3(256), 0, s, x, nf, m, cell_start,
-                     base, lw, g, cb, r0, r1, r2, e0, e1, e2, out);
-  return check_launch("lift_region");
+  hipLaunchKernelGGL(lift_combine_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, work, nf, cb, ez, g, add, out);
+  return check_launch("lift_combine");
 }
 
 // coef: DEVICE array of the 2P+1 central first-derivative weights divided by h.

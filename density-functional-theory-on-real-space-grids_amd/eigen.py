@@ -1,9 +1,15 @@
-"""Rayleigh-Ritz and Chebyshev-filtered subspace iteration on the GPU, float64 throughout.
+"""Rayleigh-Ritz and Chebyshev-filtered subspace iteration on the GPU.
 
 Mirrors ``src/cdft/eigen/rayleigh_ritz.py:51-115`` and ``src/cdft/eigen/chefsi.py:442-585``:
 Cholesky orthonormalisation in the operator's measure, the projected k x k problem, rotation,
-residual norms, and the outer loop with one host read per iteration. Every block-sized operation
-is a HIP kernel (K1-K4); the k x k algebra runs in one-thread kernels on the device.
+residual norms, and the outer loop with one host read per iteration. Every block-sized operation is
+a HIP kernel (K1-K4); the k x k algebra runs in one-thread kernels on the device.
+
+One CheFSI iteration -- the degree-d filter, Rayleigh-Ritz and the residual norms, about 35 kernel
+launches -- is captured once into a HIP graph over static buffers (the block, the recurrence table,
+the diagonal potential) and replayed every iteration, across SCF iterations too; the host only
+uploads the next recurrence table and reads the iteration's scalars. This plays the role of the
+reference's ``GraphedChebyshevFilter`` (``chefsi.py:185-392``), extended to the whole iteration.
 """
 
 from __future__ import annotations
@@ -56,9 +62,9 @@ class SpectralBoundsHint:
 
 
 class RitzWork:
-    """Device buffers of one Rayleigh-Ritz step for a block of k vectors (allocated once)."""
+    """Device buffers of one CheFSI iteration for a block of k vectors (allocated once)."""
 
-    def __init__(self, k: int, n: int, device):
+    def __init__(self, k: int, n: int, degree: int, device, mixed: bool):
         f = dict(dtype=torch.float64, device=device)
         self.k = k
         self.S = torch.empty((k, k), **f)
@@ -67,14 +73,22 @@ class RitzWork:
         self.C = torch.empty((k, k), **f)
         self.G = torch.empty((k, k), **f)
         self.flag = torch.zeros(1, dtype=torch.int32, device=device)
-        # scalars read once per iteration: evals (k) | residual norms (k) | orth error (1) | flag (1)
-        self.scalars = torch.zeros(2 * k + 2, **f)
+        # scalars read once per iteration: evals (k) | residual norms (k) | orth error (1)
+        self.scalars = torch.zeros(2 * k + 1, **f)
         self.basis = torch.empty((k, n), **f)
         self.applied = torch.empty((k, n), **f)
         self.vectors = torch.empty((k, n), **f)
         self.filtered = torch.empty((k, n), **f)
         self.work1 = torch.empty((k, n), **f)
         self.work2 = torch.empty((k, n), **f)
+        self.table = torch.zeros((degree, 3), **f)
+        self.table_host = torch.zeros((degree, 3), dtype=torch.float64).pin_memory()
+        if mixed:
+            g = dict(dtype=torch.float32, device=device)
+            self.x32 = torch.empty((k, n), **g)
+            self.y32 = torch.empty((k, n), **g)
+            self.a32 = torch.empty((k, n), **g)
+            self.b32 = torch.empty((k, n), **g)
 
     @property
     def evals(self) -> torch.Tensor:
@@ -85,28 +99,27 @@ class RitzWork:
         return self.scalars[self.k: 2 * self.k]
 
 
-def rayleigh_ritz(hamiltonian, block: torch.Tensor, work: RitzWork) -> None:
-    """Orthonormalise ``block`` (Cholesky, ``T = L^-1``), project, diagonalise, rotate; leaves the
-    eigenvalues in ``work.evals``, the Ritz vectors in ``work.vectors`` and ``max|G - I|`` in
-    ``work.scalars[2k]``. A non-positive-definite Gram matrix sets ``work.flag``."""
-    m = hamiltonian.measure
+def rayleigh_ritz(geometry, measure, block: torch.Tensor, work: RitzWork) -> None:
+    """Orthonormalise ``block`` (Cholesky, ``T = L^-1``), project, diagonalise, rotate: eigenvalues
+    in ``work.evals``, Ritz vectors in ``work.vectors``, ``max|G - I|`` in ``work.scalars[2k]``; a
+    non-positive-definite Gram matrix sets ``work.flag``."""
     k = work.k
-    m.gram(block, out=work.S)
+    measure.gram(block, out=work.S)
     ops.chol_inv(work.S, work.T, work.flag)
     ops.rotate(work.T, block, out=work.basis)
-    hamiltonian.apply(work.basis, work.applied)
-    m.cross(work.basis, work.applied, out=work.M)
+    ops.hdiv_apply(work.basis, geometry, work.applied)
+    measure.cross(work.basis, work.applied, out=work.M)
     ops.eigh(work.M, work.evals, work.C)
     ops.rotate(work.C, work.basis, transpose=True, out=work.vectors)
-    m.gram(work.vectors, out=work.G)
+    measure.gram(work.vectors, out=work.G)
     ops.orth_error(work.G, work.scalars[2 * k: 2 * k + 1])
 
 
-def residual_norms(hamiltonian, work: RitzWork) -> None:
-    """``||A phi_i - eps_i phi_i||`` in the measure, into ``work.norms`` (fresh apply, as the
+def residual_norms(geometry, measure, work: RitzWork) -> None:
+    """``||A phi_i - eps_i phi_i||`` in the measure into ``work.norms`` (a fresh apply, as the
     reference's ``residual_norms``)."""
-    hamiltonian.apply(work.vectors, work.applied)
-    ops.resnorm(work.applied, work.vectors, work.evals, hamiltonian.measure.q, out=work.norms)
+    ops.hdiv_apply(work.vectors, geometry, work.applied)
+    ops.resnorm(work.applied, work.vectors, work.evals, measure.q, out=work.norms)
 
 
 def _read(work: RitzWork) -> tuple[np.ndarray, np.ndarray, float]:
@@ -126,7 +139,7 @@ def _fraction(value, low, high):
 
 
 class ChebyshevFilteredSubspace:
-    """CheFSI (reference ``eigen/chefsi.py:408-585``) on the device."""
+    """CheFSI (reference ``eigen/chefsi.py:408-585``) on the device, one HIP graph per iteration."""
 
     #: Mixed precision: filter in float32 while the worst residual is above this...
     switch_residual: float = 1.0e-5
@@ -134,40 +147,106 @@ class ChebyshevFilteredSubspace:
     fp64_tail: int = 1
 
     def __init__(self, grid, config: EigenConfig, ritz_stop: bool = True, filter_dtype=torch.float64,
-                 filter_hook=None):
+                 filter_hook=None, use_graphs: bool = True):
         self.grid = grid
         self.config = config
         self.ritz_stop = ritz_stop
         self.filter_dtype = filter_dtype
         self.filter_hook = filter_hook
+        self.use_graphs = use_graphs
+        self.graph_record: dict = {"captures": 0, "replays": 0, "disabled": None}
         self.history: list[dict] = []
         self.filter_precisions: list[str] = []
         self._work: dict[int, RitzWork] = {}
+        self._graphs: dict[tuple, torch.cuda.CUDAGraph] = {}
+        n = grid.n_points
+        self._pot = torch.empty(n, dtype=torch.float64, device=grid.device)
+        self._pot32 = torch.empty(n, dtype=torch.float32, device=grid.device) if filter_dtype != torch.float64 else None
+        self._geom_key = None
+        self._geom64 = self._geom32 = None
+        self._measure = None
+
+    @property
+    def mixed(self) -> bool:
+        return self.filter_dtype != torch.float64
 
     def work_for(self, k: int) -> RitzWork:
         if k not in self._work:
-            self._work[k] = RitzWork(k, self.grid.n_points, self.grid.device)
+            self._work[k] = RitzWork(k, self.grid.n_points, self.config.chebyshev_degree, self.grid.device, self.mixed)
         return self._work[k]
 
-    def filter(self, hamiltonian, block: torch.Tensor, out: torch.Tensor, work: RitzWork, lower, upper, smin,
-               dtype=torch.float64):
-        rows = chebyshev_coefficients(self.config.chebyshev_degree, lower, upper, smin)
-        table = torch.tensor(rows, dtype=torch.float64).pin_memory().to(block.device, non_blocking=True)
-        if self.filter_hook is not None:
-            self.filter_hook("start", self, block if dtype == torch.float64 else block.to(dtype), len(rows))
-        if dtype == torch.float64:
-            ops.cheb_filter(block, hamiltonian.geometry, table, len(rows), out=out, work=(work.work1, work.work2))
-            self.filter_precisions.append("f64")
+    def _bind(self, hamiltonian) -> None:
+        """Point the solver's static geometry at this operator: copy its diagonal potential into the
+        static buffer the captured graphs read (the face weights, ``invw`` and ``q`` are
+        geometry-only and shared by every operator of a solve)."""
+        geom = hamiltonian.geometry
+        key = (geom.w0.data_ptr(), geom.w1.data_ptr(), geom.w2.data_ptr(), geom.invw.data_ptr(),
+               hamiltonian.measure.q.data_ptr())
+        if key != self._geom_key:
+            self._geom_key = key
+            self._graphs.clear()
+            self._geom64 = geom.with_potential(self._pot)
+            if self.mixed:
+                self._geom32 = hamiltonian.geometry_in(self.filter_dtype).with_potential(self._pot32)
+            self._measure = hamiltonian.measure
+        self._pot.copy_(hamiltonian.potential)
+        if self.mixed:
+            self._pot32.copy_(self._pot)
+
+    # ---- one iteration ----
+
+    def _filter_into(self, work: RitzWork, use32: bool) -> None:
+        degree = self.config.chebyshev_degree
+        if not use32:
+            ops.cheb_filter(work.vectors, self._geom64, work.table, degree, out=work.filtered,
+                            work=(work.work1, work.work2))
         else:
-            g32 = hamiltonian.geometry_in(dtype)
-            x32 = block.to(dtype)
-            y32 = ops.cheb_filter(x32, g32, table, len(rows))
-            out.copy_(y32)
-            self.filter_precisions.append("f32")
+            work.x32.copy_(work.vectors)
+            ops.cheb_filter(work.x32, self._geom32, work.table, degree, out=work.y32, work=(work.a32, work.b32))
+            work.filtered.copy_(work.y32)
+
+    def _ritz(self, work: RitzWork) -> None:
+        rayleigh_ritz(self._geom64, self._measure, work.filtered, work)
+        residual_norms(self._geom64, self._measure, work)
+
+    def _replay(self, key: tuple, body) -> None:
+        """Run ``body`` through its captured graph (captured on first use), or eagerly when capture
+        is off or failed (the reason is recorded)."""
+        if self.use_graphs and self.graph_record["disabled"] is None:
+            graph = self._graphs.get(key)
+            if graph is None:
+                try:
+                    graph = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(graph):
+                        body()
+                    self._graphs[key] = graph
+                    self.graph_record["captures"] += 1
+                except Exception as exc:  # noqa: BLE001 - recorded; the iteration then runs eagerly
+                    self.graph_record["disabled"] = f"capture failed: {type(exc).__name__}: {exc}"
+                    graph = None
+            if graph is not None:
+                graph.replay()
+                self.graph_record["replays"] += 1
+                return
+        body()
+
+    def _run_iteration(self, work: RitzWork, use32: bool, degree: int) -> None:
+        """Filter, then Rayleigh-Ritz and residuals: two captured graphs, with the timing hook's
+        events (if any) around the filter graph on the same stream."""
         if self.filter_hook is not None:
-            self.filter_hook("end", self, block if dtype == torch.float64 else x32, len(rows))
-        hamiltonian.note_applications(len(rows))
-        return out
+            self.filter_hook("start", self, work.x32 if use32 else work.vectors, degree)
+        self._replay(("filter", work.k, use32), lambda: self._filter_into(work, use32))
+        if self.filter_hook is not None:
+            self.filter_hook("end", self, work.x32 if use32 else work.vectors, degree)
+        self._replay(("ritz", work.k), lambda: self._ritz(work))
+
+    def _upload_table(self, work: RitzWork, lower, upper, smin) -> int:
+        rows = chebyshev_coefficients(self.config.chebyshev_degree, lower, upper, smin)
+        work.table_host.numpy()[:] = np.asarray(rows)
+        work.table.copy_(work.table_host, non_blocking=True)
+        return len(rows)
+
+    # ---- the solver ----
 
     def solve(self, hamiltonian, n_states: int, initial: torch.Tensor | None = None, generator=None,
               bounds: SpectralBoundsHint | None = None) -> EigenResult:
@@ -184,6 +263,7 @@ class ChebyshevFilteredSubspace:
                 block = torch.cat([block, pad], dim=0)
         block = block.contiguous()
         work = self.work_for(size)
+        self._bind(hamiltonian)
 
         def lanczos():
             return hamiltonian.spectral_bounds(steps=cfg.lanczos_steps, safety=cfg.bound_safety, generator=generator)
@@ -192,8 +272,9 @@ class ChebyshevFilteredSubspace:
         if bounds is None:
             spectrum_min, spectrum_max = lanczos()
             source = "lanczos"
-        rayleigh_ritz(hamiltonian, block, work)
-        residual_norms(hamiltonian, work)
+        rayleigh_ritz(self._geom64, self._measure, block, work)
+        residual_norms(self._geom64, self._measure, work)
+        hamiltonian.note_applications(2)
         evals, norms, _ = _read(work)
         worst = float(norms[:n_states].max())
         previous = evals.copy()
@@ -219,21 +300,18 @@ class ChebyshevFilteredSubspace:
                 fraction = _fraction(lower, spectrum_min, spectrum_max)
             top_fraction = max(top_fraction, fraction)
             lower = min(max(lower, spectrum_min + 1e-8), spectrum_max - 1e-8)
-            # The Ritz vectors of the previous step are the filter input.
-            src = work.vectors
-            dtype = torch.float64
-            if self.filter_dtype != torch.float64 and worst > self.switch_residual and \
-                    iteration <= cfg.max_iterations - self.fp64_tail:
-                dtype = self.filter_dtype
-            self.filter(hamiltonian, src, work.filtered, work, lower, spectrum_max, spectrum_min, dtype)
-            rayleigh_ritz(hamiltonian, work.filtered, work)
-            residual_norms(hamiltonian, work)
+            use32 = self.mixed and worst > self.switch_residual and iteration <= cfg.max_iterations - self.fp64_tail
+            degree = self._upload_table(work, lower, spectrum_max, spectrum_min)
+            self.filter_precisions.append("f32" if use32 else "f64")
+            self._run_iteration(work, use32, degree)
+            hamiltonian.note_applications(degree + 2)
             evals, norms, orth = _read(work)
             worst = float(norms[:n_states].max())
             drift = float(np.abs(evals[:n_states] - previous[:n_states]).max())
             next_lower = float(evals[min(size, evals.shape[0]) - 1])
             previous = evals.copy()
-            self.history.append({"iteration": iteration, "residual": worst, "orthonormality": orth, "eigenvalue_drift": drift})
+            self.history.append({"iteration": iteration, "residual": worst, "orthonormality": orth,
+                                 "eigenvalue_drift": drift})
             if worst < cfg.residual_tol:
                 converged = True
                 stop_reason = "residual below residual_tol"
@@ -244,7 +322,8 @@ class ChebyshevFilteredSubspace:
                 stop_reason = "Ritz values stable on the weak-form path"
                 break
         info = {"spectral_bounds": (spectrum_min, spectrum_max), "bounds_source": source, "bounds_shift": shift_used,
-                "top_ritz_fraction": top_fraction, "stop_reason": stop_reason, "eigenvalue_drift": drift}
+                "top_ritz_fraction": top_fraction, "stop_reason": stop_reason, "eigenvalue_drift": drift,
+                "graphs": dict(self.graph_record)}
         vectors = work.vectors[:n_states].clone()
         return EigenResult(evals[:n_states], vectors, norms[:n_states], iteration, converged, info,
                            eigenvalues_device=work.evals[:n_states].clone())

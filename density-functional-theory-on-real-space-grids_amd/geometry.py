@@ -351,10 +351,13 @@ class CuspQuadrature:
         x = node_values.reshape(-1, self.n_nodes)
         nf = x.shape[0]
         out = torch.empty((nf, self.grid.n_points), dtype=torch.float64, device=self.grid.device) if out is None else out
+        if getattr(self, "_lift_work", None) is None:
+            need = int(_lib.load().cdft_lift_workspace_doubles(ctypes.addressof(self._cell_box), 4))
+            self._lift_work = torch.empty(need, dtype=torch.float64, device=self.grid.device)
         _lib.call("cdft_lift_f64", x.data_ptr(), nf, self.n_nodes, self.cell_start.data_ptr(),
                   self.base.data_ptr(), self.lw.data_ptr(), self.grid.c_shape,
                   self.grid.spacing, self.grid.c_origin, ctypes.addressof(self._cell_box), _ptr_or_none(add),
-                  out.data_ptr(), stream_of(x))
+                  self._lift_work.data_ptr(), self._lift_work.numel(), out.data_ptr(), stream_of(x))
         return out.reshape(-1) if node_values.dim() == 1 else out
 
     def interpolate_at(self, points: np.ndarray, field: torch.Tensor) -> torch.Tensor:

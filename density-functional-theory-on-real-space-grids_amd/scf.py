@@ -455,6 +455,7 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
         "applications": applications,
         "fallback_events": events,
         "filter_precisions": list(solver.filter_precisions),
+        "graphs": dict(solver.graph_record),
         "wall_seconds": time.perf_counter() - started,
         "step": step,
     }
