@@ -5,196 +5,335 @@
 //   F_d(n)    = sum_j c_j phi(n+1+j)                        (face n sits at x_n + h/2)
 //
 // with c_k the staggered order-2P weights (already divided by h), a zero halo, and the face weights
-// w_d stored on the box extended by 2P along axis d, indexed from face -P. Blocks are 4x16x16
-// output tiles (x slowest, z contiguous). Each thread owns one (y, z) column of 4 x-planes: the
-// x-faces come from a register pencil loaded straight from global memory (the x halo never enters
-// LDS), the y- and z-faces from an LDS plane tile with a +-(2P-1) halo. Weighted faces are computed
-// once per face and shared through LDS.
+// w_d stored on the box extended by 2P along axis d, indexed from face -P.
+//
+// Layout: C order, x slowest, z contiguous. A block owns a (y, z) tile of at most 16 x 16 columns
+// and marches along x through a segment of planes. Each thread keeps its column's x-direction state
+// in registers -- the 2P values phi(m .. m+2P-1) and the 2P weighted x-faces F(m-P .. m+P-1) -- so
+// the x stencil costs no LDS traffic and each phi value is read from memory once per segment (plus
+// a 4P-2 plane warm-up). The y and z stencils of plane m go through an LDS plane tile with a
+// +-(2P-1) halo; weighted y/z faces are computed once per face and shared through LDS. The global
+// loads of plane m+1 are issued before the LDS work of plane m, so their latency overlaps it.
+// Tile extents are chosen per grid (ceil(n / ceil(n / 16))), so n = 65 runs 13-wide tiles rather
+// than four full tiles and a one-column tail.
 #include "common.h"
 
 namespace cdft {
 
-constexpr int TX = 4, TY = 16, TZ = 16, NT = TY * TZ;
+constexpr int TMAX = 16;   // largest tile extent along y and z
+constexpr int MAXH = 2;    // halo values staged per thread per plane
+constexpr int NTMAX = 256; // largest block
 
 template <int P>
 struct Taps {
   double c[2 * P];  // c_k / h, k = -P .. P-1
 };
 
-// Epilogue modes: 0 = A phi; 1 = (A y - c y) a; 2 = (A y - c y) a - b prev.
-//
-// One block = one 4x16x16 tile of one vector (blockIdx.y is the vector). Every global load of a
-// thread is independent and issued before the first barrier: its 4+2(2P-1) x-pencil, four y-halo
-// values (threads with ty < 2P-1... 14 rows x 16 columns x 4 planes) and four z-halo values, plus
-// the face weights. LDS holds the cross-shaped plane tiles (corners unused) and the weighted faces.
-template <typename T, int P, int MODE>
-__global__ __launch_bounds__(NT) void hdiv_kernel(
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, int bytes) {
+  // 0x00020000: the raw (untyped, stride 0) buffer format word of gfx9-family descriptors.
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+template <typename T>
+__device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ void bstore<double>(double v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned int, v), r, voff, soff, 0);
+}
+template <>
+__device__ __forceinline__ void bstore<float>(float v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
+}
+
+// Launch shape of the marching kernel (host-computed; see march_dims).
+struct March {
+  int ty, tz;            // tile extents (<= TMAX)
+  int tiles_y, tiles_z;  // tiles per plane
+  int seg, nseg;         // longest segment (planes), segments along x
+  unsigned tiles;        // tiles_y * tiles_z * nseg
+  int threads;           // block size: a multiple of 64 covering the tile and the halo staging
+};
+
+// Epilogue modes: 0 = A phi; 1 = (A y - c y) a; 2 = (A y - c y) a - b prev, where (c, a, b) is row
+// `step` of the Chebyshev coefficient table. TZ is the tile extent along z (compile-time, so the LDS
+// pitches below are constants and neighbouring reads pair into ds_read2). Threads are numbered
+// row-major over the tile and every LDS row pitch is congruent to TZ modulo 16 doubles, so any 16
+// consecutive lanes touch 16 consecutive bank pairs: the plane-tile, y-face and z-face accesses are
+// free of bank conflicts. Consecutive blocks take the vectors of one tile, so the tile's geometry
+// is reused from L2 by all of them.
+template <typename T, int P, int MODE, int TZ>
+__global__ __launch_bounds__(NTMAX) void hdiv_kernel(
     const T* __restrict__ phi, const T* __restrict__ prev, T* __restrict__ out,
     const T* __restrict__ w0, const T* __restrict__ w1, const T* __restrict__ w2,
-    const T* __restrict__ invw, const T* __restrict__ pot, int k, int n0, int n1, int n2,
-    Taps<P> taps, const double* __restrict__ table, int step, int tiles_z, int tiles_y,
-    int tiles_x) {
+    const T* __restrict__ invw, const T* __restrict__ pot, int n0, int n1, int n2, int k,
+    Taps<P> taps, const double* __restrict__ table, int step, March md) {
   constexpr int R = 2 * P - 1;  // stencil reach: 2P-1 points each side
-  constexpr int LY = TY + 2 * R, LZ = TZ + 2 * R;
-  static_assert(2 * R <= TY && 2 * R <= TZ, "halo rows must fit one pass of the block");
-  __shared__ T sphi[TX][LY][LZ];
-  __shared__ T syf[TX][TY + R][TZ];
-  __shared__ T szf[TX][TY][TZ + R];
+  static_assert(2 * R <= 16, "halo must fit the 16-double pitch pad");
+  constexpr int LZ = TZ + 16;   // plane-tile pitch (>= TZ + 2R)
+  constexpr int YP = TZ;        // y-face pitch
+  constexpr int ZP = TZ + 16;   // z-face pitch (>= TZ + R)
+  __shared__ T s_phi[(TMAX + 2 * R) * LZ];
+  __shared__ T s_yf[(TMAX + R) * YP];
+  __shared__ T s_zf[TMAX * ZP];
 
-  const unsigned ntiles = (unsigned)tiles_z * tiles_y * tiles_x;
-  const unsigned tile = xcd_tile(blockIdx.x, ntiles);
-  const int v = blockIdx.y;
-  const int bz = tile % tiles_z;
-  const int by = (tile / tiles_z) % tiles_y;
-  const int bx = tile / (tiles_z * tiles_y);
-  const int X0 = bx * TX, Y0 = by * TY, Z0 = bz * TZ;
-  const int tid = threadIdx.x;
-  const int tz = tid % TZ, ty = tid / TZ;
+  const int TY = md.ty;
+  const unsigned lin = xcd_tile(blockIdx.x, gridDim.x);
+  const unsigned tile = lin / (unsigned)k;
+  const int v = (int)(lin - tile * (unsigned)k);
+  const int bz = (int)(tile % md.tiles_z);
+  const int by = (int)((tile / md.tiles_z) % md.tiles_y);
+  const int bs = (int)(tile / ((unsigned)md.tiles_z * md.tiles_y));
+  // Segments split the planes evenly: segment s covers [s n0 / nseg, (s + 1) n0 / nseg).
+  const int X0 = (int)((long)bs * n0 / md.nseg), X1 = (int)((long)(bs + 1) * n0 / md.nseg);
+  const int Y0 = by * TY, Z0 = bz * TZ;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const bool act = t < TY * TZ;
+  const int ty = act ? t / TZ : 0, tz = act ? t - (t / TZ) * TZ : 0;
   const int y = Y0 + ty, z = Z0 + tz;
-  const bool yz_in = (y < n1) && (z < n2);
-  const long plane = (long)n1 * n2;
-  const long N = plane * n0;
+  const bool own = act && y < n1 && z < n2;
+  // All loads are raw buffer loads: a per-array descriptor (base, size), a per-thread byte offset
+  // that is the same in every plane, and the plane's byte offset as a uniform scalar. Lanes with
+  // nothing to load (outside the grid or the tile) use an offset past the end of the buffer, which
+  // the hardware returns as zero. Offsets are 32-bit (the host checks every array fits).
+  constexpr int S = (int)sizeof(T);
+  constexpr int NONE = (int)0x80000000;
+  const int plane = n1 * n2;
   const int n1f = n1 + 2 * P, n2f = n2 + 2 * P;
-  const T* ph = phi + (long)v * N;
+  const int pl1 = n1f * n2, pl2 = n1 * n2f;
+  const long vbase = (long)v * plane * n0;
+  const int vbytes = plane * n0 * S;
+  const auto r_phi = make_rsrc(phi + vbase, vbytes);
+  const auto r_prev = make_rsrc(MODE == 2 ? prev + vbase : phi, vbytes);
+  const auto r_out = make_rsrc(out + vbase, vbytes);
+  const auto r_w0 = make_rsrc(w0, (n0 + 2 * P) * plane * S);
+  const auto r_w1 = make_rsrc(w1, n0 * pl1 * S);
+  const auto r_w2 = make_rsrc(w2, n0 * pl2 * S);
+  const auto r_iw = make_rsrc(invw, n0 * plane * S);
+  const auto r_pt = make_rsrc(pot, n0 * plane * S);
+  const int col = own ? (y * n2 + z) * S : NONE;
 
   T c[2 * P];
 #pragma unroll
   for (int i = 0; i < 2 * P; ++i) c[i] = (T)taps.c[i];
-
-  // ---- global loads (all independent) ----
-  T px[TX + 2 * R];
-#pragma unroll
-  for (int i = 0; i < TX + 2 * R; ++i) {
-    const int gx = X0 - R + i;
-    px[i] = (yz_in && gx >= 0 && gx < n0) ? ph[gx * plane + (long)y * n2 + z] : T(0);
-  }
-  // y halo: rows Y0-R .. Y0-1 (hy < R) and Y0+TY .. Y0+TY+R-1 (hy >= R), column z.
-  T hyv[TX];
-  const bool do_hy = ty < 2 * R;
-  const int hy_row = ty < R ? Y0 - R + ty : Y0 + TY + (ty - R);
-  const int hy_l = ty < R ? ty : TY + ty;  // local LDS row
-#pragma unroll
-  for (int p = 0; p < TX; ++p) {
-    const int gx = X0 + p;
-    hyv[p] = (do_hy && gx < n0 && z < n2 && hy_row >= 0 && hy_row < n1) ? ph[gx * plane + (long)hy_row * n2 + z] : T(0);
-  }
-  // z halo: columns Z0-R .. Z0-1 and Z0+TZ .. Z0+TZ+R-1, row y.
-  T hzv[TX];
-  const bool do_hz = tz < 2 * R;
-  const int hz_col = tz < R ? Z0 - R + tz : Z0 + TZ + (tz - R);
-  const int hz_l = tz < R ? tz : TZ + tz;
-#pragma unroll
-  for (int p = 0; p < TX; ++p) {
-    const int gx = X0 + p;
-    hzv[p] = (do_hz && gx < n0 && y < n1 && hz_col >= 0 && hz_col < n2) ? ph[gx * plane + (long)y * n2 + hz_col] : T(0);
-  }
-  // Face weights this thread applies: x faces of its pencil, y faces (rows fy = ty, ty+TY), z faces.
-  T wx[TX + R];
-#pragma unroll
-  for (int fx = 0; fx < TX + R; ++fx) {
-    const int a = X0 + fx;
-    wx[fx] = (yz_in && a < n0 + 2 * P) ? w0[((long)a * n1 + y) * n2 + z] : T(0);
-  }
-  T wy0[TX], wy1[TX], wz0[TX], wz1[TX];
-#pragma unroll
-  for (int p = 0; p < TX; ++p) {
-    const int gx = X0 + p;
-    const bool xin = gx < n0;
-    const int ay0 = Y0 + ty, ay1 = Y0 + ty + TY;
-    wy0[p] = (xin && z < n2 && ay0 < n1f) ? w1[((long)gx * n1f + ay0) * n2 + z] : T(0);
-    wy1[p] = (xin && z < n2 && ty < R && ay1 < n1f) ? w1[((long)gx * n1f + ay1) * n2 + z] : T(0);
-    const int az0 = Z0 + tz, az1 = Z0 + tz + TZ;
-    wz0[p] = (xin && y < n1 && az0 < n2f) ? w2[((long)gx * n1 + y) * n2f + az0] : T(0);
-    wz1[p] = (xin && y < n1 && tz < R && az1 < n2f) ? w2[((long)gx * n1 + y) * n2f + az1] : T(0);
-  }
-
-  // ---- stage the plane tiles ----
-#pragma unroll
-  for (int p = 0; p < TX; ++p) {
-    sphi[p][ty + R][tz + R] = px[R + p];
-    if (do_hy) sphi[p][hy_l][tz + R] = hyv[p];
-    if (do_hz) sphi[p][ty + R][hz_l] = hzv[p];
-  }
-  __syncthreads();
-
-  // ---- weighted faces ----
-#pragma unroll
-  for (int p = 0; p < TX; ++p) {
-    {
-      T f = 0;
-#pragma unroll
-      for (int i = 0; i < 2 * P; ++i) f += c[i] * sphi[p][ty + i][tz + R];
-      syf[p][ty][tz] = f * wy0[p];
-    }
-    if (ty < R) {
-      T f = 0;
-#pragma unroll
-      for (int i = 0; i < 2 * P; ++i) f += c[i] * sphi[p][ty + TY + i][tz + R];
-      syf[p][ty + TY][tz] = f * wy1[p];
-    }
-    {
-      T f = 0;
-#pragma unroll
-      for (int i = 0; i < 2 * P; ++i) f += c[i] * sphi[p][ty + R][tz + i];
-      szf[p][ty][tz] = f * wz0[p];
-    }
-    if (tz < R) {
-      T f = 0;
-#pragma unroll
-      for (int i = 0; i < 2 * P; ++i) f += c[i] * sphi[p][ty + R][tz + TZ + i];
-      szf[p][ty][tz + TZ] = f * wz1[p];
-    }
-  }
-  T gxf[TX + R];
-#pragma unroll
-  for (int fx = 0; fx < TX + R; ++fx) {
-    T f = 0;
-#pragma unroll
-    for (int i = 0; i < 2 * P; ++i) f += c[i] * px[fx + i];
-    gxf[fx] = f * wx[fx];
-  }
   T cc = 0, ca = 0, cb = 0;
   if (MODE >= 1) {
     cc = (T)table[3 * step];
     ca = (T)table[3 * step + 1];
     cb = (T)table[3 * step + 2];
   }
-  __syncthreads();
 
-  if (!yz_in) return;
+  // Halo slots of this thread (the same in every plane): in-plane source byte offset (NONE =
+  // outside the grid, staged as zero) and LDS destination (-1 = no slot).
+  const int nhy = 2 * R * TZ, nhalo = nhy + 2 * R * TY;
+  int hsrc[MAXH], hdst[MAXH];
 #pragma unroll
-  for (int p = 0; p < TX; ++p) {
-    const int gx = X0 + p;
-    if (gx < n0) {
-      T ax = 0, ay = 0, az = 0;
-#pragma unroll
-      for (int i = 0; i < 2 * P; ++i) ax += c[i] * gxf[p + R - i];
-#pragma unroll
-      for (int i = 0; i < 2 * P; ++i) ay += c[i] * syf[p][ty + R - i][tz];
-#pragma unroll
-      for (int i = 0; i < 2 * P; ++i) az += c[i] * szf[p][ty][tz + R - i];
-      const T lap = (ax + ay) + az;
-      const long idx = gx * plane + (long)y * n2 + z;
-      const T pv = px[R + p];
-      T val = lap * T(0.5) * invw[idx] + pv * pot[idx];
-      if (MODE == 1) val = (val - cc * pv) * ca;
-      if (MODE == 2) val = (val - cc * pv) * ca - cb * prev[(long)v * N + idx];
-      out[(long)v * N + idx] = val;
+  for (int s = 0; s < MAXH; ++s) {
+    const int h = t + s * nt;
+    hsrc[s] = NONE;
+    hdst[s] = -1;
+    if (h < nhy) {
+      const int hr = h / TZ, hc = h - (h / TZ) * TZ;
+      const int rl = hr < R ? hr : TY + hr;
+      const int gy = Y0 - R + rl, gz = Z0 + hc;
+      hdst[s] = rl * LZ + R + hc;
+      if (gy >= 0 && gy < n1 && gz < n2) hsrc[s] = (gy * n2 + gz) * S;
+    } else if (h < nhalo) {
+      const int hh = h - nhy;
+      const int r = hh / (2 * R), hc = hh - (hh / (2 * R)) * (2 * R);
+      const int cl = hc < R ? hc : TZ + hc;
+      const int gy = Y0 + r, gz = Z0 - R + cl;
+      hdst[s] = (R + r) * LZ + cl;
+      if (gy < n1 && gz >= 0 && gz < n2) hsrc[s] = (gy * n2 + gz) * S;
     }
+  }
+  const bool zin = act && z < n2, yin = act && y < n1;
+  const int ay0 = Y0 + ty, ay1 = Y0 + ty + TY;
+  const int az0 = Z0 + tz, az1 = Z0 + tz + TZ;
+  const int wy0_off = (zin && ay0 < n1f) ? (ay0 * n2 + z) * S : NONE;
+  const int wy1_off = (zin && ty < R && ay1 < n1f) ? (ay1 * n2 + z) * S : NONE;
+  const int wz0_off = (yin && az0 < n2f) ? (y * n2f + az0) * S : NONE;
+  const int wz1_off = (yin && tz < R && az1 < n2f) ? (y * n2f + az1) * S : NONE;
+
+  // ---- x warm-up: phi(X0-2P+1 .. X0+2P-1) and the faces F(X0-P .. X0+P-2) ----
+  T qphi[2 * P], qf[2 * P];
+  {
+    T pen[4 * P - 1];
+#pragma unroll
+    for (int q = 0; q < 4 * P - 1; ++q) {
+      const int gx = X0 - 2 * P + 1 + q;  // block-uniform
+      pen[q] = (gx >= 0 && gx < n0) ? bload<T>(r_phi, col, gx * plane * S) : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < 2 * P - 1; ++j) {
+      T f = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) f += c[i] * pen[j + i];
+      qf[j] = f * bload<T>(r_w0, col, (X0 + j) * plane * S);  // extended index of face X0-P+j
+    }
+    qf[2 * P - 1] = 0;
+#pragma unroll
+    for (int j = 0; j < 2 * P; ++j) qphi[j] = pen[2 * P - 1 + j];
+  }
+
+  // Per-plane operands, loaded one plane ahead (their latency overlaps the LDS work of a plane).
+  struct Plane {
+    T wxf;        // w0 at the newest x-face of the plane (extended index m + 2P - 1)
+    T phn;        // phi(m + 2P): enters the x queue after the plane
+    T hal[MAXH];  // y/z halo values
+    T wy0, wy1, wz0, wz1;
+    T iw, pt, pr;
+  };
+  auto load_plane = [&](int m, Plane& L) {
+    const int base = m * plane * S;
+    L.wxf = bload<T>(r_w0, col, (m + 2 * P - 1) * plane * S);
+    L.phn = (m + 2 * P < n0) ? bload<T>(r_phi, col, (m + 2 * P) * plane * S) : T(0);
+#pragma unroll
+    for (int s = 0; s < MAXH; ++s) L.hal[s] = bload<T>(r_phi, hsrc[s], base);
+    L.wy0 = bload<T>(r_w1, wy0_off, m * pl1 * S);
+    L.wy1 = bload<T>(r_w1, wy1_off, m * pl1 * S);
+    L.wz0 = bload<T>(r_w2, wz0_off, m * pl2 * S);
+    L.wz1 = bload<T>(r_w2, wz1_off, m * pl2 * S);
+    L.iw = bload<T>(r_iw, col, base);
+    L.pt = bload<T>(r_pt, col, base);
+    L.pr = MODE == 2 ? bload<T>(r_prev, col, base) : T(0);
+  };
+
+  Plane cur;
+  load_plane(X0, cur);
+  for (int m = X0; m < X1; ++m) {
+    Plane nxt;
+    if (m + 1 < X1)
+      load_plane(m + 1, nxt);
+    else
+      nxt = cur;
+
+    // newest x-face F(m+P-1) and the x part of the divergence
+    {
+      T f = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) f += c[i] * qphi[i];
+      qf[2 * P - 1] = f * cur.wxf;
+    }
+    T ax = 0;
+#pragma unroll
+    for (int i = 0; i < 2 * P; ++i) ax += c[i] * qf[2 * P - 1 - i];
+
+    // stage plane m
+    if (act) s_phi[(ty + R) * LZ + tz + R] = qphi[0];
+#pragma unroll
+    for (int s = 0; s < MAXH; ++s)
+      if (hdst[s] >= 0) s_phi[hdst[s]] = cur.hal[s];
+    __syncthreads();
+
+    // weighted y and z faces
+    if (act) {
+      T f = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) f += c[i] * s_phi[(ty + i) * LZ + tz + R];
+      s_yf[ty * YP + tz] = f * cur.wy0;
+      f = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) f += c[i] * s_phi[(ty + R) * LZ + tz + i];
+      s_zf[ty * ZP + tz] = f * cur.wz0;
+      if (ty < R) {
+        f = 0;
+#pragma unroll
+        for (int i = 0; i < 2 * P; ++i) f += c[i] * s_phi[(ty + TY + i) * LZ + tz + R];
+        s_yf[(ty + TY) * YP + tz] = f * cur.wy1;
+      }
+      if (tz < R) {
+        f = 0;
+#pragma unroll
+        for (int i = 0; i < 2 * P; ++i) f += c[i] * s_phi[(ty + R) * LZ + tz + TZ + i];
+        s_zf[ty * ZP + tz + TZ] = f * cur.wz1;
+      }
+      // Tiles narrower than R (grids thinner than 2P-1 points) own further face rows/columns.
+      for (int r = ty + 2 * TY; r < TY + R; r += TY) {
+        f = 0;
+#pragma unroll
+        for (int i = 0; i < 2 * P; ++i) f += c[i] * s_phi[(r + i) * LZ + tz + R];
+        const int ay = Y0 + r;
+        const T w = (z < n2 && ay < n1f) ? w1[m * pl1 + ay * n2 + z] : T(0);
+        s_yf[r * YP + tz] = f * w;
+      }
+      for (int q = tz + 2 * TZ; q < TZ + R; q += TZ) {
+        f = 0;
+#pragma unroll
+        for (int i = 0; i < 2 * P; ++i) f += c[i] * s_phi[(ty + R) * LZ + q + i];
+        const int az = Z0 + q;
+        const T w = (y < n1 && az < n2f) ? w2[m * pl2 + y * n2f + az] : T(0);
+        s_zf[ty * ZP + q] = f * w;
+      }
+    }
+    __syncthreads();
+
+    if (own) {
+      T ay = 0, az = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) ay += c[i] * s_yf[(ty + R - i) * YP + tz];
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) az += c[i] * s_zf[ty * ZP + tz + R - i];
+      const T lap = (ax + ay) + az;
+      const T pv = qphi[0];
+      T val = lap * T(0.5) * cur.iw + pv * cur.pt;
+      if (MODE == 1) val = (val - cc * pv) * ca;
+      if (MODE == 2) val = (val - cc * pv) * ca - cb * cur.pr;
+      bstore<T>(val, r_out, col, m * plane * S);
+    }
+
+    // advance the x queues and the plane pipeline
+#pragma unroll
+    for (int j = 0; j < 2 * P - 1; ++j) {
+      qf[j] = qf[j + 1];
+      qphi[j] = qphi[j + 1];
+    }
+    qphi[2 * P - 1] = cur.phn;
+    cur = nxt;
   }
 }
 
-struct Dims {
-  int tiles_z, tiles_y, tiles_x;
-  unsigned blocks;
-};
+static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
-static Dims dims_for(int n0, int n1, int n2) {
-  Dims d;
-  d.tiles_z = (n2 + TZ - 1) / TZ;
-  d.tiles_y = (n1 + TY - 1) / TY;
-  d.tiles_x = (n0 + TX - 1) / TX;
-  d.blocks = (unsigned)d.tiles_z * d.tiles_y * d.tiles_x;
+// Segment length override (0 = automatic); set by cdft_hdiv_set_segment.
+static int g_segment = 0;
+static March march_dims(int n0, int n1, int n2, int k, int reach) {
+  March d;
+  // z tiles are 13 or 16 wide (the kernel's LDS pitches are built for those); take the one that
+  // leaves fewer idle columns (65 = 5 x 13, 64 = 4 x 16).
+  const int idle13 = ceil_div(n2, 13) * 13 - n2, idle16 = ceil_div(n2, 16) * 16 - n2;
+  d.tz = idle13 < idle16 ? 13 : 16;
+  d.tiles_z = ceil_div(n2, d.tz);
+  d.tiles_y = ceil_div(n1, TMAX);
+  d.ty = ceil_div(n1, d.tiles_y);
+  const int per_plane = d.tiles_y * d.tiles_z;
+  if (g_segment > 0) {
+    d.nseg = ceil_div(n0, g_segment);
+  } else {
+    // About 1100 blocks (four or more per CU), but segments of at least 8 planes so the 4P-2
+    // plane warm-up stays a minor part of the reads. Measured on 65^3 (k = 5, 8) and 101 x 81 x 81
+    // (k = 5): the best segment counts were 8, 5 and 5-6.
+    d.nseg = (1100 + per_plane * k / 2) / (per_plane * k);
+    if (d.nseg > n0 / 8) d.nseg = n0 / 8;
+  }
+  if (d.nseg < 1) d.nseg = 1;
+  if (d.nseg > n0) d.nseg = n0;
+  d.seg = ceil_div(n0, d.nseg);
+  d.tiles = (unsigned)per_plane * d.nseg;
+  const int cover = d.ty * d.tz;
+  const int halo = ceil_div(2 * reach * (d.ty + d.tz), MAXH);  // every halo value has a slot
+  int th = cover > halo ? cover : halo;
+  d.threads = ceil_div(th, 64) * 64;
   return d;
 }
 
@@ -204,10 +343,20 @@ static int launch_p(const T* phi, const T* prev, T* out, const T* w0, const T* w
                     const double* table, int step, hipStream_t stream) {
   Taps<P> taps;
   for (int i = 0; i < 2 * P; ++i) taps.c[i] = coef[i];
-  const Dims d = dims_for(n0, n1, n2);
-  hipLaunchKernelGGL((hdiv_kernel<T, P, MODE>), dim3(d.blocks, k), dim3(NT), 0, stream, phi, prev, out,
-                     w0, w1, w2, invw, pot, k, n0, n1, n2, taps, table, step, d.tiles_z, d.tiles_y,
-                     d.tiles_x);
+  const March d = march_dims(n0, n1, n2, k, 2 * P - 1);
+  if (d.threads > NTMAX) return fail(CDFT_E_ARG, "hdiv: tile needs %d threads", d.threads);
+  // Offsets inside one vector and inside the extended face-weight arrays are 32-bit.
+  if ((long)(n0 + 2 * P) * (n1 + 2 * P) * (n2 + 2 * P) >= 0x7fffffffL)
+    return fail(CDFT_E_ARG, "hdiv: grid %d x %d x %d too large for one launch", n0, n1, n2);
+  const unsigned long blocks = (unsigned long)d.tiles * (unsigned long)k;
+  if (blocks > 0x7fffffffUL) return fail(CDFT_E_ARG, "hdiv: too many blocks");
+  const dim3 grid((unsigned)blocks), block(d.threads);
+  if (d.tz == 13)
+    hipLaunchKernelGGL((hdiv_kernel<T, P, MODE, 13>), grid, block, 0, stream, phi, prev, out, w0, w1, w2,
+                       invw, pot, n0, n1, n2, k, taps, table, step, d);
+  else
+    hipLaunchKernelGGL((hdiv_kernel<T, P, MODE, 16>), grid, block, 0, stream, phi, prev, out, w0, w1, w2,
+                       invw, pot, n0, n1, n2, k, taps, table, step, d);
   return check_launch("hdiv_kernel");
 }
 
@@ -336,4 +485,24 @@ extern "C" int cdft_cheb_filter_f32(const float* x, float* out, float* work1, fl
                                     int degree, void* stream) {
   return filter_impl<float>(x, out, work1, work2, w0, w1, w2, invw, pot, k, n0, n1, n2, coef,
                             order, table, degree, (hipStream_t)stream);
+}
+
+extern "C" int cdft_hdiv_set_segment(int planes) {
+  if (planes < 0) return fail(CDFT_E_ARG, "hdiv_set_segment: planes must be >= 0 (0 = automatic)");
+  g_segment = planes;
+  return CDFT_OK;
+}
+
+extern "C" int cdft_hdiv_launch_shape(int n0, int n1, int n2, int k, int order, int* shape6) {
+  if (!shape6 || n0 < 1 || n1 < 1 || n2 < 1 || k < 1) return fail(CDFT_E_ARG, "hdiv_launch_shape: bad arguments");
+  if (order != 2 && order != 4 && order != 6 && order != 8)
+    return fail(CDFT_E_ARG, "hdiv_launch_shape: unsupported stencil order %d", order);
+  const March d = march_dims(n0, n1, n2, k, order - 1);
+  shape6[0] = d.ty;
+  shape6[1] = d.tz;
+  shape6[2] = d.seg;
+  shape6[3] = d.nseg;
+  shape6[4] = d.threads;
+  shape6[5] = (int)(d.tiles * (unsigned)k);
+  return CDFT_OK;
 }

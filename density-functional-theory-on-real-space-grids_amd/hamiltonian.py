@@ -16,12 +16,44 @@ from .ops import StencilGeometry, host_doubles
 from .stencil import staggered_coefficients
 
 
-def seeded_randn(shape, generator: torch.Generator | None, device) -> torch.Tensor:
+def seeded_randn(shape, generator, device) -> torch.Tensor:
     """Standard-normal draw on the generator's own (CPU) device, then moved (reference
-    ``precision.py:246-263``), so the GPU solve starts from the reference's vectors."""
+    ``precision.py:246-263``), so the GPU solve starts from the reference's vectors. ``generator``
+    may be a :class:`SeededDraws`, which serves repeated draw sequences from device memory."""
     size = (shape,) if isinstance(shape, int) else tuple(shape)
+    if isinstance(generator, SeededDraws):
+        return generator.draw(size, device)
     draw = torch.randn(size, generator=generator, dtype=torch.float64)
     return draw.to(device)
+
+
+class SeededDraws:
+    """A freshly seeded CPU generator whose draws are memoised by their position in the draw
+    sequence. Every diagonalisation of one self-consistent solve re-seeds the same generator, so
+    its draws repeat exactly; the memo (shared by the solve through ``store``) turns the repeats
+    into device-to-device copies of the first draw instead of fresh CPU draws and uploads. The
+    values are bitwise those of ``torch.randn`` on ``torch.Generator().manual_seed(seed)``."""
+
+    def __init__(self, seed: int, store: dict):
+        self.seed = int(seed)
+        self.store = store
+        self.sequence: tuple = ()
+        self._gen = None
+
+    def draw(self, size: tuple, device) -> torch.Tensor:
+        key = (self.seed, self.sequence + (size,), str(device))
+        hit = self.store.get(key)
+        if hit is None:
+            if self._gen is None:
+                self._gen = torch.Generator(device="cpu").manual_seed(self.seed)
+                for earlier in self.sequence:  # replay the positions already served from the memo
+                    torch.randn(earlier, generator=self._gen, dtype=torch.float64)
+            hit = torch.randn(size, generator=self._gen, dtype=torch.float64).to(device)
+            self.store[key] = hit
+        elif self._gen is not None:
+            torch.randn(size, generator=self._gen, dtype=torch.float64)
+        self.sequence = self.sequence + (size,)
+        return hit.clone()
 
 
 class DeviceMeasure:

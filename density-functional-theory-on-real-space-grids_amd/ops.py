@@ -119,6 +119,19 @@ def hdiv_apply(phi: torch.Tensor, geom: StencilGeometry, out: torch.Tensor | Non
     return out.reshape(-1) if squeeze else out
 
 
+def hdiv_set_segment(planes: int) -> None:
+    """Planes per x-segment of the marching stencil kernels (0 restores the automatic choice)."""
+    _lib.call("cdft_hdiv_set_segment", int(planes))
+
+
+def hdiv_launch_shape(shape, k: int, order: int = 8) -> dict:
+    """The launch the stencil kernels use for a ``(k, *shape)`` block (host-side, no device work)."""
+    buf = (ctypes.c_int * 6)()
+    _lib.call("cdft_hdiv_launch_shape", int(shape[0]), int(shape[1]), int(shape[2]), int(k), int(order),
+              ctypes.addressof(buf))
+    return dict(zip(("tile_y", "tile_z", "segment", "segments", "threads", "blocks"), list(buf)))
+
+
 def cheb_step(y: torch.Tensor, prev: torch.Tensor | None, geom: StencilGeometry, table: torch.Tensor, step: int,
               out: torch.Tensor) -> torch.Tensor:
     """One fused recurrence step ``(A y - c y) a - b prev`` with ``(c, a, b) = table[step]``."""

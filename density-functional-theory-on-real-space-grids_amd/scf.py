@@ -20,7 +20,7 @@ import torch
 from . import _lib, ops
 from .eigen import ChebyshevFilteredSubspace, EigenConfig, SpectralBoundsHint
 from .geometry import CuspFactor, CuspQuadrature, UniformGrid
-from .hamiltonian import CuspFactoredHamiltonian
+from .hamiltonian import CuspFactoredHamiltonian, SeededDraws
 from .ops import stream_of
 from .poisson import CoulombCutoffPoisson
 
@@ -70,6 +70,7 @@ class KohnShamStep:
         self.ion_ion = nuclear_repulsion(self.charges, self.positions)
         self.poisson = CoulombCutoffPoisson(grid, numerics.poisson_pad_factor)
         self.bare = CuspFactoredHamiltonian(grid, factor, quadrature)
+        self.draws: dict = {}  # seeded start vectors of this solve, by position in the draw sequence
         dev = grid.device
         na = len(self.charges)
         dist = [[float(np.linalg.norm(self.positions[a] - self.positions[b])) for b in range(na)] for a in range(na)]
@@ -158,7 +159,7 @@ class KohnShamStep:
         return CuspFactoredHamiltonian(self.grid, self.factor, self.q, extra_potential=pot["effective"])
 
     def diagonalise(self, solver: ChebyshevFilteredSubspace, h, initial=None, max_iterations=None, bounds=None):
-        gen = torch.Generator(device="cpu").manual_seed(self.numerics.seed)
+        gen = SeededDraws(self.numerics.seed, self.draws)
         saved = solver.config.max_iterations
         if max_iterations is not None:
             solver.config.max_iterations = max_iterations
