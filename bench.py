@@ -28,6 +28,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the fused step on this workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+# `bench.py --kernel-only`, each scaled by the same counter on a 1 GiB copy in the same run (summary and
+# commands: profiles/hdiv_fused_step_he65_k5.md). Re-measured whenever the stencil kernel changes.
+FUSED_STEP_TRAFFIC_BYTES = (27979 / 0.500 + 11064 / 1.000) * 1024
 METRIC = "He-LDA SCF s/solve & H-apply GB/s vs HBM peak; mixed vs FP64 at <=1e-8 Ha"
 PUBLISHED_CPU_S_PER_SOLVE = 109.5  # reference, He LDA 65^3, CPU float64 2 threads (convergence_ladders.json)
 
@@ -144,6 +148,39 @@ def cpu_baseline(gpu_result):
                        "the reference's published CPU figure for this solve is 109.5 s (2 threads)")}
 
 
+def kernel_only(pkg, device, k=5, filters=3):
+    """The dominant kernel alone, for counter runs: ``filters`` degree-20 filters (20 fused steps each) of
+    a seeded U(-1,1) block on the bare He 65^3 geometry, then a 1 GiB copy of known size."""
+    import torch
+    from cdft_amd import scf
+    from cdft_amd.chefsi import chebyshev_coefficients
+    from cdft_amd.hamiltonian import CuspFactoredHamiltonian
+
+    grid, factor, quad = scf.build_geometry(he_lda_scenario(), numerics(), device)
+    geom = CuspFactoredHamiltonian(grid, factor, quad).geometry
+    gen = torch.Generator().manual_seed(0)
+    x = (torch.rand((k, grid.n_points), generator=gen, dtype=torch.float64) * 2 - 1).to(device)
+    table = torch.tensor(chebyshev_coefficients(20, -1.0, 40.0, -1.5), dtype=torch.float64, device=device)
+    out, w1, w2 = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    pkg.ops.cheb_filter(x, geom, table, 20, out=out, work=(w1, w2))
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(filters):
+        pkg.ops.cheb_filter(x, geom, table, 20, out=out, work=(w1, w2))
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / (20 * filters)
+    a = torch.ones(2**30 // 8, dtype=torch.float64, device=device)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    n = grid.n_points
+    bytes_step = (19 * stencil_bytes(grid.shape, k, 8, 8) + stencil_bytes(grid.shape, k, 8, 8, first=True)) / 20
+    return {"kernel_only": True, "grid": list(grid.shape), "k": k, "us_per_step": us, "algorithmic_bytes_per_step": bytes_step,
+            "GB_s": bytes_step / us / 1e3, "copy_bytes": 2**30, "points": n}
+
+
 def run_solve(pkg, device, filter_dtype, timer=None):
     import torch
     from cdft_amd import scf
@@ -160,6 +197,9 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-mixed", action="store_true")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="run only the fused Chebyshev step on the He 65^3 geometry (k=5, three degree-20 filters) "
+                         "and a 1 GiB device copy that calibrates the HBM byte counters; for rocprofv3 --pmc runs")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the Kohn-Sham path here runs on a "
@@ -173,6 +213,9 @@ def main(argv=None):
     pkg = ge.load_package()
     pkg.load_library()
     device = torch.device("cuda:0")
+    if args.kernel_only:
+        print(json.dumps(kernel_only(pkg, device)), flush=True)
+        return 0
     for _ in range(args.warmup):
         run_solve(pkg, device, torch.float64)
     timer = FilterTimer()
@@ -213,7 +256,7 @@ def main(argv=None):
         "block_applies": int(sum(r["applications"])),
         "roofline": {"kernel": "hdiv_kernel<double,4,2> (fused Chebyshev step, k=5)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_launch_us": avg_launch * 1e6, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "traffic": FUSED_STEP_TRAFFIC_BYTES, "avg_launch_us": avg_launch * 1e6, "algorithmic_bytes_per_launch": bytes_per_launch,
                      "launches_timed": timer.launches},
         "h_apply_microbench": micro,
     }

@@ -106,6 +106,146 @@ int cdft_eigh_f64(const double* m, int k, double* evals, double* c, void* stream
 /* out[0] = max |M - I| over a k x k matrix (src/cdft/eigen/rayleigh_ritz.py:44-48). */
 int cdft_orth_error_f64(const double* m, int k, double* out, void* stream);
 
+/* ---- stencil launch control (no reference counterpart: tuning of the K1/K2 launch) ---- */
+
+/* Planes per x-segment of the marching stencil kernels (0 = automatic choice). Process-wide. */
+int cdft_hdiv_set_segment(int planes);
+/* Host-only: the launch the stencil kernels use for a (k, n0, n1, n2) block of the given order:
+ * shape6 (host) = {tile_y, tile_z, longest segment, segments, threads per block, blocks}. */
+int cdft_hdiv_launch_shape(int n0, int n1, int n2, int k, int order, int* shape6);
+
+/* ---- K7: geometry of the cusp factor and the cusp-aware quadrature (once per geometry) ----
+ * Grid points sit at origin + h * (i0, i1, i2); shape (host, 3 ints) = (n0, n1, n2); origin (host,
+ * 3 doubles). charges (host, n_atoms) and positions (host, n_atoms x 3) describe the nuclei. */
+
+/* u = sum_a Z_a |r - R_a|, grad u (3, N), weight = f^2 = exp(-2u), and the transformed potential
+ * W = -|grad u|^2 / 2 cell-averaged with the 4^3 Gauss-Legendre rule (gl_offsets: 4 host node
+ * offsets in units of h, gl_weights: 64 host weights, gl_total their sum) at points within `reach`
+ * of a nucleus. Replaces CuspFactor.__init__ / _build_potential (src/cdft/operators/cusp.py:46-145). */
+int cdft_cusp_grid_fields_f64(int n_atoms, const double* charges, const double* positions,
+                              const int* shape, double h, const double* origin, double reach,
+                              const double* gl_offsets, const double* gl_weights, double gl_total,
+                              double* u, double* grad_u, double* weight, double* wpot, void* stream);
+
+/* exp(-2u) at the staggered faces of each axis on the box extended by `order` along that axis
+ * (w0, w1, w2 as in the conventions above); umax (device, zeroed by the caller) receives the bits
+ * of max u so the host can refuse an underflowing factor. Replaces CuspFactor.face_weights
+ * (src/cdft/operators/cusp.py:168-214). */
+int cdft_face_weights_f64(int n_atoms, const double* charges, const double* positions,
+                          const int* shape, double h, const double* origin, int order, double* w0,
+                          double* w1, double* w2, unsigned long long* umax, void* stream);
+
+/* Per quadrature node (pts (m, 3), owner atom, base weight): partition P = (1 - far) * (Becke
+ * share of the owner plus the lune of the other cells), weights = base_w * P, f^2, grad u (3, m),
+ * W = -|grad u|^2 / 2 and P itself. spheres (host): the sphere table of the quadrature (centres,
+ * radii, taper width). Replaces CuspQuadrature._partition_weights / _becke_cells / _bump
+ * (src/cdft/operators/quadrature.py:360-419). */
+int cdft_node_fields_f64(const double* pts, const int* owner, const double* base_w, long m,
+                         int n_atoms, const double* charges, const double* positions,
+                         const double* spheres, double cell_width, double* weights, double* f2,
+                         double* grad_u, double* wpot, double* partition, void* stream);
+
+/* far[p] = prod_a (1 - taper_a(|r_p - c_a|)) at every grid point. Replaces
+ * CuspQuadrature._far_factor (src/cdft/operators/quadrature.py:499-504). */
+int cdft_far_factor_f64(const int* shape, double h, const double* origin, int n_spheres,
+                        const double* spheres, double* far, void* stream);
+
+/* Degree-7 Lagrange stencil of each point: base (m, 3) = the lower-left grid index of the 8^3
+ * stencil's base cell, lw (m, 24) = the 8 weights per axis; bad[0] = 1 if a stencil leaves the
+ * grid. Replaces CuspQuadrature._build_stencils / _point_stencil and lagrange_weights
+ * (src/cdft/operators/quadrature.py:56-98,506-609,662-700). */
+int cdft_node_stencils_f64(const double* pts, long m, const int* shape, double h,
+                           const double* origin, int* base, double* lw, int* bad, void* stream);
+
+/* out[f][i] = sum over the 8^3 stencil of node i of lw products times field[f] (nf <= 4 fields of
+ * N values each). Replaces CuspQuadrature.interpolate / interpolate_at
+ * (src/cdft/operators/quadrature.py:613-620,652-660). */
+int cdft_interpolate_f64(const double* field, int nf, const int* base, const double* lw, long m,
+                         const int* shape, double h, const double* origin, double* out,
+                         void* stream);
+
+/* Doubles of workspace cdft_lift_f64 needs for a cell box (host: lo0, lo1, lo2, c0, c1, c2). */
+long cdft_lift_workspace_doubles(const int* cell_box, int nf);
+/* The transpose of interpolate: out[f] = add[f] + L^T x[f] (add may be NULL), deterministic and
+ * atomic-free. Nodes must be numbered in base-cell order with cell_start (c0*c1*c2 + 1 offsets)
+ * over the cell box. Replaces CuspQuadrature.lift and the precomputed L^T CSR product
+ * (src/cdft/operators/quadrature.py:582-609,628-638). */
+int cdft_lift_f64(const double* x, int nf, long m, const int* cell_start, const int* base,
+                  const double* lw, const int* shape, double h, const double* origin,
+                  const int* cell_box, const double* add, double* work, long work_len, double* out,
+                  void* stream);
+
+/* Central order-`order` first derivatives of nf fields (coef: DEVICE array of order+1 weights
+ * divided by h), out (nf, 3, N), zero halo. Replaces gradient_from_padded
+ * (src/cdft/operators/laplacian.py:133-150) as used by the energy terms. */
+int cdft_gradient_f64(const double* field, int nf, const int* shape, double h, const double* origin,
+                      const double* coef, int order, double* out, void* stream);
+
+/* Per nucleus a: the unit cusp density exp(-2 Z_a s_a) and its closed-form potential at the grid
+ * points (pts NULL, m = N) or at pts (m, 3); dens, v1s (n_atoms, m), either may be NULL. Replaces
+ * hydrogenic_1s_potential and StepGeometry.build (src/cdft/operators/quadrature.py:805-820,
+ * src/cdft/scf/step.py:128-165). */
+int cdft_hydrogenic_fields_f64(const double* pts, long m, const int* shape, double h,
+                               const double* origin, int n_atoms, const double* charges,
+                               const double* positions, double* dens, double* v1s, void* stream);
+
+/* ---- K6 / K8: XC, densities, pointwise algebra and reductions of the Kohn-Sham map ---- */
+
+/* Spin-unpolarised Slater exchange + VWN5 correlation: e = n eps_xc(n) and v = d e / d n
+ * (analytic), zero below n = 1e-14. Replaces LDA(VWN5) through SemiLocalFunctional.evaluate
+ * (src/cdft/xc/lda.py:122-159,201-217; src/cdft/xc/base.py:115-210). */
+int cdft_lda_vwn_f64(const double* n, long len, double* e, double* v, void* stream);
+
+/* out = op(a, b, c, s) elementwise over len values: 0 a b; 1 (a b) c; 2 s / a; 3 a / b; 4 a s;
+ * 5 (a + b) / c; 6 a - b; 7 (a / s) s; 8 s; 9 a + b; 10 a (b . c) with b, c of shape (3, len).
+ * The pointwise steps of KohnShamStep.potentials and energy_terms
+ * (src/cdft/scf/step.py:224-313,398-439). */
+int cdft_ewise_f64(int op, const double* a, const double* b, const double* c, double s, double* out,
+                   long len, void* stream);
+
+/* rho = sum_i occ_i phi_i^2 and n = f^2 rho (occ: host, k <= 4). Replaces
+ * KohnShamStep.output_density and CuspFactor.density (src/cdft/scf/step.py:366-370,
+ * src/cdft/operators/cusp.py:325-331). */
+int cdft_density_f64(const double* phi, int k, long len, const double* f2, const double* occ,
+                     double* rho, double* n, void* stream);
+
+/* out = base + sign * sum_a c[a] fields[a] (c: device, na values; fields (na, len)). The cusp
+ * split of the Hartree term (src/cdft/scf/step.py:239-250). */
+int cdft_atom_combine_f64(const double* base, const double* c, const double* fields, int na,
+                          long len, double sign, double* out, void* stream);
+
+/* x = A^-1 (b * scale) for a small dense system on the device (na <= 8, partial pivoting). The
+ * cusp coefficients c = S^-1 n(R_a) (src/cdft/scf/step.py:239-250). */
+int cdft_small_solve_f64(const double* a, const double* scale, const double* b, int na, double* x,
+                         void* stream);
+
+/* Doubles of block-partials workspace cdft_reduce_f64 needs for len values. */
+long cdft_reduce_workspace_doubles(long len);
+/* out[slot] = scale * R (or += with accumulate), R by mode: 0 sum a b w (b, w may be NULL);
+ * 1 sum w |a|; 2 max |a - b|. Deterministic two-pass reduction. The integrals of
+ * CuspQuadrature.integrate, the energy terms and the SCF residual norms
+ * (src/cdft/operators/quadrature.py:701-709; src/cdft/scf/step.py:398-439;
+ * src/cdft/scf/loop.py:345-412). */
+int cdft_reduce_f64(int mode, const double* a, const double* b, const double* w, long len,
+                    double scale, int accumulate, double* partials, long partials_len, double* out,
+                    int slot, void* stream);
+
+/* ---- K5: free-space Hartree potential by the Coulomb-cutoff FFT (rocFFT through hipFFT) ---- */
+
+/* Plans for an (n0, n1, n2) grid zero-padded to (p0, p1, p2); *handle owns them. Replaces
+ * CoulombCutoffPoisson.__init__ (src/cdft/operators/poisson.py:27-42,110-120). */
+int cdft_poisson_create(int n0, int n1, int n2, int p0, int p1, int p2, void** handle);
+int cdft_poisson_destroy(void* handle);
+/* kernel (p0 x p1 x (p2/2+1)) = K(G) / (p0 p1 p2), K = 4 pi (1 - cos G R) / G^2, K(0) = 2 pi R^2.
+ * Replaces CoulombCutoffPoisson._build_kernel (src/cdft/operators/poisson.py:43-64). */
+int cdft_poisson_kernel_f64(void* handle, double h, double radius, double* kernel, void* stream);
+/* potential = the cut-off Coulomb potential of density; padded_in/padded_out (p0 p1 p2 doubles)
+ * and spectrum (p0 p1 (p2/2+1) complex doubles) are caller workspace, padded_in's padding must be
+ * zero. Replaces CoulombCutoffPoisson.solve (src/cdft/operators/poisson.py:66-84). */
+int cdft_poisson_solve_f64(void* handle, const double* density, double* padded_in, void* spectrum,
+                           double* padded_out, const double* kernel, double* potential,
+                           void* stream);
+
 #ifdef __cplusplus
 }
 #endif

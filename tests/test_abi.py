@@ -34,6 +34,19 @@ def test_ctypes_table_covers_header(pkg):
     assert set(declared_symbols()) <= bound
 
 
+def test_header_declares_every_bound_symbol(pkg):
+    """The header is the whole ABI: nothing the host binding calls is missing from it."""
+    assert set(pkg._lib.exported_symbols()) <= set(declared_symbols())
+
+
+def test_launch_shape_is_host_only_and_tiles_odd_grids(pkg):
+    """The stencil launch plan (a host computation) covers 65 = 5 x 13 with 13-wide tiles."""
+    shape = pkg.ops.hdiv_launch_shape((65, 65, 65), 5, 8)
+    assert (shape["tile_y"], shape["tile_z"]) == (13, 13)
+    assert shape["threads"] % 64 == 0 and shape["blocks"] == 25 * 5 * shape["segments"]
+    assert shape["segment"] >= 8
+
+
 def test_abi_version_and_error_string(pkg):
     lib = pkg.load_library()
     assert lib.cdft_abi_version() == 1
