@@ -175,6 +175,23 @@ int cdft_lift_f64(const double* x, int nf, long m, const int* cell_start, const 
                   const int* cell_box, const double* add, double* work, long work_len, double* out,
                   void* stream);
 
+/* Lumped far mass weights of overlapping spheres (src/cdft/operators/quadrature.py:421-497), three
+ * steps. (1) out (f0, f1, f2) = (h_f^3 far) exp(-2u) at the points (fx[i], fy[j], fz[l]) of the
+ * refined lattice (fx, fy, fz: device coordinate arrays; spheres (host): cx, cy, cz, radius,
+ * taper centre, taper width per sphere, n_spheres <= 8; charges (host)). */
+int cdft_far_fine_f64(int n_spheres, const double* charges, const double* spheres, const int* fine_shape,
+                      const double* fx, const double* fy, const double* fz, double volume, double* out,
+                      void* stream);
+/* (2) The transposed one-dimensional Lagrange interpolation along `axis` of a (dims) array onto
+ * n_coarse points: out[.., c, ..] = sum_{f in [range[2c], range[2c+1])} w[8 f + c - first[f]]
+ * in[.., f, ..] (range, first, w: device tables of the axis). */
+int cdft_interp_transpose_axis_f64(const double* in, const int* dims, int axis, int n_coarse,
+                                   const int* range, const int* first, const double* w, double* out,
+                                   void* stream);
+/* (3) out = plain + interior (lumped - plain), interior = erfc((margin - d_face) / width) / 2. */
+int cdft_far_blend_f64(const int* shape, double h, const double* origin, double margin, double width,
+                       const double* plain, const double* lumped, double* out, void* stream);
+
 /* Central order-`order` first derivatives of nf fields (coef: DEVICE array of order+1 weights
  * divided by h), out (nf, 3, N), zero halo. Replaces gradient_from_padded
  * (src/cdft/operators/laplacian.py:133-150) as used by the energy terms. */

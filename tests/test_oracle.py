@@ -157,3 +157,17 @@ def test_helium_lda_ladder_rung_h05():
     pin = GOLDEN["ladder.he_lda.h0.5.total"]
     assert abs(r["energies"]["total"] - pin) < 1e-11
     assert r["scf_iterations"] == GOLDEN["ladder.he_lda.h0.5.scf_iterations"]
+
+
+def test_h2plus_ladder_rung_h05():
+    """Bare H2+ at R = 2 (protons at z = -1, +1) on the reference's derived grid for h = 0.5: box
+    edge = nuclear span + 12 bohr vacuum = 14, lattice anchored on the first nucleus (29^3). The
+    two spheres overlap, so this pins the overlapping-sphere quadrature (Becke cells, lune, 0.7
+    angular node spacing) and the lumped far mass weights of ``quadrature.py:421-497`` against the
+    reference ladder's eigenvalue."""
+    from oracle.physics import Numerics, Scenario, solve_noninteracting
+
+    sc = Scenario((1.0, 1.0), ((0.0, 0.0, -1.0), (0.0, 0.0, 1.0)), 1.0)
+    r = solve_noninteracting(sc, Numerics(box_lengths=(14.0, 14.0, 14.0), spacing=0.5), anchor=(0.0, 0.0, -1.0))
+    assert r["hamiltonian"].quadrature.far_lumped
+    assert abs(r["eigenvalues"][0] - GOLDEN["ladder.h2plus_R2.h0.5.eigenvalue"]) < 1e-11

@@ -177,3 +177,44 @@ def test_helium_lda_scf_h05_matches_ladder(pkg, device):
     r = scf.solve_self_consistent(ophys.Scenario((2.0,), ((0.0, 0.0, 0.0),), 2.0, "lda"), num, device)
     assert r["scf_iterations"] == GOLDEN["ladder.he_lda.h0.5.scf_iterations"]
     assert abs(r["energies"]["total"] - GOLDEN["ladder.he_lda.h0.5.total"]) < 1e-10
+
+
+def test_lumped_far_weights_match_oracle(h2_case):
+    """Overlapping spheres: Becke/lune partition at the nodes, the lumped far mass weights on the
+    twice-refined lattice with the edge blend, and the mass weights built from them."""
+    from cdft_amd import geometry as g
+
+    grid_o, factor_o, grid, factor = h2_case
+    quad_o = oquad.CuspQuadrature(grid_o, factor_o)
+    quad = g.CuspQuadrature(grid, factor)
+    assert quad.far_lumped and quad_o.far_lumped
+    assert quad.n_nodes == quad_o.n_nodes
+    perm = quad.node_order
+    assert _rel(quad.weights.cpu(), quad_o.weights[perm]) < 1e-13
+    assert _rel(quad.lumped_far.cpu(), quad_o.lumped_far) < 1e-13
+    assert _rel(quad.mass_weights.cpu(), quad_o.mass_weights) < 1e-13
+
+
+def test_h2plus_solve_matches_ladder(pkg, device):
+    """Bare H2+ at R = 2 on the reference's derived grid for h = 0.5 (box 14, anchored on the first
+    nucleus, 29^3): the ladder eigenvalue of the reference."""
+    from cdft_amd import scf
+
+    sc = ophys.Scenario((1.0, 1.0), ((0.0, 0.0, -1.0), (0.0, 0.0, 1.0)), 1.0)
+    num = ophys.Numerics(box_lengths=(14.0, 14.0, 14.0), spacing=0.5)
+    r = scf.solve_noninteracting(sc, num, device, anchor=(0.0, 0.0, -1.0))
+    assert abs(r["eigenvalues"][0] - GOLDEN["ladder.h2plus_R2.h0.5.eigenvalue"]) < 1e-11
+
+
+def test_h2_lda_scf_matches_oracle(pkg, device):
+    """H2 at VWN LDA, R = 1.4 along x, lattice anchored on the first proton (both protons on the
+    lattice), 29 x 25 x 25 at h = 0.35: the GPU SCF against the oracle SCF."""
+    from cdft_amd import scf
+
+    sc = ophys.Scenario((1.0, 1.0), ((-0.7, 0.0, 0.0), (0.7, 0.0, 0.0)), 2.0, "lda")
+    num = ophys.Numerics(box_lengths=(9.8, 8.4, 8.4), spacing=0.35)
+    anchor = (-0.7, 0.0, 0.0)
+    r = scf.solve_self_consistent(sc, num, device, anchor=anchor)
+    o = ophys.solve_self_consistent(sc, num, anchor=anchor)
+    assert r["scf_iterations"] == o["scf_iterations"]
+    assert abs(r["energies"]["total"] - o["energies"]["total"]) < 1e-10
