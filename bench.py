@@ -181,6 +181,90 @@ def kernel_only(pkg, device, k=5, filters=3):
             "GB_s": bytes_step / us / 1e3, "copy_bytes": 2**30, "points": n}
 
 
+def compare_precisions(device, scenario, numerics_, anchor=None, reference=None):
+    """One float64 and one mixed (float32 filter, float64 refinement) solve of a configuration, each
+    after an untimed warm-up solve of the same precision: seconds, energies and the mixed error."""
+    import torch
+    from cdft_amd import scf
+
+    out = {}
+    for label, dtype in (("f64", torch.float64), ("mixed", torch.float32)):
+        scf.solve_self_consistent(scenario, numerics_, device, anchor=anchor, filter_dtype=dtype)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = scf.solve_self_consistent(scenario, numerics_, device, anchor=anchor, filter_dtype=dtype)
+        torch.cuda.synchronize()
+        out[label] = {"s_per_solve": time.perf_counter() - t0, "energy_total": r["energies"]["total"],
+                      "scf_iterations": r["scf_iterations"],
+                      "f32_filter_passes": r["filter_precisions"].count("f32"),
+                      "f64_filter_passes": r["filter_precisions"].count("f64")}
+    out["speedup_mixed_vs_f64"] = out["f64"]["s_per_solve"] / out["mixed"]["s_per_solve"]
+    out["energy_error_mixed_vs_f64"] = out["mixed"]["energy_total"] - out["f64"]["energy_total"]
+    if reference is not None:
+        out["energy_reference_cpu"] = reference
+        out["energy_error_f64_vs_reference"] = out["f64"]["energy_total"] - reference
+    return out
+
+
+def adjoint_inversion_step(device, spacing=0.2, seed=0):
+    """One Kohn-Sham inversion step on He LDA at h = 0.20 (81^3): V_0 is the converged LDA effective
+    potential; the target density is that of V_0 + 8 seeded Gaussian bumps (amplitude U(-0.05, 0.05)
+    Ha, width 1 bohr, centres uniform in the 2 bohr ball); L = sum omega f^2 (rho - rho_t)^2; dL/dV on
+    the grid by the Sternheimer backward, and one central finite difference (step 1e-4) along a
+    further seeded bump field as a check."""
+    import torch
+    from cdft_amd import scf
+    from cdft_amd.adjoint import KohnShamDensity, KohnShamProblem, density_loss
+
+    r = scf.solve_self_consistent(he_lda_scenario(), numerics(spacing=spacing), device)
+    step = r["step"]
+    ham = step.hamiltonian({"effective": r["effective_potential"]})
+    grid = step.grid
+    pts = grid_points(grid)
+
+    def bumps(s, amplitude):
+        rng = np.random.default_rng(s)
+        a = rng.uniform(-amplitude, amplitude, 8)
+        d = rng.normal(size=(8, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        c = d * (2.0 * rng.uniform(0, 1, 8) ** (1.0 / 3.0))[:, None]
+        out = np.zeros(pts.shape[0])
+        for j in range(8):
+            out += a[j] * np.exp(-((pts - c[j]) ** 2).sum(-1) / 2.0)
+        return torch.as_tensor(out).to(device)
+
+    weights = step.q.mass_weights * step.f2
+    problem = KohnShamProblem(grid, ham.geometry, ham.measure.q, [2.0])
+    rho_t, _ = problem.forward(bumps(seed, 0.05))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    v = torch.zeros(grid.n_points, dtype=torch.float64, device=device, requires_grad=True)
+    loss = density_loss(KohnShamDensity.apply(v, problem), rho_t, weights)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    d = bumps(seed + 1, 1.0)
+    h = 1e-4
+    fd = float(density_loss(problem.forward(h * d)[0], rho_t, weights) - density_loss(problem.forward(-h * d)[0], rho_t, weights)) / (2 * h)
+    an = float((v.grad * d).sum())
+    cg = problem.history[-1]["sternheimer"][0]
+    return {"workload": "He LDA 81^3 (h=0.20, 16 bohr box): dL/dV of L = integral (n - n_t)^2 by the Sternheimer backward",
+            "loss": float(loss), "forward_s": t1 - t0, "backward_s": t2 - t1, "cg_iterations": cg["iterations"],
+            "cg_relative_residual": cg["relative_residual"], "fd_directional": fd, "analytic_directional": an,
+            "fd_relative_error": abs(fd - an) / max(abs(an), 1e-300), "fd_step": h}
+
+
+def grid_points(grid):
+    n0, n1, n2 = grid.shape
+    o = np.asarray(grid.origin)
+    i = np.arange(n0)[:, None, None] * grid.spacing + o[0]
+    j = np.arange(n1)[None, :, None] * grid.spacing + o[1]
+    k = np.arange(n2)[None, None, :] * grid.spacing + o[2]
+    return np.stack(np.broadcast_arrays(i, j, k), axis=-1).reshape(-1, 3)
+
+
 def run_solve(pkg, device, filter_dtype, timer=None):
     import torch
     from cdft_amd import scf
@@ -197,6 +281,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-mixed", action="store_true")
+    ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
                     help="run only the fused Chebyshev step on the He 65^3 geometry (k=5, three degree-20 filters) "
                          "and a 1 GiB device copy that calibrates the HBM byte counters; for rocprofv3 --pmc runs")
@@ -270,7 +355,21 @@ def main(argv=None):
         line["mixed"] = {"s_per_solve": tm, "speedup_vs_f64": s_per_solve / tm,
                          "energy_total": rm["energies"]["total"],
                          "energy_error_vs_f64": rm["energies"]["total"] - r["energies"]["total"],
-                         "scf_iterations": rm["scf_iterations"]}
+                         "scf_iterations": rm["scf_iterations"],
+                         "f32_filter_passes": rm["filter_precisions"].count("f32"),
+                         "f64_filter_passes": rm["filter_precisions"].count("f64")}
+        # The precision target's configuration: He LDA on the 16 bohr box at h = 0.20 (81^3).
+        line["he_lda_h020_81"] = compare_precisions(device, he_lda_scenario(), numerics(spacing=0.2),
+                                                    reference=-2.834830140859734)
+        # H2 LDA, protons at (+-0.7, 0, 0), 20 x 16 x 16 box at h = 0.20 (101 x 81 x 81), lattice
+        # anchored on the first proton.
+        from types import SimpleNamespace
+        from cdft_amd.contract import Numerics
+        h2 = SimpleNamespace(charges=(1.0, 1.0), positions=((-0.7, 0.0, 0.0), (0.7, 0.0, 0.0)), n_electrons=2.0, xc="lda")
+        line["h2_lda_101x81x81"] = compare_precisions(device, h2, Numerics(box_lengths=(20.0, 16.0, 16.0), spacing=0.2),
+                                                      anchor=(-0.7, 0.0, 0.0))
+    if not args.skip_adjoint:
+        line["adjoint_inversion_step"] = adjoint_inversion_step(device)
     if not args.skip_cpu:
         line["cpu_baseline"] = cpu_baseline(gpu_result)
     print(json.dumps(line), flush=True)

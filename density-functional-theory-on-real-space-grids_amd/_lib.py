@@ -29,6 +29,8 @@ SIGNATURES: dict[str, list] = {
     "cdft_cheb_step_f32": [P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
     "cdft_cheb_filter_f64": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
     "cdft_cheb_filter_f32": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
+    "cdft_hdiv_apply_adjoint_f64": [P, P, P, P, P, P, P, P, I, I, I, I, P, I, P],
+    "cdft_hdiv_apply_adjoint_f32": [P, P, P, P, P, P, P, P, I, I, I, I, P, I, P],
     "cdft_hdiv_set_segment": [I],
     "cdft_hdiv_launch_shape": [I, I, I, I, I, P],
     "cdft_wgram_f64": [P, I, P, I, P, L, D, I, P, L, P, P],

@@ -66,7 +66,8 @@ struct March {
 };
 
 // Epilogue modes: 0 = A phi; 1 = (A y - c y) a; 2 = (A y - c y) a - b prev, where (c, a, b) is row
-// `step` of the Chebyshev coefficient table. TZ is the tile extent along z (compile-time, so the LDS
+// `step` of the Chebyshev coefficient table; 3 = the transpose A^T x = 0.5 sum_d D_d^T[w_d D_d
+// (invw x)] + V x, with phi = invw x and prev = x. TZ is the tile extent along z (compile-time, so the LDS
 // pitches below are constants and neighbouring reads pair into ds_read2). Threads are numbered
 // row-major over the tile and every LDS row pitch is congruent to TZ modulo 16 doubles, so any 16
 // consecutive lanes touch 16 consecutive bank pairs: the plane-tile, y-face and z-face accesses are
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(NTMAX) void hdiv_kernel(
   const long vbase = (long)v * plane * n0;
   const int vbytes = plane * n0 * S;
   const auto r_phi = make_rsrc(phi + vbase, vbytes);
-  const auto r_prev = make_rsrc(MODE == 2 ? prev + vbase : phi, vbytes);
+  const auto r_prev = make_rsrc(MODE >= 2 ? prev + vbase : phi, vbytes);
   const auto r_out = make_rsrc(out + vbase, vbytes);
   const auto r_w0 = make_rsrc(w0, (n0 + 2 * P) * plane * S);
   const auto r_w1 = make_rsrc(w1, n0 * pl1 * S);
@@ -204,9 +205,9 @@ __global__ __launch_bounds__(NTMAX) void hdiv_kernel(
     L.wy1 = bload<T>(r_w1, wy1_off, m * pl1 * S);
     L.wz0 = bload<T>(r_w2, wz0_off, m * pl2 * S);
     L.wz1 = bload<T>(r_w2, wz1_off, m * pl2 * S);
-    L.iw = bload<T>(r_iw, col, base);
+    L.iw = MODE == 3 ? T(0) : bload<T>(r_iw, col, base);
     L.pt = bload<T>(r_pt, col, base);
-    L.pr = MODE == 2 ? bload<T>(r_prev, col, base) : T(0);
+    L.pr = MODE >= 2 ? bload<T>(r_prev, col, base) : T(0);
   };
 
   Plane cur;
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(NTMAX) void hdiv_kernel(
       for (int i = 0; i < 2 * P; ++i) az += c[i] * s_zf[ty * ZP + tz + R - i];
       const T lap = (ax + ay) + az;
       const T pv = qphi[0];
-      T val = lap * T(0.5) * cur.iw + pv * cur.pt;
+      T val = MODE == 3 ? lap * T(0.5) + cur.pt * cur.pr : lap * T(0.5) * cur.iw + pv * cur.pt;
       if (MODE == 1) val = (val - cc * pv) * ca;
       if (MODE == 2) val = (val - cc * pv) * ca - cb * cur.pr;
       bstore<T>(val, r_out, col, m * plane * S);
@@ -485,6 +486,50 @@ extern "C" int cdft_cheb_filter_f32(const float* x, float* out, float* work1, fl
                                     int degree, void* stream) {
   return filter_impl<float>(x, out, work1, work2, w0, w1, w2, invw, pot, k, n0, n1, n2, coef,
                             order, table, degree, (hipStream_t)stream);
+}
+
+namespace cdft {
+__global__ void scale_rows_kernel(const double* __restrict__ x, const double* __restrict__ w, long n, long total,
+                                  double* __restrict__ out) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < total) out[p] = w[p % n] * x[p];
+}
+__global__ void scale_rows_kernel_f32(const float* __restrict__ x, const float* __restrict__ w, long n, long total,
+                                      float* __restrict__ out) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < total) out[p] = w[p % n] * x[p];
+}
+}  // namespace cdft
+
+template <typename T>
+static int adjoint_impl(const T* x, const T* w0, const T* w1, const T* w2, const T* invw, const T* pot, T* out,
+                        T* work, int k, int n0, int n1, int n2, const double* coef, int order, hipStream_t s) {
+  int rc = validate(x, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order);
+  if (rc) return rc;
+  if (!work || work == x || work == out) return fail(CDFT_E_ARG, "hdiv_apply_adjoint: work must be a distinct (k, N) block");
+  const long n = (long)n0 * n1 * n2, total = n * k;
+  const unsigned nb = (unsigned)((total + 255) / 256);
+  if (sizeof(T) == 8)
+    hipLaunchKernelGGL(scale_rows_kernel, dim3(nb), dim3(256), 0, s, (const double*)x, (const double*)invw, n, total,
+                       (double*)work);
+  else
+    hipLaunchKernelGGL(scale_rows_kernel_f32, dim3(nb), dim3(256), 0, s, (const float*)x, (const float*)invw, n,
+                       total, (float*)work);
+  rc = check_launch("scale_rows_kernel");
+  if (rc) return rc;
+  return launch<T, 3>(work, x, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order, nullptr, 0, s);
+}
+
+extern "C" int cdft_hdiv_apply_adjoint_f64(const double* x, const double* w0, const double* w1, const double* w2,
+                                           const double* invw, const double* pot, double* out, double* work, int k,
+                                           int n0, int n1, int n2, const double* coef, int order, void* stream) {
+  return adjoint_impl<double>(x, w0, w1, w2, invw, pot, out, work, k, n0, n1, n2, coef, order, (hipStream_t)stream);
+}
+
+extern "C" int cdft_hdiv_apply_adjoint_f32(const float* x, const float* w0, const float* w1, const float* w2,
+                                           const float* invw, const float* pot, float* out, float* work, int k,
+                                           int n0, int n1, int n2, const double* coef, int order, void* stream) {
+  return adjoint_impl<float>(x, w0, w1, w2, invw, pot, out, work, k, n0, n1, n2, coef, order, (hipStream_t)stream);
 }
 
 extern "C" int cdft_hdiv_set_segment(int planes) {

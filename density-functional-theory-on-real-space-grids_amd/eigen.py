@@ -145,6 +145,8 @@ class ChebyshevFilteredSubspace:
     switch_residual: float = 1.0e-5
     #: ...and always finish an iteration-limited solve with this many float64 filter passes.
     fp64_tail: int = 1
+    #: A float32 pass whose residual is above this fraction of the previous one has stalled.
+    stall_ratio: float = 0.7
 
     def __init__(self, grid, config: EigenConfig, ritz_stop: bool = True, filter_dtype=torch.float64,
                  filter_hook=None, use_graphs: bool = True):
@@ -249,7 +251,12 @@ class ChebyshevFilteredSubspace:
     # ---- the solver ----
 
     def solve(self, hamiltonian, n_states: int, initial: torch.Tensor | None = None, generator=None,
-              bounds: SpectralBoundsHint | None = None) -> EigenResult:
+              bounds: SpectralBoundsHint | None = None, precision: str = "auto",
+              min_fp64_passes: int = 0) -> EigenResult:
+        """``precision`` (mixed solvers only): ``"auto"`` filters in float32 while the worst residual
+        is above ``switch_residual``; ``"f32"`` filters every pass in float32 (the warm passes of an
+        SCF iteration, whose vectors the outer loop refines); ``"f64"`` filters in float64 and
+        runs at least ``min_fp64_passes`` passes (the refinement of a mixed solve)."""
         cfg = self.config
         g = self.grid
         extra = max(cfg.n_extra_states, int(cfg.extra_states_fraction * n_states))
@@ -277,6 +284,7 @@ class ChebyshevFilteredSubspace:
         hamiltonian.note_applications(2)
         evals, norms, _ = _read(work)
         worst = float(norms[:n_states].max())
+        stalled32 = False
         previous = evals.copy()
         stable = 0
         stop_reason = "iteration limit"
@@ -300,19 +308,30 @@ class ChebyshevFilteredSubspace:
                 fraction = _fraction(lower, spectrum_min, spectrum_max)
             top_fraction = max(top_fraction, fraction)
             lower = min(max(lower, spectrum_min + 1e-8), spectrum_max - 1e-8)
-            use32 = self.mixed and worst > self.switch_residual and iteration <= cfg.max_iterations - self.fp64_tail
+            if precision == "f32":
+                use32 = self.mixed
+            elif precision == "f64":
+                use32 = False
+            else:
+                use32 = (self.mixed and not stalled32 and worst > self.switch_residual
+                         and iteration <= cfg.max_iterations - self.fp64_tail)
             degree = self._upload_table(work, lower, spectrum_max, spectrum_min)
             self.filter_precisions.append("f32" if use32 else "f64")
             self._run_iteration(work, use32, degree)
             hamiltonian.note_applications(degree + 2)
             evals, norms, orth = _read(work)
+            before = worst
             worst = float(norms[:n_states].max())
+            # A float32 pass that no longer shrinks the residual has reached float32's floor
+            # (about eps32 times the spectral width); the rest of an "auto" solve runs in float64.
+            if use32 and worst > self.stall_ratio * before:
+                stalled32 = True
             drift = float(np.abs(evals[:n_states] - previous[:n_states]).max())
             next_lower = float(evals[min(size, evals.shape[0]) - 1])
             previous = evals.copy()
             self.history.append({"iteration": iteration, "residual": worst, "orthonormality": orth,
                                  "eigenvalue_drift": drift})
-            if worst < cfg.residual_tol:
+            if worst < cfg.residual_tol and iteration >= min_fp64_passes:
                 converged = True
                 stop_reason = "residual below residual_tol"
                 break

@@ -119,6 +119,24 @@ def hdiv_apply(phi: torch.Tensor, geom: StencilGeometry, out: torch.Tensor | Non
     return out.reshape(-1) if squeeze else out
 
 
+def hdiv_apply_adjoint(x: torch.Tensor, geom: StencilGeometry, out: torch.Tensor | None = None,
+                       work: torch.Tensor | None = None) -> torch.Tensor:
+    """``A^T x`` (the Euclidean transpose of K1: ``invw`` applied on the input) for a block ``(k, N)``."""
+    squeeze = x.dim() == 1
+    block = x.reshape(1, -1) if squeeze else x
+    k = block.shape[0]
+    require(block, (k, geom.n_points), geom.dtype, geom.device, "x")
+    out = torch.empty_like(block) if out is None else out
+    work = torch.empty_like(block) if work is None else work
+    require(out, (k, geom.n_points), geom.dtype, geom.device, "out")
+    require(work, (k, geom.n_points), geom.dtype, geom.device, "work")
+    n0, n1, n2 = geom.shape
+    _lib.call(f"cdft_hdiv_apply_adjoint_{_suffix(geom.dtype)}", block.data_ptr(), geom.w0.data_ptr(), geom.w1.data_ptr(),
+              geom.w2.data_ptr(), geom.invw.data_ptr(), geom.potential.data_ptr(), out.data_ptr(), work.data_ptr(), k,
+              n0, n1, n2, ctypes.addressof(geom.taps), geom.order, stream_of(block))
+    return out.reshape(-1) if squeeze else out
+
+
 def hdiv_set_segment(planes: int) -> None:
     """Planes per x-segment of the marching stencil kernels (0 restores the automatic choice)."""
     _lib.call("cdft_hdiv_set_segment", int(planes))

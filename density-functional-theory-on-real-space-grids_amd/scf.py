@@ -158,13 +158,15 @@ class KohnShamStep:
     def hamiltonian(self, pot: dict) -> CuspFactoredHamiltonian:
         return CuspFactoredHamiltonian(self.grid, self.factor, self.q, extra_potential=pot["effective"])
 
-    def diagonalise(self, solver: ChebyshevFilteredSubspace, h, initial=None, max_iterations=None, bounds=None):
+    def diagonalise(self, solver: ChebyshevFilteredSubspace, h, initial=None, max_iterations=None, bounds=None,
+                    precision="auto", min_fp64_passes=0):
         gen = SeededDraws(self.numerics.seed, self.draws)
         saved = solver.config.max_iterations
         if max_iterations is not None:
             solver.config.max_iterations = max_iterations
         try:
-            return solver.solve(h, self.n_states, initial=initial, generator=gen, bounds=bounds)
+            return solver.solve(h, self.n_states, initial=initial, generator=gen, bounds=bounds, precision=precision,
+                                min_fp64_passes=min_fp64_passes)
         finally:
             solver.config.max_iterations = saved
 
@@ -348,12 +350,16 @@ def initial_density(step: KohnShamStep) -> torch.Tensor:
     return ops.ewise(ops.EW_SCALE, g.empty(), total, s=scale).reshape(1, -1)
 
 
+#: Float64 filter passes that finish a mixed-precision solve.
+REFINEMENT_PASSES = 2
+
+
 def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None, filter_dtype=torch.float64,
                           filter_hook=None):
     """The SCF driver (reference ``scf/loop.py:255-460``). ``filter_dtype=float32`` runs the Chebyshev
-    filter in float32 while the eigen-residual is above the solver's switch and finishes every solve
-    with float64 passes; Rayleigh-Ritz, orthonormalisation, densities, Poisson, XC and energies stay
-    float64."""
+    filter of every warm SCF pass in float32 (and the cold first solve until its residual is small),
+    and the final diagonalisation in float64 with at least ``REFINEMENT_PASSES`` passes; Rayleigh-Ritz,
+    orthonormalisation, densities, Poisson, XC and energies stay float64 throughout."""
     started = time.perf_counter()
     grid, factor, q = geometry if geometry is not None else build_geometry(scenario, numerics, device, anchor)
     step = KohnShamStep(scenario, numerics, grid, factor, q)
@@ -392,7 +398,10 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
             hint = SpectralBoundsHint(policy.bounds[0], policy.bounds[1], float(norm_buf[2].item()))
         elif reason is None:
             reason = "no previous potential"
-        eigen = step.diagonalise(solver, h, initial=initial_block, max_iterations=steps, bounds=hint)
+        # Mixed precision: the warm passes of an iteration filter in float32; the first (cold) solve
+        # switches to float64 once its residual is small.
+        eigen = step.diagonalise(solver, h, initial=initial_block, max_iterations=steps, bounds=hint,
+                                 precision="auto" if steps is None else "f32")
         policy.observe(reason, eigen.info)
         previous_effective = pin["effective"]
         rung_changed = False
@@ -434,7 +443,10 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
             ops.lincomb(n_next.view(-1), [(n_in.reshape(-1), 1.0), (residual, 0.5 * numerics.alpha)])
         n_in = n_next
     h_final = step.hamiltonian(pin)
-    eigen = step.diagonalise(solver, h_final, initial=eigen.eigenvectors)
+    # The final diagonalisation is float64 throughout; a mixed solve refines its float32-filtered
+    # vectors with at least REFINEMENT_PASSES float64 filter passes.
+    eigen = step.diagonalise(solver, h_final, initial=eigen.eigenvectors, precision="f64",
+                             min_fp64_passes=REFINEMENT_PASSES if solver.mixed else 0)
     occ = aufbau(eigen.eigenvalues, step.n_electrons)
     n_out, rho_out = step.output_density(eigen, occ)
     pout = step.potentials(n_out)
@@ -459,6 +471,7 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
         "graphs": dict(solver.graph_record),
         "wall_seconds": time.perf_counter() - started,
         "step": step,
+        "effective_potential": pin["effective"],
     }
 
 

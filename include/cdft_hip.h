@@ -39,6 +39,17 @@ int cdft_hdiv_apply_f32(const float* phi, const float* w0, const float* w1, cons
                         const float* invw, const float* pot, float* out, int k, int n0, int n1,
                         int n2, const double* coef, int order, void* stream);
 
+/* The Euclidean transpose of K1 for the backward pass: out = 0.5 sum_d D_d^T[w_d D_d (invw x)]
+ * + pot x (invw applied on the input). work: a distinct (k, N) scratch block. No reference
+ * counterpart (the reference's inversion is a stub, src/cdft/inversion/__init__.py:1; interface
+ * InversionProtocol contract.py:1263-1283). */
+int cdft_hdiv_apply_adjoint_f64(const double* x, const double* w0, const double* w1, const double* w2,
+                                const double* invw, const double* pot, double* out, double* work, int k,
+                                int n0, int n1, int n2, const double* coef, int order, void* stream);
+int cdft_hdiv_apply_adjoint_f32(const float* x, const float* w0, const float* w1, const float* w2,
+                                const float* invw, const float* pot, float* out, float* work, int k,
+                                int n0, int n1, int n2, const double* coef, int order, void* stream);
+
 /* K2: one fused scaled-Chebyshev step, out = (A y - c y) a - b prev with (c, a, b) = table[step]
  * read on the device (table is a (degree, 3) float64 device array), prev ignored at step 0.
  * Replaces chebyshev_update (src/cdft/operators/fused.py:391-417) and the body of
