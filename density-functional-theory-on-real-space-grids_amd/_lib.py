@@ -31,6 +31,9 @@ SIGNATURES: dict[str, list] = {
     "cdft_cheb_filter_f32": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
     "cdft_hdiv_apply_adjoint_f64": [P, P, P, P, P, P, P, P, I, I, I, I, P, I, P],
     "cdft_hdiv_apply_adjoint_f32": [P, P, P, P, P, P, P, P, I, I, I, I, P, I, P],
+    "cdft_batch_plan_f64": [I, P, I, I, P, P, P, L, P],
+    "cdft_cheb_filter_batched_f64": [P, I, P, I, I, P],
+    "cdft_hdiv_apply_batched_f64": [P, I, P, I, P],
     "cdft_hdiv_set_segment": [I],
     "cdft_hdiv_launch_shape": [I, I, I, I, I, P],
     "cdft_wgram_f64": [P, I, P, I, P, L, D, I, P, L, P, P],
@@ -69,6 +72,7 @@ VALUED = {
     "cdft_reduction_workspace_doubles": (ctypes.c_long, [L]),
     "cdft_reduce_workspace_doubles": (ctypes.c_long, [L]),
     "cdft_lift_workspace_doubles": (ctypes.c_long, [P, I]),
+    "cdft_batch_desc_bytes": (ctypes.c_long, [I]),
 }
 
 

@@ -73,25 +73,36 @@ struct March {
 // consecutive lanes touch 16 consecutive bank pairs: the plane-tile, y-face and z-face accesses are
 // free of bank conflicts. Consecutive blocks take the vectors of one tile, so the tile's geometry
 // is reused from L2 by all of them.
+template <typename T, int P, int TZ>
+struct StencilLds {
+  static constexpr int R = 2 * P - 1;
+  static constexpr int LZ = TZ + 16;  // plane-tile pitch (>= TZ + 2R)
+  static constexpr int YP = TZ;       // y-face pitch
+  static constexpr int ZP = TZ + 16;  // z-face pitch (>= TZ + R)
+  T phi[(TMAX + 2 * R) * LZ];
+  T yf[(TMAX + R) * YP];
+  T zf[TMAX * ZP];
+};
+
+// One block's work: x-segment `bs` of (y, z) tile (by, bz) of vector v (tile = bz + tiles_z (by +
+// tiles_y bs)). Shared by the single-grid and the batched kernels.
 template <typename T, int P, int MODE, int TZ>
-__global__ __launch_bounds__(NTMAX) void hdiv_kernel(
+__device__ __forceinline__ void stencil_block(
     const T* __restrict__ phi, const T* __restrict__ prev, T* __restrict__ out,
     const T* __restrict__ w0, const T* __restrict__ w1, const T* __restrict__ w2,
-    const T* __restrict__ invw, const T* __restrict__ pot, int n0, int n1, int n2, int k,
-    Taps<P> taps, const double* __restrict__ table, int step, March md) {
+    const T* __restrict__ invw, const T* __restrict__ pot, int n0, int n1, int n2,
+    const Taps<P>& taps, const double* __restrict__ table, int step, const March& md, unsigned tile, int v,
+    StencilLds<T, P, TZ>& lds) {
   constexpr int R = 2 * P - 1;  // stencil reach: 2P-1 points each side
   static_assert(2 * R <= 16, "halo must fit the 16-double pitch pad");
   constexpr int LZ = TZ + 16;   // plane-tile pitch (>= TZ + 2R)
   constexpr int YP = TZ;        // y-face pitch
   constexpr int ZP = TZ + 16;   // z-face pitch (>= TZ + R)
-  __shared__ T s_phi[(TMAX + 2 * R) * LZ];
-  __shared__ T s_yf[(TMAX + R) * YP];
-  __shared__ T s_zf[TMAX * ZP];
+  T* s_phi = lds.phi;
+  T* s_yf = lds.yf;
+  T* s_zf = lds.zf;
 
   const int TY = md.ty;
-  const unsigned lin = xcd_tile(blockIdx.x, gridDim.x);
-  const unsigned tile = lin / (unsigned)k;
-  const int v = (int)(lin - tile * (unsigned)k);
   const int bz = (int)(tile % md.tiles_z);
   const int by = (int)((tile / md.tiles_z) % md.tiles_y);
   const int bs = (int)(tile / ((unsigned)md.tiles_z * md.tiles_y));
@@ -302,6 +313,70 @@ __global__ __launch_bounds__(NTMAX) void hdiv_kernel(
     qphi[2 * P - 1] = cur.phn;
     cur = nxt;
   }
+}
+
+template <typename T, int P, int MODE, int TZ>
+__global__ __launch_bounds__(NTMAX) void hdiv_kernel(
+    const T* __restrict__ phi, const T* __restrict__ prev, T* __restrict__ out,
+    const T* __restrict__ w0, const T* __restrict__ w1, const T* __restrict__ w2,
+    const T* __restrict__ invw, const T* __restrict__ pot, int n0, int n1, int n2, int k,
+    Taps<P> taps, const double* __restrict__ table, int step, March md) {
+  __shared__ StencilLds<T, P, TZ> lds;
+  const unsigned lin = xcd_tile(blockIdx.x, gridDim.x);
+  const unsigned tile = lin / (unsigned)k;
+  const int v = (int)(lin - tile * (unsigned)k);
+  stencil_block<T, P, MODE, TZ>(phi, prev, out, w0, w1, w2, invw, pot, n0, n1, n2, taps, table, step, md, tile, v,
+                                lds);
+}
+
+// ---- batched launches: many grids (one descriptor each) in one launch ----
+
+// One grid of a batch. The filter rotates through out / work1 / work2 exactly as the single-grid
+// filter does, so the batched steps are bitwise the single-grid steps.
+template <typename T>
+struct BatchGrid {
+  const T* x;
+  T* out;
+  T* work1;
+  T* work2;
+  const T *w0, *w1, *w2, *invw, *pot;
+  const double* table;  // (degree, 3) Chebyshev rows of this grid
+  double taps[8];
+  int n0, n1, n2, k;
+  March md;
+  unsigned block_start;  // first block of this grid in the launch
+  unsigned blocks;
+};
+
+template <typename T, int P, int MODE, int TZ>
+__global__ __launch_bounds__(NTMAX) void hdiv_batched_kernel(const BatchGrid<T>* __restrict__ grids, int n_grids,
+                                                             int step, int degree) {
+  __shared__ StencilLds<T, P, TZ> lds;
+  const unsigned lin = xcd_tile(blockIdx.x, gridDim.x);
+  int lo = 0, hi = n_grids - 1;  // the grid whose block range holds lin (block-uniform search)
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (grids[mid].block_start <= lin) lo = mid; else hi = mid - 1;
+  }
+  const BatchGrid<T>& g = grids[lo];
+  const unsigned local = lin - g.block_start;
+  const unsigned tile = local / (unsigned)g.k;
+  const int v = (int)(local - tile * (unsigned)g.k);
+  Taps<P> taps;
+#pragma unroll
+  for (int i = 0; i < 2 * P; ++i) taps.c[i] = g.taps[i];
+  const T* y = g.x;
+  const T* prev = nullptr;
+  T* dst = g.out;
+  if (MODE != 0) {
+    T* bufs[3] = {g.out, g.work1, g.work2};
+    const int offset = ((-(degree - 1)) % 3 + 3) % 3;
+    dst = bufs[(step + offset) % 3];
+    y = step == 0 ? g.x : bufs[(step - 1 + offset) % 3];
+    prev = step <= 1 ? g.x : bufs[(step - 2 + offset) % 3];
+  }
+  stencil_block<T, P, MODE, TZ>(y, prev, dst, g.w0, g.w1, g.w2, g.invw, g.pot, g.n0, g.n1, g.n2, taps, g.table, step,
+                                g.md, tile, v, lds);
 }
 
 static int ceil_div(int a, int b) { return (a + b - 1) / b; }
@@ -550,4 +625,114 @@ extern "C" int cdft_hdiv_launch_shape(int n0, int n1, int n2, int k, int order, 
   shape6[4] = d.threads;
   shape6[5] = (int)(d.tiles * (unsigned)k);
   return CDFT_OK;
+}
+
+// ---- batched grids ----
+
+namespace cdft {
+
+template <typename T, int P, int MODE>
+static int launch_batched_p(const void* desc, int n_grids, const int* launch, int step, int degree, hipStream_t s) {
+  const dim3 grid((unsigned)launch[0]), block((unsigned)launch[1]);
+  const BatchGrid<T>* g = (const BatchGrid<T>*)desc;
+  if (launch[2] == 13)
+    hipLaunchKernelGGL((hdiv_batched_kernel<T, P, MODE, 13>), grid, block, 0, s, g, n_grids, step, degree);
+  else
+    hipLaunchKernelGGL((hdiv_batched_kernel<T, P, MODE, 16>), grid, block, 0, s, g, n_grids, step, degree);
+  return check_launch("hdiv_batched_kernel");
+}
+
+template <typename T, int MODE>
+static int launch_batched(const void* desc, int n_grids, const int* launch, int order, int step, int degree,
+                          hipStream_t s) {
+  switch (order) {
+    case 2: return launch_batched_p<T, 1, MODE>(desc, n_grids, launch, step, degree, s);
+    case 4: return launch_batched_p<T, 2, MODE>(desc, n_grids, launch, step, degree, s);
+    case 6: return launch_batched_p<T, 3, MODE>(desc, n_grids, launch, step, degree, s);
+    case 8: return launch_batched_p<T, 4, MODE>(desc, n_grids, launch, step, degree, s);
+    default: return fail(CDFT_E_ARG, "batched: unsupported stencil order %d", order);
+  }
+}
+
+static int check_batch_launch(const void* desc, int n_grids, const int* launch, int order) {
+  if (!desc || !launch || n_grids < 1) return fail(CDFT_E_ARG, "batched: null descriptor or no grids");
+  if (launch[0] < 1 || launch[1] < 64 || launch[1] > NTMAX || (launch[2] != 13 && launch[2] != 16))
+    return fail(CDFT_E_ARG, "batched: launch {%d, %d, %d} is not one cdft_batch_plan wrote", launch[0], launch[1],
+                launch[2]);
+  if (order != 2 && order != 4 && order != 6 && order != 8) return fail(CDFT_E_ARG, "batched: bad order %d", order);
+  return CDFT_OK;
+}
+
+}  // namespace cdft
+
+extern "C" long cdft_batch_desc_bytes(int n_grids) {
+  return n_grids < 1 ? 0L : (long)n_grids * (long)sizeof(BatchGrid<double>);
+}
+
+extern "C" int cdft_batch_plan_f64(int n_grids, const int* shapes, int k, int order, const double* coefs,
+                                   const void* const* ptrs, void* desc, long desc_bytes, int* launch) {
+  if (n_grids < 1 || !shapes || !coefs || !ptrs || !desc || !launch) return fail(CDFT_E_ARG, "batch_plan: null argument");
+  if (desc_bytes < cdft_batch_desc_bytes(n_grids)) return fail(CDFT_E_ARG, "batch_plan: descriptor buffer too small");
+  if (order != 2 && order != 4 && order != 6 && order != 8) return fail(CDFT_E_ARG, "batch_plan: bad order %d", order);
+  if (k < 1 || k > 65535) return fail(CDFT_E_ARG, "batch_plan: bad block width %d", k);
+  BatchGrid<double>* g = (BatchGrid<double>*)desc;
+  unsigned long start = 0;
+  int threads = 64, tz = 0;
+  for (int i = 0; i < n_grids; ++i) {
+    const int n0 = shapes[3 * i], n1 = shapes[3 * i + 1], n2 = shapes[3 * i + 2];
+    if (n0 < 1 || n1 < 1 || n2 < 1) return fail(CDFT_E_ARG, "batch_plan: grid %d has a bad shape", i);
+    if ((long)(n0 + order) * (n1 + order) * (n2 + order) * 8L >= 0x7fffffffL)
+      return fail(CDFT_E_ARG, "batch_plan: grid %d too large for 32-bit offsets", i);
+    const March d = march_dims(n0, n1, n2, k, order - 1);
+    if (d.threads > NTMAX) return fail(CDFT_E_ARG, "batch_plan: grid %d needs %d threads", i, d.threads);
+    if (tz == 0) tz = d.tz;
+    if (d.tz != tz) return fail(CDFT_E_ARG, "batch_plan: grid %d has tile width %d, the batch %d", i, d.tz, tz);
+    threads = d.threads > threads ? d.threads : threads;
+    const void* const* p = ptrs + 10 * i;
+    for (int j = 0; j < 10; ++j)
+      if (!p[j]) return fail(CDFT_E_ARG, "batch_plan: grid %d pointer %d is null", i, j);
+    g[i].x = (const double*)p[0];
+    g[i].out = (double*)p[1];
+    g[i].work1 = (double*)p[2];
+    g[i].work2 = (double*)p[3];
+    g[i].w0 = (const double*)p[4];
+    g[i].w1 = (const double*)p[5];
+    g[i].w2 = (const double*)p[6];
+    g[i].invw = (const double*)p[7];
+    g[i].pot = (const double*)p[8];
+    g[i].table = (const double*)p[9];
+    for (int j = 0; j < 8; ++j) g[i].taps[j] = j < order ? coefs[order * i + j] : 0.0;
+    g[i].n0 = n0;
+    g[i].n1 = n1;
+    g[i].n2 = n2;
+    g[i].k = k;
+    g[i].md = d;
+    g[i].block_start = (unsigned)start;
+    g[i].blocks = d.tiles * (unsigned)k;
+    start += (unsigned long)d.tiles * (unsigned long)k;
+    if (start > 0x7fffffffUL) return fail(CDFT_E_ARG, "batch_plan: too many blocks");
+  }
+  launch[0] = (int)start;
+  launch[1] = threads;
+  launch[2] = tz;
+  return CDFT_OK;
+}
+
+extern "C" int cdft_cheb_filter_batched_f64(const void* desc, int n_grids, const int* launch, int order, int degree,
+                                            void* stream) {
+  int rc = check_batch_launch(desc, n_grids, launch, order);
+  if (rc) return rc;
+  if (degree < 1) return fail(CDFT_E_ARG, "cheb_filter_batched: degree must be >= 1");
+  for (int s = 0; s < degree; ++s) {
+    rc = s == 0 ? launch_batched<double, 1>(desc, n_grids, launch, order, s, degree, (hipStream_t)stream)
+                : launch_batched<double, 2>(desc, n_grids, launch, order, s, degree, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return CDFT_OK;
+}
+
+extern "C" int cdft_hdiv_apply_batched_f64(const void* desc, int n_grids, const int* launch, int order, void* stream) {
+  int rc = check_batch_launch(desc, n_grids, launch, order);
+  if (rc) return rc;
+  return launch_batched<double, 0>(desc, n_grids, launch, order, 0, 1, (hipStream_t)stream);
 }

@@ -137,6 +137,13 @@ def lanczos_bounds(hamiltonian, start: torch.Tensor, steps: int, safety: float) 
     """Short Lanczos run with one reorthogonalisation pass against the previous two vectors; the
     alphas and betas stay on the device and are read in one transfer (reference
     ``operators/hamiltonian.py:29-89``)."""
+    return lanczos_finish(lanczos_launch(hamiltonian, start, steps).cpu().numpy(), steps, safety)
+
+
+def lanczos_launch(hamiltonian, start: torch.Tensor, steps: int) -> torch.Tensor:
+    """The device part of :func:`lanczos_bounds`: enqueue the run and return the device array whose
+    first ``2 steps`` entries are the alphas and betas (read it when the bounds are needed, so
+    several runs can share one transfer)."""
     m = hamiltonian.measure
     dev = start.device
     n = start.numel()
@@ -161,7 +168,12 @@ def lanczos_bounds(hamiltonian, start: torch.Tensor, steps: int, safety: float) 
         m.norm_into(w, scal[b_slot(i):b_slot(i) + 1])
         v_new = ops.lincomb(torch.empty_like(v), [(w, 1.0, 2, b_slot(i))], scal)
         v_prev, v = v, v_new
-    values = scal.cpu().numpy()
+    return scal
+
+
+def lanczos_finish(values: np.ndarray, steps: int, safety: float) -> tuple[float, float]:
+    """The host part of :func:`lanczos_bounds`: the tridiagonal's extreme Ritz values widened by the
+    last beta and by ``safety``."""
     alphas = list(values[:steps])
     betas = list(values[steps:2 * steps])
     for index, value in enumerate(betas):

@@ -75,6 +75,26 @@ int cdft_cheb_filter_f32(const float* x, float* out, float* work1, float* work2,
                          int k, int n0, int n1, int n2, const double* coef, int order,
                          const double* table, int degree, void* stream);
 
+/* ---- batched grids: many independent operators in one launch ----
+ * The batched scan of independent solves that the reference runs as separate processes or streams
+ * (scripts/scan_throughput.py:212-313); no single reference function corresponds. */
+
+/* Bytes of the device descriptor table of n_grids grids. */
+long cdft_batch_desc_bytes(int n_grids);
+/* Host-only planning: fills `desc` (host buffer of cdft_batch_desc_bytes bytes, to be copied to the
+ * device unchanged) for n_grids grids of one stencil order and block width k. shapes (host,
+ * n_grids x 3), coefs (host, n_grids x order staggered weights / h), ptrs (host, n_grids x 10 device
+ * pointers: x, out, work1, work2, w0, w1, w2, invw, pot, Chebyshev table (degree x 3)). launch (host,
+ * 3 ints) receives {blocks, threads, tile width}; every grid must get the same tile width. */
+int cdft_batch_plan_f64(int n_grids, const int* shapes, int k, int order, const double* coefs,
+                        const void* const* ptrs, void* desc, long desc_bytes, int* launch);
+/* K2 for every grid of a batch: each grid's degree-`degree` filter of x into out through work1 /
+ * work2 with its own table, one launch per step for all grids (bitwise the single-grid filter). */
+int cdft_cheb_filter_batched_f64(const void* desc, int n_grids, const int* launch, int order, int degree,
+                                 void* stream);
+/* K1 for every grid of a batch: out = A x. */
+int cdft_hdiv_apply_batched_f64(const void* desc, int n_grids, const int* launch, int order, void* stream);
+
 /* ---- K3 / K4: float64 reductions and rotations of Rayleigh-Ritz (deterministic, no atomics) ---- */
 
 /* Doubles of block-partials workspace the reductions below need for length n. */
