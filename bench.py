@@ -299,6 +299,7 @@ def batched_scan(device, count=64, seed=0, spacing=0.25):
             "speedup_batched_vs_sequential": t_solo / t_batch, "solves_per_s_batched": count / t_batch,
             "lockstep_iterations": solver.iterations, "blocks_per_filter_launch": solver.blocks_per_launch,
             "batched_phases_s": solver.timings,
+            "autotuned_segment": solver.tuning,
             "sequential_iterations_total": int(sum(s["n_iterations"] for s in solo)),
             "graph": solver.graph_record,
             "max_abs_eigenvalue_diff_vs_sequential": max(diffs),

@@ -108,7 +108,7 @@ class BatchedCheFSI:
     """Lockstep CheFSI for the lowest states of many independent operators (one per grid)."""
 
     def __init__(self, hamiltonians: list, config: EigenConfig, n_states: int = 1, ritz_stop: bool = True,
-                 seed: int = 0, use_graphs: bool = True):
+                 seed: int = 0, use_graphs: bool = True, autotune: bool = True):
         self.h = hamiltonians
         self.config = config
         self.n_states = n_states
@@ -117,6 +117,8 @@ class BatchedCheFSI:
         self.use_graphs = use_graphs
         self.graph_record = {"captures": 0, "replays": 0}
         self.timings: dict[str, float] = {}
+        self.autotune = autotune
+        self.tuning: dict | None = None
 
     def _iteration_body(self, insts, plan_filter, plan_rr, plan_res, degree):
         cfg = self.config
@@ -199,7 +201,20 @@ class BatchedCheFSI:
         self.timings["start_draws_s"] = t_draw
         self.timings["start_lanczos_s"] = t_lanczos
         geoms = [(it.h.geometry, it.work) for it in insts]
-        plan_filter = BatchPlan([(g, w.vectors, w.filtered, w.work1, w.work2, w.table) for g, w in geoms], k)
+        filter_rows = [(g, w.vectors, w.filtered, w.work1, w.work2, w.table) for g, w in geoms]
+        segment = 0
+        if self.autotune:
+            # The tables are still zero here, so the timed filters only overwrite scratch blocks.
+            from . import autotune
+
+            signature = ("batched-filter", tuple(it.h.grid.shape for it in insts), k, degree)
+            self.tuning = autotune.tune(signature, lambda: BatchPlan(filter_rows, k).filter(degree, insts[0].work.vectors))
+            segment = self.tuning["best"]
+        ops.hdiv_set_segment(segment)
+        try:
+            plan_filter = BatchPlan(filter_rows, k)
+        finally:
+            ops.hdiv_set_segment(0)
         plan_rr = BatchPlan([(g, w.basis, w.applied, w.work1, w.work2, w.table) for g, w in geoms], k)
         plan_res = BatchPlan([(g, w.vectors, w.applied, w.work1, w.work2, w.table) for g, w in geoms], k)
         self.blocks_per_launch = plan_filter.blocks

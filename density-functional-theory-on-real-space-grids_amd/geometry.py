@@ -11,6 +11,7 @@ transpose are matrix-free kernels (no CSR operator is stored).
 from __future__ import annotations
 
 import ctypes
+import functools
 import math
 from dataclasses import dataclass
 
@@ -23,9 +24,14 @@ from .ops import host_doubles, stream_of
 from .stencil import central_coefficients, staggered_coefficients
 
 
+@functools.lru_cache(maxsize=None)
 def gauss_legendre(order: int) -> tuple[np.ndarray, np.ndarray]:
-    """The ``order``-point Gauss-Legendre rule on [-1, 1] (numpy's Golub-Welsch + Newton polish)."""
-    return leggauss(int(order))
+    """The ``order``-point Gauss-Legendre rule on [-1, 1] (numpy's Golub-Welsch + Newton polish),
+    computed once per order (read-only arrays)."""
+    x, w = leggauss(int(order))
+    x.setflags(write=False)
+    w.setflags(write=False)
+    return x, w
 
 
 def _ints(values) -> ctypes.Array:

@@ -31,8 +31,9 @@ class SeededDraws:
     """A freshly seeded CPU generator whose draws are memoised by their position in the draw
     sequence. Every diagonalisation of one self-consistent solve re-seeds the same generator, so
     its draws repeat exactly; the memo (shared by the solve through ``store``) turns the repeats
-    into device-to-device copies of the first draw instead of fresh CPU draws and uploads. The
-    values are bitwise those of ``torch.randn`` on ``torch.Generator().manual_seed(seed)``."""
+    into device-to-device copies of the first draw instead of fresh CPU draws and uploads, and
+    :func:`prefetch_draws` can fill it on a host thread ahead of use. The values are bitwise those
+    of ``torch.randn`` on ``torch.Generator().manual_seed(seed)``."""
 
     def __init__(self, seed: int, store: dict):
         self.seed = int(seed)
@@ -41,8 +42,12 @@ class SeededDraws:
         self._gen = None
 
     def draw(self, size: tuple, device) -> torch.Tensor:
-        key = (self.seed, self.sequence + (size,), str(device))
+        key = (self.seed, self.sequence + (size,))
         hit = self.store.get(key)
+        if isinstance(hit, tuple):  # prefetched on a host thread: (future, position)
+            future, position = hit
+            hit = future.result()[position].to(device)
+            self.store[key] = hit
         if hit is None:
             if self._gen is None:
                 self._gen = torch.Generator(device="cpu").manual_seed(self.seed)
@@ -54,6 +59,29 @@ class SeededDraws:
             torch.randn(size, generator=self._gen, dtype=torch.float64)
         self.sequence = self.sequence + (size,)
         return hit.clone()
+
+
+_draw_pool = None
+
+
+def prefetch_draws(store: dict, seed: int, sequences) -> None:
+    """Start drawing each sequence of sizes (from a fresh generator seeded ``seed``) on a host
+    thread, and register every prefix in ``store`` for :class:`SeededDraws` to pick up."""
+    global _draw_pool
+    if _draw_pool is None:
+        import concurrent.futures
+
+        _draw_pool = concurrent.futures.ThreadPoolExecutor(max_workers=2, thread_name_prefix="seeded-draws")
+
+    def job(seq):
+        gen = torch.Generator(device="cpu").manual_seed(int(seed))
+        return [torch.randn(size, generator=gen, dtype=torch.float64) for size in seq]
+
+    for seq in sequences:
+        seq = tuple(tuple(s) for s in seq)
+        future = _draw_pool.submit(job, seq)
+        for i in range(len(seq)):
+            store.setdefault((int(seed), seq[: i + 1]), (future, i))
 
 
 class DeviceMeasure:

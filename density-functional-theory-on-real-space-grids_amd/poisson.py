@@ -3,7 +3,8 @@
 Mirrors ``CoulombCutoffPoisson`` (reference ``src/cdft/operators/poisson.py:27-120``): padded shape
 ``fft_friendly(ceil(pad * n))`` per axis, cutoff radius ``R = min(P_d h)/2``, kernel
 ``K(G) = 4 pi (1 - cos G R)/G^2`` with ``K(0) = 2 pi R^2``. The plan and the Green's-function table
-are built once; a solve is pad-scatter, D2Z, multiply, Z2D, gather (K5).
+are built once per grid and spacing (and shared by later solvers of the same grid); a solve is
+pad-scatter, D2Z, multiply, Z2D, gather (K5).
 """
 
 from __future__ import annotations
@@ -30,6 +31,38 @@ def fft_friendly(n: int) -> int:
         c += 1
 
 
+class _Plan:
+    """hipFFT plans, padded buffers, spectrum and Green's-function table of one (grid, padding,
+    spacing) on one device."""
+
+    def __init__(self, shape, padded_shape, spacing, radius, device):
+        p0, p1, p2 = padded_shape
+        n0, n1, n2 = shape
+        handle = ctypes.c_void_p()
+        _lib.call("cdft_poisson_create", n0, n1, n2, p0, p1, p2, ctypes.addressof(handle))
+        self.handle = handle
+        self.padded_in = torch.zeros((p0, p1, p2), dtype=torch.float64, device=device)  # padding stays zero
+        self.padded_out = torch.empty((p0, p1, p2), dtype=torch.float64, device=device)
+        self.spectrum = torch.empty((p0, p1, p2 // 2 + 1, 2), dtype=torch.float64, device=device)
+        self.kernel = torch.empty((p0, p1, p2 // 2 + 1), dtype=torch.float64, device=device)
+        _lib.call("cdft_poisson_kernel_f64", handle, spacing, radius, self.kernel.data_ptr(), stream_of(self.kernel))
+
+    def __del__(self):
+        handle = getattr(self, "handle", None)
+        if handle is not None and handle.value:
+            try:
+                _lib.call("cdft_poisson_destroy", handle)
+            except Exception:  # noqa: BLE001 - interpreter shutdown
+                pass
+            self.handle = None
+
+
+#: Plans are built once per (shape, padded shape, spacing, device) and shared by every solver of that
+#: key for the life of the process, as FFT libraries cache plans. Solvers sharing a plan share its
+#: scratch buffers, so their solves must be ordered (one stream), as every solve of one SCF is.
+_plans: dict[tuple, _Plan] = {}
+
+
 class CoulombCutoffPoisson:
     def __init__(self, grid, pad_factor: float = 2.5):
         if pad_factor < 2.0:
@@ -37,18 +70,16 @@ class CoulombCutoffPoisson:
         self.grid = grid
         self.padded_shape = tuple(fft_friendly(int(math.ceil(pad_factor * n))) for n in grid.shape)
         self.cutoff_radius = 0.5 * min(n * grid.spacing for n in self.padded_shape)
-        p0, p1, p2 = self.padded_shape
-        handle = ctypes.c_void_p()
-        n0, n1, n2 = grid.shape
-        _lib.call("cdft_poisson_create", n0, n1, n2, p0, p1, p2, ctypes.addressof(handle))
-        self._handle = handle
-        dev = grid.device
-        self.padded_in = torch.zeros((p0, p1, p2), dtype=torch.float64, device=dev)
-        self.padded_out = torch.empty((p0, p1, p2), dtype=torch.float64, device=dev)
-        self.spectrum = torch.empty((p0, p1, p2 // 2 + 1, 2), dtype=torch.float64, device=dev)
-        self.kernel = torch.empty((p0, p1, p2 // 2 + 1), dtype=torch.float64, device=dev)
-        _lib.call("cdft_poisson_kernel_f64", handle, grid.spacing, self.cutoff_radius, self.kernel.data_ptr(),
-                  stream_of(self.kernel))
+        key = (tuple(grid.shape), self.padded_shape, float(grid.spacing), str(grid.device))
+        plan = _plans.get(key)
+        if plan is None:
+            plan = _plans[key] = _Plan(grid.shape, self.padded_shape, grid.spacing, self.cutoff_radius, grid.device)
+        self._plan = plan
+        self._handle = plan.handle
+        self.padded_in = plan.padded_in
+        self.padded_out = plan.padded_out
+        self.spectrum = plan.spectrum
+        self.kernel = plan.kernel
 
     def solve(self, density: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """``v_H`` on the grid for a total density ``(N,)``."""
@@ -60,11 +91,3 @@ class CoulombCutoffPoisson:
                   stream_of(density))
         return out
 
-    def __del__(self):
-        handle = getattr(self, "_handle", None)
-        if handle is not None and handle.value:
-            try:
-                _lib.call("cdft_poisson_destroy", handle)
-            except Exception:  # noqa: BLE001 - interpreter shutdown
-                pass
-            self._handle = None

@@ -20,7 +20,7 @@ import torch
 from . import _lib, ops
 from .eigen import ChebyshevFilteredSubspace, EigenConfig, SpectralBoundsHint
 from .geometry import CuspFactor, CuspQuadrature, UniformGrid
-from .hamiltonian import CuspFactoredHamiltonian, SeededDraws
+from .hamiltonian import CuspFactoredHamiltonian, SeededDraws, prefetch_draws
 from .ops import stream_of
 from .poisson import CoulombCutoffPoisson
 
@@ -361,8 +361,19 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
     and the final diagonalisation in float64 with at least ``REFINEMENT_PASSES`` passes; Rayleigh-Ritz,
     orthonormalisation, densities, Poisson, XC and energies stay float64 throughout."""
     started = time.perf_counter()
+    store: dict = {}
+    if geometry is None:
+        # The start vectors a solve draws (the cold block then its Lanczos start; a warm solve's
+        # Lanczos start) are known before the geometry exists: draw them on a host thread meanwhile.
+        counts = [int(round(length / numerics.spacing)) + 1 for length in numerics.box_lengths]
+        npts = counts[0] * counts[1] * counts[2]
+        n_states = max(1, int(-(-float(scenario.n_electrons) // 2.0)))
+        cfg = numerics.eigen
+        k = min(n_states + max(cfg.n_extra_states, int(cfg.extra_states_fraction * n_states)), npts)
+        prefetch_draws(store, numerics.seed, [((k, npts), (npts,)), ((npts,),)])
     grid, factor, q = geometry if geometry is not None else build_geometry(scenario, numerics, device, anchor)
     step = KohnShamStep(scenario, numerics, grid, factor, q)
+    step.draws = store
     g = grid
     inner_weights = ops.ewise(ops.EW_DIV, g.empty(), q.mass_weights, ops.ewise(ops.EW_MUL, g.empty(), step.f2, step.f2))
     solver = ChebyshevFilteredSubspace(grid, numerics.eigen, ritz_stop=False, filter_dtype=filter_dtype,

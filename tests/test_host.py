@@ -1,0 +1,73 @@
+"""Host-side logic that needs no GPU: the memoised seeded draws (the reference's start vectors),
+the stencil weights, the Chebyshev table, the batched-scan geometry rule and the lumped-weight
+axis tables."""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+def _torch_sequence(seed, sizes):
+    gen = torch.Generator().manual_seed(seed)
+    return [torch.randn(s, generator=gen, dtype=torch.float64) for s in sizes]
+
+
+def test_seeded_draws_reproduce_torch_with_memo_and_prefetch(pkg):
+    from cdft_amd.hamiltonian import SeededDraws, prefetch_draws, seeded_randn
+
+    store: dict = {}
+    prefetch_draws(store, 0, [((5, 300), (300,)), ((300,),)])
+    for sizes in ([(5, 300), (300,)], [(300,)], [(5, 300), (300,), (2, 300)], [(300,), (300,)]):
+        expect = _torch_sequence(0, sizes)
+        for _ in range(2):  # second pass is served from the memo
+            draws = SeededDraws(0, store)
+            got = [seeded_randn(s, draws, "cpu") for s in sizes]
+            assert all(torch.equal(a, b) for a, b in zip(got, expect))
+
+
+def test_staggered_weights_closed_forms(pkg):
+    from cdft_amd.stencil import staggered_coefficients
+
+    assert list(staggered_coefficients(2)) == [-1.0, 1.0]
+    assert np.allclose(staggered_coefficients(4), [1 / 24, -9 / 8, 9 / 8, -1 / 24], rtol=0, atol=1e-16)
+
+
+def test_chebyshev_rows_match_the_oracle(pkg):
+    from cdft_amd.chefsi import chebyshev_coefficients
+    from oracle.eigen import chebyshev_coefficients as oracle_rows
+
+    assert np.array_equal(np.asarray(chebyshev_coefficients(20, -0.3, 40.0, -2.0)),
+                          np.asarray(oracle_rows(20, -0.3, 40.0, -2.0)))
+
+
+def test_bond_scan_rule(pkg):
+    from cdft_amd.batch import bond_scan_geometries
+
+    scan = bond_scan_geometries(count=64, seed=0)
+    bonds = [r for r, _, _ in scan]
+    assert len(bonds) == 64 and 0.8 <= min(bonds) and max(bonds) <= 6.0
+    for r, sc, box in scan:
+        cells = box[0] / 0.25
+        assert abs(cells - round(cells)) < 1e-9 and box[0] >= r + 14.0 and box[1:] == (14.0, 14.0)
+        assert sc.positions[0][0] == -0.5 * r and sc.positions[1][0] == 0.5 * r
+
+
+def test_lumped_axis_tables_match_the_oracle_matrices(pkg):
+    """The device's per-axis (first, weights, ranges) tables rebuild the oracle's dense transposed
+    interpolation matrix of ``quadrature.py:447-466``."""
+    from cdft_amd.lumped import axis_table
+
+    n, refine, h, origin = 21, 2, 0.4, -4.0
+    fine, first, w, ranges = axis_table(origin, h, n, refine, 7)
+    mat = np.zeros((n, n * refine))
+    for f in range(n * refine):
+        mat[first[f]:first[f] + 8, f] = w[f]
+    # every coarse row's nonzeros are exactly its fine range
+    for c in range(n):
+        nz = np.nonzero(mat[c])[0]
+        assert nz.min() >= ranges[c, 0] and nz.max() < ranges[c, 1]
+    # degree-7 reproduction: sum_c M[c, f] p(x_c) = p(x_f) for cubic p
+    xc = origin + h * np.arange(n)
+    p = lambda x: 1.0 + 0.3 * x - 0.05 * x**2 + 0.01 * x**3  # noqa: E731
+    assert np.allclose(mat.T @ p(xc), p(fine), rtol=0, atol=1e-12)
