@@ -49,7 +49,7 @@ SIGNATURES: dict[str, list] = {
     "cdft_far_factor_f64": [P, D, P, I, P, P, P],
     "cdft_node_stencils_f64": [P, L, P, D, P, P, P, P, P],
     "cdft_interpolate_f64": [P, I, P, P, L, P, D, P, P, P],
-    "cdft_lift_f64": [P, I, L, P, P, P, P, D, P, P, P, P, L, P, P],
+    "cdft_lift_f64": [P, I, L, P, P, P, D, P, P, I, P, P, P, P, P, L, P, P, L, P, P],
     "cdft_gradient_f64": [P, I, P, D, P, P, I, P, P],
     "cdft_far_fine_f64": [I, P, P, P, P, P, P, D, P, P],
     "cdft_interp_transpose_axis_f64": [P, P, I, I, P, P, P, P, P],
@@ -71,7 +71,6 @@ SIGNATURES: dict[str, list] = {
 VALUED = {
     "cdft_reduction_workspace_doubles": (ctypes.c_long, [L]),
     "cdft_reduce_workspace_doubles": (ctypes.c_long, [L]),
-    "cdft_lift_workspace_doubles": (ctypes.c_long, [P, I]),
     "cdft_batch_desc_bytes": (ctypes.c_long, [I]),
 }
 

@@ -278,81 +278,78 @@ struct CellBox {
   int c0, c1, c2;     // cell-box extents
 };
 
-// Stage 1 of the transpose, one block of 64 threads per z-column (b0, b1) of the cell box. Thread
-// t = 8 j0 + j1 accumulates Z[f][col][t][iz] = sum over the column's nodes of
-// x[f][n] * (wx_n[j0] * wy_n[j1]) * wz_n[iz - (b2_n - lo2)], iz indexing grid z = lo2 - 3 + iz. The
-// column's nodes are contiguous and sorted by b2 (nodes are numbered in base-cell order), so each
-// thread keeps a sliding register window of the 8 z-values a node touches and retires the lowest
-// one whenever b2 advances: no dynamic register indexing, no atomics, node data broadcast to the
-// block.
-__global__ __launch_bounds__(64) void lift_columns_kernel(const double* __restrict__ x, int nf, long m,
-                                                          const int* __restrict__ cell_start,
-                                                          const int* __restrict__ base,
-                                                          const double* __restrict__ lw, CellBox cb,
-                                                          int ez, double* __restrict__ z) {
-  const int col = blockIdx.x;
+// Stage 1 of the transpose, one block of 64 threads per chunk of at most 64 consecutive nodes of
+// one z-column (b0, b1) of the cell box. Nodes are numbered in base-cell order, so a chunk's nodes
+// are sorted by b2 and reach z-cells [zlo, zlo + hz) (relative to the box's lo2, 8 cells per node).
+// Thread t = 8 j0 + j1 accumulates Z[t][iz - zlo] = sum over the chunk's nodes of
+// x[n] (wx_n[j0] wy_n[j1]) wz_n[iz - b2_n] in a sliding window of 8 registers, retiring the lowest
+// z-cell whenever b2 advances: no dynamic register indexing, no atomics. Splitting columns into
+// chunks keeps the serial walk short (the columns through a nucleus hold ~1,100 nodes) and gives
+// ~1,000 blocks instead of one per column.
+template <int NF>
+__global__ __launch_bounds__(64) void lift_chunks_kernel(const double* __restrict__ x, long m,
+                                                         const int* __restrict__ base,
+                                                         const double* __restrict__ lw, int lo2,
+                                                         const int* __restrict__ chunk_node,
+                                                         const int* __restrict__ chunk_zlo,
+                                                         const int* __restrict__ chunk_hz,
+                                                         const int* __restrict__ chunk_off, long total,
+                                                         double* __restrict__ z) {
+  const int c = blockIdx.x;
   const int t = threadIdx.x;
   const int j0 = t >> 3, j1 = t & 7;
-  const long ncols = (long)cb.c0 * cb.c1;
-  const int s = cell_start[(long)col * cb.c2], e = cell_start[(long)(col + 1) * cb.c2];
-  double acc[4][8];
+  const int s = chunk_node[c], cnt = chunk_node[c + 1] - s;
+  const int zlo = chunk_zlo[c], hz = chunk_hz[c];
+  double* zc = z + chunk_off[c] + (long)t * hz;
+  __shared__ double s_lw[64][25];
+  __shared__ double s_x[NF][64];
+  __shared__ int s_b2[64];
+  const double* lc = lw + (long)s * 24;
 #pragma unroll
-  for (int f = 0; f < 4; ++f)
+  for (int q = 0; q < 24; ++q) {
+    const int idx = q * 64 + t;  // coalesced over the chunk's contiguous 24-double records
+    const int r = idx / 24;
+    if (r < cnt) s_lw[r][idx - 24 * r] = lc[idx];
+  }
+  if (t < cnt) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) s_x[f][t] = x[f * m + s + t];
+    s_b2[t] = base[3 * (s + t) + 2] - lo2;
+  }
+  __syncthreads();
+  double acc[NF][8];
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
 #pragma unroll
     for (int k = 0; k < 8; ++k) acc[f][k] = 0.0;
-  int cur = 0;
-  double* zc = z + ((long)col * 64 + t) * ez;
-  const long fstride = ncols * 64 * ez;
-  // Node records are staged through LDS 64 at a time (one per thread, independent loads), so the
-  // serial walk reads LDS broadcasts instead of waiting on a global load per node.
-  __shared__ double s_lw[64][25];
-  __shared__ double s_x[4][64];
-  __shared__ int s_b2[64];
-  for (int chunk = s; chunk < e; chunk += 64) {
-    const int cnt = min(64, e - chunk);
-    __syncthreads();
-    const double* lc = lw + (long)chunk * 24;
+  int cur = zlo;
+  for (int u = 0; u < cnt; ++u) {
+    const int b2 = s_b2[u];
+    while (cur < b2) {
 #pragma unroll
-    for (int q = 0; q < 24; ++q) {
-      const int idx = q * 64 + t;  // coalesced over the chunk's contiguous 24-double records
-      const int r = idx / 24;
-      if (r < cnt) s_lw[r][idx - 24 * r] = lc[idx];
-    }
-    if (t < cnt) {
-      const int node = chunk + t;
+      for (int f = 0; f < NF; ++f) {
+        zc[f * total + (cur - zlo)] = acc[f][0];
 #pragma unroll
-      for (int f = 0; f < 4; ++f) s_x[f][t] = (f < nf) ? x[f * m + node] : 0.0;
-      s_b2[t] = base[3 * node + 2] - cb.lo2;
-    }
-    __syncthreads();
-    for (int u = 0; u < cnt; ++u) {
-      const int b2 = s_b2[u];
-      while (cur < b2) {
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-          if (f < nf) zc[f * fstride + cur] = acc[f][0];
-#pragma unroll
-          for (int k = 0; k < 7; ++k) acc[f][k] = acc[f][k + 1];
-          acc[f][7] = 0.0;
-        }
-        ++cur;
+        for (int k = 0; k < 7; ++k) acc[f][k] = acc[f][k + 1];
+        acc[f][7] = 0.0;
       }
-      const double wxy = s_lw[u][j0] * s_lw[u][8 + j1];
-      double xv[4];
+      ++cur;
+    }
+    double wz[8];
 #pragma unroll
-      for (int f = 0; f < 4; ++f) xv[f] = s_x[f][u];
+    for (int k = 0; k < 8; ++k) wz[k] = s_lw[u][16 + k];
+    const double wxy = s_lw[u][j0] * s_lw[u][8 + j1];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const double w = wxy * s_lw[u][16 + k];
+    for (int k = 0; k < 8; ++k) {
+      const double w = wxy * wz[k];
 #pragma unroll
-        for (int f = 0; f < 4; ++f) acc[f][k] += xv[f] * w;
-      }
+      for (int f = 0; f < NF; ++f) acc[f][k] += s_x[f][u] * w;
     }
   }
-  while (cur < ez) {
+  while (cur < zlo + hz) {
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      if (f < nf) zc[f * fstride + cur] = acc[f][0];
+    for (int f = 0; f < NF; ++f) {
+      zc[f * total + (cur - zlo)] = acc[f][0];
 #pragma unroll
       for (int k = 0; k < 7; ++k) acc[f][k] = acc[f][k + 1];
       acc[f][7] = 0.0;
@@ -361,10 +358,14 @@ __global__ __launch_bounds__(64) void lift_columns_kernel(const double* __restri
   }
 }
 
-// Stage 2: out[f][p] = add[f][p] + sum_{j0, j1} Z[f][col(i0 - j0 + 3, i1 - j1 + 3)][8 j0 + j1][iz]
-// (j0 outer, j1 inner), or add / 0 outside the region the nodes reach.
-__global__ void lift_combine_kernel(const double* __restrict__ z, int nf, CellBox cb, int ez, GridDesc g,
-                                    const double* __restrict__ add, double* __restrict__ out) {
+// Stage 2: out[f][p] = add[f][p] + sum over (j0, j1) (j0 outer, j1 inner) and over the chunks of
+// column (i0 - j0 + 3, i1 - j1 + 3) in node order of the chunk's Z[8 j0 + j1][iz - zlo], iz =
+// i2 - lo2 + 3; add (or zero) where no node reaches.
+template <int NF>
+__global__ void lift_combine_kernel(const double* __restrict__ z, CellBox cb, GridDesc g,
+                                    const int* __restrict__ chunk_zlo, const int* __restrict__ chunk_hz,
+                                    const int* __restrict__ chunk_off, const int* __restrict__ col_chunk,
+                                    long total, const double* __restrict__ add, double* __restrict__ out) {
   const long n = (long)g.n0 * g.n1 * g.n2;
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -372,23 +373,29 @@ __global__ void lift_combine_kernel(const double* __restrict__ z, int nf, CellBo
   const int i1 = (int)((p / g.n2) % g.n1);
   const int i0 = (int)(p / ((long)g.n1 * g.n2));
   const int iz = i2 - (cb.lo2 - 3);
-  const long ncols = (long)cb.c0 * cb.c1;
-  const long fstride = ncols * 64 * ez;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  if (iz >= 0 && iz < ez) {
+  double acc[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) acc[f] = 0.0;
+  if (iz >= 0 && iz < cb.c2 + 7) {
     for (int j0 = 0; j0 < 8; ++j0) {
       const int b0 = i0 - j0 + 3 - cb.lo0;
       if (b0 < 0 || b0 >= cb.c0) continue;
       for (int j1 = 0; j1 < 8; ++j1) {
         const int b1 = i1 - j1 + 3 - cb.lo1;
         if (b1 < 0 || b1 >= cb.c1) continue;
-        const long idx = (((long)b0 * cb.c1 + b1) * 64 + j0 * 8 + j1) * ez + iz;
-        for (int f = 0; f < 4; ++f)
-          if (f < nf) acc[f] += z[f * fstride + idx];
+        const int col = b0 * cb.c1 + b1;
+        for (int c = col_chunk[col]; c < col_chunk[col + 1]; ++c) {
+          const int r = iz - chunk_zlo[c];
+          if (r < 0 || r >= chunk_hz[c]) continue;
+          const long idx = (long)chunk_off[c] + (long)(j0 * 8 + j1) * chunk_hz[c] + r;
+#pragma unroll
+          for (int f = 0; f < NF; ++f) acc[f] += z[f * total + idx];
+        }
       }
     }
   }
-  for (int f = 0; f < nf; ++f) out[f * n + p] = add ? acc[f] + add[f * n + p] : acc[f];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) out[f * n + p] = add ? acc[f] + add[f * n + p] : acc[f];
 }
 
 template <int P>
@@ -603,21 +610,21 @@ extern "C" int cdft_interpolate_f64(const double* field, int nf, const int* base
   return check_launch("interpolate");
 }
 
-// cell_box (host): lo0, lo1, lo2, c0, c1, c2. add may be NULL.
-// Workspace doubles the transpose needs for a cell box of extents (c0, c1, c2) and nf fields.
-extern "C" long cdft_lift_workspace_doubles(const int* cell_box, int nf) {
-  if (!cell_box) return 0;
-  return (long)nf * cell_box[3] * cell_box[4] * 64 * (cell_box[5] + 7);
-}
-
-// cell_box (host): lo0, lo1, lo2, c0, c1, c2; cell_start: c0*c1*c2 + 1 node offsets (nodes numbered
-// in base-cell order); work: cdft_lift_workspace_doubles doubles. add may be NULL.
-extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* cell_start, const int* base,
-                             const double* lw, const int* shape, double h, const double* origin,
-                             const int* cell_box, const double* add, double* work, long work_len,
-                             double* out, void* stream) {
-  if (!x || !cell_start || !base || !lw || !out || !cell_box || !work || nf < 1 || nf > 4)
-    return fail(CDFT_E_ARG, "lift: bad arguments (nf=%d)", nf);
+// cell_box (host): lo0, lo1, lo2, c0, c1, c2; the chunk tables (device, int32): chunk_node
+// (n_chunks + 1 node offsets), chunk_zlo and chunk_hz (first z-cell relative to lo2 and height of
+// each chunk's slab), chunk_off (n_chunks + 1 offsets of the slabs, in doubles per field, each slab
+// 64 * hz), col_chunk (c0 * c1 + 1 chunk offsets per column). work: nf * chunk_off[n_chunks]
+// doubles (work_per_field = chunk_off[n_chunks]). add may be NULL.
+extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* base, const double* lw,
+                             const int* shape, double h, const double* origin, const int* cell_box,
+                             int n_chunks, const int* chunk_node, const int* chunk_zlo, const int* chunk_hz,
+                             const int* chunk_off, const int* col_chunk, long work_per_field,
+                             const double* add, double* work, long work_len, double* out, void* stream) {
+  if (!x || !base || !lw || !out || !cell_box || !work || !chunk_node || !chunk_zlo || !chunk_hz || !chunk_off ||
+      !col_chunk || nf < 1 || nf > 4 || n_chunks < 1)
+    return fail(CDFT_E_ARG, "lift: bad arguments (nf=%d, chunks=%d)", nf, n_chunks);
+  if (work_len < work_per_field * nf)
+    return fail(CDFT_E_CAPACITY, "lift: workspace holds %ld doubles, needs %ld", work_len, work_per_field * nf);
   GridDesc g = make_grid(shape, h, origin);
   CellBox cb;
   cb.lo0 = cell_box[0];
@@ -626,18 +633,24 @@ extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* cell_st
   cb.c0 = cell_box[3];
   cb.c1 = cell_box[4];
   cb.c2 = cell_box[5];
-  const int ez = cb.c2 + 7;
-  if (work_len < cdft_lift_workspace_doubles(cell_box, nf))
-    return fail(CDFT_E_CAPACITY, "lift: workspace holds %ld doubles, needs %ld", work_len,
-                cdft_lift_workspace_doubles(cell_box, nf));
   const long n = (long)g.n0 * g.n1 * g.n2;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(lift_columns_kernel, dim3((unsigned)(cb.c0 * cb.c1)), dim3(64), 0, s, x, nf, m, cell_start,
-                     base, lw, cb, ez, work);
-  int rc = check_launch("lift_columns");
-  if (rc) return rc;
-  hipLaunchKernelGGL(lift_combine_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, work, nf, cb, ez, g, add, out);
-  return check_launch("lift_combine");
+#define CDFT_LIFT(NFV)                                                                                        \
+  hipLaunchKernelGGL(lift_chunks_kernel<NFV>, dim3((unsigned)n_chunks), dim3(64), 0, s, x, m, base, lw, cb.lo2, \
+                     chunk_node, chunk_zlo, chunk_hz, chunk_off, work_per_field, work);                        \
+  rc = check_launch("lift_chunks");                                                                          \
+  if (rc) return rc;                                                                                         \
+  hipLaunchKernelGGL(lift_combine_kernel<NFV>, dim3(blocks_for(n, 256)), dim3(256), 0, s, work, cb, g,         \
+                     chunk_zlo, chunk_hz, chunk_off, col_chunk, work_per_field, add, out);                   \
+  return check_launch("lift_combine")
+  int rc = CDFT_OK;
+  switch (nf) {
+    case 1: CDFT_LIFT(1);
+    case 2: CDFT_LIFT(2);
+    case 3: CDFT_LIFT(3);
+    default: CDFT_LIFT(4);
+  }
+#undef CDFT_LIFT
 }
 
 // coef: DEVICE array of the 2P+1 central first-derivative weights divided by h.

@@ -324,6 +324,7 @@ class CuspQuadrature:
         self.cell_start = as_dev(start, torch.int32)
         self.node_order = perm  # GPU node i is node perm[i] of the sphere-by-sphere enumeration
         self._cell_box = _ints([lo[0], lo[1], lo[2], ext[0], ext[1], ext[2]])
+        self._build_lift_chunks(base_host, start, lo, ext, as_dev)
         # Far part of the rule.
         self.far = grid.empty()
         _lib.call("cdft_far_factor_f64", grid.c_shape, h, grid.c_origin, n_atoms, ctypes.addressof(self._spheres_c),
@@ -343,6 +344,38 @@ class CuspQuadrature:
 
     # --- matrix-free interpolation and its transpose ---
 
+    #: Nodes per chunk of the transpose's column walk (one LDS staging of node records).
+    lift_chunk = 64
+
+    def _build_lift_chunks(self, base_host, cell_start, lo, ext, as_dev) -> None:
+        """Split every z-column of the cell box into chunks of at most ``lift_chunk`` consecutive
+        nodes, with each chunk's z-cell range and the offset of its (64 x height) slab."""
+        c2 = int(ext[2])
+        ncols = int(ext[0] * ext[1])
+        col_start = cell_start[::c2][: ncols + 1] if cell_start.size == ncols * c2 + 1 else cell_start[np.arange(ncols + 1) * c2]
+        node, zlo, hz, col_chunk = [], [], [], [0]
+        b2 = base_host[:, 2] - lo[2]
+        for col in range(ncols):
+            s, e = int(col_start[col]), int(col_start[col + 1])
+            for a in range(s, e, self.lift_chunk):
+                b = min(e, a + self.lift_chunk)
+                node.append(a)
+                zlo.append(int(b2[a]))
+                hz.append(int(b2[b - 1]) - int(b2[a]) + 8)
+            col_chunk.append(len(node))
+        node.append(self.n_nodes)
+        off = np.zeros(len(hz) + 1, dtype=np.int64)
+        np.cumsum(64 * np.asarray(hz, dtype=np.int64), out=off[1:])
+        if off[-1] >= 2**31:
+            raise ValueError("the transpose's workspace exceeds 32-bit offsets")
+        self._lift_chunks = len(hz)
+        self._chunk_node = as_dev(np.asarray(node, dtype=np.int32), torch.int32)
+        self._chunk_zlo = as_dev(np.asarray(zlo, dtype=np.int32), torch.int32)
+        self._chunk_hz = as_dev(np.asarray(hz, dtype=np.int32), torch.int32)
+        self._chunk_off = as_dev(off.astype(np.int32), torch.int32)
+        self._col_chunk = as_dev(np.asarray(col_chunk, dtype=np.int32), torch.int32)
+        self._lift_per_field = int(off[-1])
+
     def interpolate(self, field: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """Degree-7 interpolant at the nodes, ``(nf, m)`` for ``(nf, N)`` (``nf <= 4``)."""
         f = field.reshape(-1, self.grid.n_points)
@@ -358,12 +391,13 @@ class CuspQuadrature:
         nf = x.shape[0]
         out = torch.empty((nf, self.grid.n_points), dtype=torch.float64, device=self.grid.device) if out is None else out
         if getattr(self, "_lift_work", None) is None:
-            need = int(_lib.load().cdft_lift_workspace_doubles(ctypes.addressof(self._cell_box), 4))
-            self._lift_work = torch.empty(need, dtype=torch.float64, device=self.grid.device)
-        _lib.call("cdft_lift_f64", x.data_ptr(), nf, self.n_nodes, self.cell_start.data_ptr(),
-                  self.base.data_ptr(), self.lw.data_ptr(), self.grid.c_shape,
-                  self.grid.spacing, self.grid.c_origin, ctypes.addressof(self._cell_box), _ptr_or_none(add),
-                  self._lift_work.data_ptr(), self._lift_work.numel(), out.data_ptr(), stream_of(x))
+            self._lift_work = torch.empty(4 * self._lift_per_field, dtype=torch.float64, device=self.grid.device)
+        _lib.call("cdft_lift_f64", x.data_ptr(), nf, self.n_nodes, self.base.data_ptr(), self.lw.data_ptr(),
+                  self.grid.c_shape, self.grid.spacing, self.grid.c_origin, ctypes.addressof(self._cell_box),
+                  self._lift_chunks, self._chunk_node.data_ptr(), self._chunk_zlo.data_ptr(),
+                  self._chunk_hz.data_ptr(), self._chunk_off.data_ptr(), self._col_chunk.data_ptr(),
+                  self._lift_per_field, _ptr_or_none(add), self._lift_work.data_ptr(), self._lift_work.numel(),
+                  out.data_ptr(), stream_of(x))
         return out.reshape(-1) if node_values.dim() == 1 else out
 
     def interpolate_at(self, points: np.ndarray, field: torch.Tensor) -> torch.Tensor:

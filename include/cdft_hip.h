@@ -195,15 +195,19 @@ int cdft_interpolate_f64(const double* field, int nf, const int* base, const dou
                          const int* shape, double h, const double* origin, double* out,
                          void* stream);
 
-/* Doubles of workspace cdft_lift_f64 needs for a cell box (host: lo0, lo1, lo2, c0, c1, c2). */
-long cdft_lift_workspace_doubles(const int* cell_box, int nf);
 /* The transpose of interpolate: out[f] = add[f] + L^T x[f] (add may be NULL), deterministic and
- * atomic-free. Nodes must be numbered in base-cell order with cell_start (c0*c1*c2 + 1 offsets)
- * over the cell box. Replaces CuspQuadrature.lift and the precomputed L^T CSR product
+ * atomic-free. Nodes must be numbered in base-cell order over the cell box (host: lo0, lo1, lo2,
+ * c0, c1, c2). The walk is split into chunks of at most 64 consecutive nodes of one z-column; the
+ * chunk tables (device, int32) are chunk_node (n_chunks + 1 node offsets), chunk_zlo / chunk_hz
+ * (each chunk's first z-cell relative to lo2 and its slab height), chunk_off (n_chunks + 1 slab
+ * offsets in doubles per field; a slab is 64 x height) and col_chunk (c0 c1 + 1 chunk offsets per
+ * column); work holds nf * work_per_field doubles, work_per_field = chunk_off[n_chunks]. Replaces
+ * CuspQuadrature.lift and the precomputed L^T CSR product
  * (src/cdft/operators/quadrature.py:582-609,628-638). */
-int cdft_lift_f64(const double* x, int nf, long m, const int* cell_start, const int* base,
-                  const double* lw, const int* shape, double h, const double* origin,
-                  const int* cell_box, const double* add, double* work, long work_len, double* out,
+int cdft_lift_f64(const double* x, int nf, long m, const int* base, const double* lw, const int* shape,
+                  double h, const double* origin, const int* cell_box, int n_chunks, const int* chunk_node,
+                  const int* chunk_zlo, const int* chunk_hz, const int* chunk_off, const int* col_chunk,
+                  long work_per_field, const double* add, double* work, long work_len, double* out,
                   void* stream);
 
 /* Lumped far mass weights of overlapping spheres (src/cdft/operators/quadrature.py:421-497), three
