@@ -358,39 +358,62 @@ __global__ __launch_bounds__(64) void lift_chunks_kernel(const double* __restric
   }
 }
 
-// Stage 2: out[f][p] = add[f][p] + sum over (j0, j1) (j0 outer, j1 inner) and over the chunks of
-// column (i0 - j0 + 3, i1 - j1 + 3) in node order of the chunk's Z[8 j0 + j1][iz - zlo], iz =
-// i2 - lo2 + 3; add (or zero) where no node reaches.
+// Stage 2: each column's chunk slabs summed (chunks in node order) into one column slab
+// Zcol[f][col][t][iz], iz in [0, c2 + 7): one block per column.
 template <int NF>
-__global__ void lift_combine_kernel(const double* __restrict__ z, CellBox cb, GridDesc g,
-                                    const int* __restrict__ chunk_zlo, const int* __restrict__ chunk_hz,
-                                    const int* __restrict__ chunk_off, const int* __restrict__ col_chunk,
-                                    long total, const double* __restrict__ add, double* __restrict__ out) {
+__global__ void lift_columns_kernel(const double* __restrict__ z, CellBox cb, const int* __restrict__ chunk_zlo,
+                                    const int* __restrict__ chunk_hz, const int* __restrict__ chunk_off,
+                                    const int* __restrict__ col_chunk, long total, double* __restrict__ zcol) {
+  const int col = blockIdx.x;
+  const int ez = cb.c2 + 7;
+  const long ncols = (long)cb.c0 * cb.c1;
+  const int c0 = col_chunk[col], c1 = col_chunk[col + 1];
+  for (int e = threadIdx.x; e < 64 * ez; e += blockDim.x) {
+    const int t = e / ez, iz = e - (e / ez) * ez;
+    double acc[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) acc[f] = 0.0;
+    for (int c = c0; c < c1; ++c) {
+      const int zlo = chunk_zlo[c];
+      if (zlo > iz) break;  // chunks of a column start at increasing z-cells
+      const int r = iz - zlo, hz = chunk_hz[c];
+      if (r >= hz) continue;
+      const long idx = (long)chunk_off[c] + (long)t * hz + r;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) acc[f] += z[f * total + idx];
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) zcol[((f * ncols + col) * 64 + t) * ez + iz] = acc[f];
+  }
+}
+
+// Stage 3: out[f][p] = add[f][p] + sum_{j0, j1} Zcol[f][col(i0 - j0 + 3, i1 - j1 + 3)][8 j0 + j1][iz]
+// (j0 outer, j1 inner), iz = i2 - lo2 + 3; add (or zero) where no node reaches.
+template <int NF>
+__global__ void lift_combine_kernel(const double* __restrict__ zcol, CellBox cb, GridDesc g,
+                                    const double* __restrict__ add, double* __restrict__ out) {
   const long n = (long)g.n0 * g.n1 * g.n2;
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int i2 = (int)(p % g.n2);
   const int i1 = (int)((p / g.n2) % g.n1);
   const int i0 = (int)(p / ((long)g.n1 * g.n2));
+  const int ez = cb.c2 + 7;
   const int iz = i2 - (cb.lo2 - 3);
+  const long ncols = (long)cb.c0 * cb.c1;
   double acc[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) acc[f] = 0.0;
-  if (iz >= 0 && iz < cb.c2 + 7) {
+  if (iz >= 0 && iz < ez) {
     for (int j0 = 0; j0 < 8; ++j0) {
       const int b0 = i0 - j0 + 3 - cb.lo0;
       if (b0 < 0 || b0 >= cb.c0) continue;
       for (int j1 = 0; j1 < 8; ++j1) {
         const int b1 = i1 - j1 + 3 - cb.lo1;
         if (b1 < 0 || b1 >= cb.c1) continue;
-        const int col = b0 * cb.c1 + b1;
-        for (int c = col_chunk[col]; c < col_chunk[col + 1]; ++c) {
-          const int r = iz - chunk_zlo[c];
-          if (r < 0 || r >= chunk_hz[c]) continue;
-          const long idx = (long)chunk_off[c] + (long)(j0 * 8 + j1) * chunk_hz[c] + r;
+        const long idx = (((long)b0 * cb.c1 + b1) * 64 + j0 * 8 + j1) * ez + iz;
 #pragma unroll
-          for (int f = 0; f < NF; ++f) acc[f] += z[f * total + idx];
-        }
+        for (int f = 0; f < NF; ++f) acc[f] += zcol[f * ncols * 64 * ez + idx];
       }
     }
   }
@@ -613,8 +636,8 @@ extern "C" int cdft_interpolate_f64(const double* field, int nf, const int* base
 // cell_box (host): lo0, lo1, lo2, c0, c1, c2; the chunk tables (device, int32): chunk_node
 // (n_chunks + 1 node offsets), chunk_zlo and chunk_hz (first z-cell relative to lo2 and height of
 // each chunk's slab), chunk_off (n_chunks + 1 offsets of the slabs, in doubles per field, each slab
-// 64 * hz), col_chunk (c0 * c1 + 1 chunk offsets per column). work: nf * chunk_off[n_chunks]
-// doubles (work_per_field = chunk_off[n_chunks]). add may be NULL.
+// 64 * hz), col_chunk (c0 * c1 + 1 chunk offsets per column). work: nf * (work_per_field +
+// c0 c1 64 (c2 + 7)) doubles, work_per_field = chunk_off[n_chunks]. add may be NULL.
 extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* base, const double* lw,
                              const int* shape, double h, const double* origin, const int* cell_box,
                              int n_chunks, const int* chunk_node, const int* chunk_zlo, const int* chunk_hz,
@@ -623,8 +646,10 @@ extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* base, c
   if (!x || !base || !lw || !out || !cell_box || !work || !chunk_node || !chunk_zlo || !chunk_hz || !chunk_off ||
       !col_chunk || nf < 1 || nf > 4 || n_chunks < 1)
     return fail(CDFT_E_ARG, "lift: bad arguments (nf=%d, chunks=%d)", nf, n_chunks);
-  if (work_len < work_per_field * nf)
-    return fail(CDFT_E_CAPACITY, "lift: workspace holds %ld doubles, needs %ld", work_len, work_per_field * nf);
+  const long col_doubles = (long)cell_box[3] * cell_box[4] * 64 * (cell_box[5] + 7);
+  if (work_len < (work_per_field + col_doubles) * nf)
+    return fail(CDFT_E_CAPACITY, "lift: workspace holds %ld doubles, needs %ld", work_len,
+                (work_per_field + col_doubles) * nf);
   GridDesc g = make_grid(shape, h, origin);
   CellBox cb;
   cb.lo0 = cell_box[0];
@@ -634,14 +659,18 @@ extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* base, c
   cb.c1 = cell_box[4];
   cb.c2 = cell_box[5];
   const long n = (long)g.n0 * g.n1 * g.n2;
+  double* zcol = work + work_per_field * nf;
   hipStream_t s = (hipStream_t)stream;
 #define CDFT_LIFT(NFV)                                                                                        \
   hipLaunchKernelGGL(lift_chunks_kernel<NFV>, dim3((unsigned)n_chunks), dim3(64), 0, s, x, m, base, lw, cb.lo2, \
                      chunk_node, chunk_zlo, chunk_hz, chunk_off, work_per_field, work);                        \
   rc = check_launch("lift_chunks");                                                                          \
   if (rc) return rc;                                                                                         \
-  hipLaunchKernelGGL(lift_combine_kernel<NFV>, dim3(blocks_for(n, 256)), dim3(256), 0, s, work, cb, g,         \
-                     chunk_zlo, chunk_hz, chunk_off, col_chunk, work_per_field, add, out);                   \
+  hipLaunchKernelGGL(lift_columns_kernel<NFV>, dim3((unsigned)(cb.c0 * cb.c1)), dim3(256), 0, s, work, cb,     \
+                     chunk_zlo, chunk_hz, chunk_off, col_chunk, work_per_field, zcol);                        \
+  rc = check_launch("lift_columns");                                                                         \
+  if (rc) return rc;                                                                                         \
+  hipLaunchKernelGGL(lift_combine_kernel<NFV>, dim3(blocks_for(n, 256)), dim3(256), 0, s, zcol, cb, g, add, out); \
   return check_launch("lift_combine")
   int rc = CDFT_OK;
   switch (nf) {

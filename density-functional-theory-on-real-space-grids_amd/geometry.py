@@ -391,7 +391,10 @@ class CuspQuadrature:
         nf = x.shape[0]
         out = torch.empty((nf, self.grid.n_points), dtype=torch.float64, device=self.grid.device) if out is None else out
         if getattr(self, "_lift_work", None) is None:
-            self._lift_work = torch.empty(4 * self._lift_per_field, dtype=torch.float64, device=self.grid.device)
+            box = self._cell_box
+            columns = int(box[3]) * int(box[4]) * 64 * (int(box[5]) + 7)
+            self._lift_work = torch.empty(4 * (self._lift_per_field + columns), dtype=torch.float64,
+                                          device=self.grid.device)
         _lib.call("cdft_lift_f64", x.data_ptr(), nf, self.n_nodes, self.base.data_ptr(), self.lw.data_ptr(),
                   self.grid.c_shape, self.grid.spacing, self.grid.c_origin, ctypes.addressof(self._cell_box),
                   self._lift_chunks, self._chunk_node.data_ptr(), self._chunk_zlo.data_ptr(),

@@ -201,7 +201,8 @@ int cdft_interpolate_f64(const double* field, int nf, const int* base, const dou
  * chunk tables (device, int32) are chunk_node (n_chunks + 1 node offsets), chunk_zlo / chunk_hz
  * (each chunk's first z-cell relative to lo2 and its slab height), chunk_off (n_chunks + 1 slab
  * offsets in doubles per field; a slab is 64 x height) and col_chunk (c0 c1 + 1 chunk offsets per
- * column); work holds nf * work_per_field doubles, work_per_field = chunk_off[n_chunks]. Replaces
+ * column); work holds nf * (work_per_field + c0 c1 64 (c2 + 7)) doubles, work_per_field =
+ * chunk_off[n_chunks]. Replaces
  * CuspQuadrature.lift and the precomputed L^T CSR product
  * (src/cdft/operators/quadrature.py:582-609,628-638). */
 int cdft_lift_f64(const double* x, int nf, long m, const int* base, const double* lw, const int* shape,
