@@ -54,6 +54,7 @@ SIGNATURES: dict[str, list] = {
     "cdft_far_fine_f64": [I, P, P, P, P, P, P, D, P, P],
     "cdft_interp_transpose_axis_f64": [P, P, I, I, P, P, P, P, P],
     "cdft_far_blend_f64": [P, D, P, D, D, P, P, P, P],
+    "cdft_mt19937_states": [ctypes.c_uint64, I, P, P, P],
     "cdft_hydrogenic_fields_f64": [P, L, P, D, P, I, P, P, P, P, P],
     "cdft_lda_vwn_f64": [P, L, P, P, P],
     "cdft_ewise_f64": [I, P, P, P, D, P, L, P],

@@ -403,20 +403,30 @@ class CuspQuadrature:
                   out.data_ptr(), stream_of(x))
         return out.reshape(-1) if node_values.dim() == 1 else out
 
-    def interpolate_at(self, points: np.ndarray, field: torch.Tensor) -> torch.Tensor:
-        """The same degree-7 stencil at arbitrary points (the nuclei), ``(p,)``."""
+    def stencils_at(self, points: np.ndarray) -> tuple[torch.Tensor, torch.Tensor]:
+        """The degree-7 stencils (base cells, Lagrange rows) of arbitrary points, e.g. the nuclei."""
         pts = torch.as_tensor(np.ascontiguousarray(points, dtype=np.float64)).to(self.grid.device)
         p = pts.shape[0]
         base = torch.empty((p, 3), dtype=torch.int32, device=self.grid.device)
         lw = torch.empty((p, 24), dtype=torch.float64, device=self.grid.device)
         bad = torch.zeros(1, dtype=torch.int32, device=self.grid.device)
-        s = stream_of(pts)
         _lib.call("cdft_node_stencils_f64", pts.data_ptr(), p, self.grid.c_shape, self.grid.spacing, self.grid.c_origin,
-                  base.data_ptr(), lw.data_ptr(), bad.data_ptr(), s)
-        out = torch.empty(p, dtype=torch.float64, device=self.grid.device)
+                  base.data_ptr(), lw.data_ptr(), bad.data_ptr(), stream_of(pts))
+        return base, lw
+
+    def interpolate_with(self, stencils: tuple[torch.Tensor, torch.Tensor], field: torch.Tensor,
+                         out: torch.Tensor | None = None) -> torch.Tensor:
+        """``field`` at the points whose stencils are given (from :meth:`stencils_at`), ``(p,)``."""
+        base, lw = stencils
+        p = base.shape[0]
+        out = torch.empty(p, dtype=torch.float64, device=self.grid.device) if out is None else out
         _lib.call("cdft_interpolate_f64", field.data_ptr(), 1, base.data_ptr(), lw.data_ptr(), p, self.grid.c_shape,
-                  self.grid.spacing, self.grid.c_origin, out.data_ptr(), s)
+                  self.grid.spacing, self.grid.c_origin, out.data_ptr(), stream_of(field))
         return out
+
+    def interpolate_at(self, points: np.ndarray, field: torch.Tensor) -> torch.Tensor:
+        """The same degree-7 stencil at arbitrary points (the nuclei), ``(p,)``."""
+        return self.interpolate_with(self.stencils_at(points), field)
 
     def bare_operator_terms(self):
         """``(W~, h^3/omega, q)`` of the bare transformed operator, built once (reference

@@ -46,7 +46,7 @@ class SeededDraws:
         hit = self.store.get(key)
         if isinstance(hit, tuple):  # prefetched on a host thread: (future, position)
             future, position = hit
-            hit = future.result()[position].to(device)
+            hit = future.result()[position].to(device, non_blocking=True)
             self.store[key] = hit
         if hit is None:
             if self._gen is None:
@@ -62,24 +62,126 @@ class SeededDraws:
 
 
 _draw_pool = None
+_chunk_pool = None
+_DRAW_THREADS = 8
+_CHUNK_VALUES = 1 << 16  # values per chunk; a multiple of 16
+_state_layout_ok = None
 
 
-def prefetch_draws(store: dict, seed: int, sequences) -> None:
-    """Start drawing each sequence of sizes (from a fresh generator seeded ``seed``) on a host
-    thread, and register every prefix in ``store`` for :class:`SeededDraws` to pick up."""
-    global _draw_pool
+def _pools():
+    global _draw_pool, _chunk_pool
     if _draw_pool is None:
         import concurrent.futures
 
         _draw_pool = concurrent.futures.ThreadPoolExecutor(max_workers=2, thread_name_prefix="seeded-draws")
+        _chunk_pool = concurrent.futures.ThreadPoolExecutor(max_workers=_DRAW_THREADS, thread_name_prefix="draw-chunks")
+    return _draw_pool, _chunk_pool
 
-    def job(seq):
-        gen = torch.Generator(device="cpu").manual_seed(int(seed))
-        return [torch.randn(size, generator=gen, dtype=torch.float64) for size in seq]
 
+def _generator_at(seed: int, key: np.ndarray, index: int) -> torch.Generator:
+    """A CPU generator whose Mersenne-Twister stream sits at the given state (the 624-word array
+    and the index of its next output), in the layout ``torch.Generator.get_state`` uses: initial
+    seed, then the remaining-count and seeded flag, the next index, and the words as 64-bit slots."""
+    state = torch.Generator().manual_seed(int(seed)).get_state()
+    q = state.numpy()[: 8 * (3 + 624)].view(np.uint64)
+    q[1] = (1 << 32) | (625 - int(index))
+    q[2] = int(index)
+    q[3:] = key
+    gen = torch.Generator()
+    gen.set_state(state)
+    return gen
+
+
+def _states(seed: int, positions: list[int]) -> tuple[np.ndarray, np.ndarray]:
+    import ctypes
+
+    from . import _lib
+
+    n = len(positions)
+    pos = (ctypes.c_int64 * n)(*positions)
+    key = np.empty((n, 624), dtype=np.uint32)
+    index = np.empty(n, dtype=np.int32)
+    _lib.call("cdft_mt19937_states", int(seed) & 0xFFFFFFFFFFFFFFFF, n, ctypes.addressof(pos), key.ctypes.data,
+              index.ctypes.data)
+    return key, index
+
+
+def _layout_verified() -> bool:
+    """Whether a generator placed by :func:`_generator_at` continues exactly as torch's own does
+    (checked once: the state mid-stream, across a twist, and at a fresh seed)."""
+    global _state_layout_ok
+    if _state_layout_ok is None:
+        ok = True
+        for seed, skip in ((7, 0), (7, 5), (2**32 + 11, 311), (123, 312)):
+            ref = torch.Generator().manual_seed(seed)
+            if skip:
+                torch.rand(skip, generator=ref, dtype=torch.float64)
+            key, index = _states(seed, [2 * skip])
+            mine = _generator_at(seed, key[0], int(index[0]))
+            ok &= torch.equal(torch.randn(40, generator=ref, dtype=torch.float64),
+                              torch.randn(40, generator=mine, dtype=torch.float64))
+        _state_layout_ok = bool(ok)
+    return _state_layout_ok
+
+
+def parallel_randn(seed: int, sizes) -> list[torch.Tensor]:
+    """``[torch.randn(s, generator=g, dtype=float64) for s in sizes]`` with ``g`` freshly seeded
+    ``seed``, bitwise, drawn in chunks on host threads into pinned memory.
+
+    torch fills ``n >= 16`` float64 normals by drawing ``n`` uniforms (two 32-bit outputs each),
+    transforming every full group of 16 on its own, and, when 16 does not divide ``n``, drawing 16
+    more uniforms for the last 16 values. A chunk of whole groups drawn from the generator state at
+    its first uniform is therefore bitwise that part of the serial draw; the states come from
+    ``cdft_mt19937_states``. Sizes below 16 (a different torch path) or a generator layout this
+    code does not recognise fall back to the serial draw."""
+    sizes = [tuple(s) if not isinstance(s, int) else (s,) for s in sizes]
+    counts = [int(np.prod(s)) for s in sizes]
+    pinned = torch.cuda.is_available()
+    if min(counts, default=16) < 16 or not _layout_verified():
+        gen = torch.Generator().manual_seed(int(seed))
+        return [torch.randn(s, generator=gen, dtype=torch.float64) for s in sizes]
+    _, chunk_pool = _pools()
+    plan = []  # (draw, lo, hi, uniform offset of lo) per chunk, then the tails
+    tails = []
+    uniform = 0
+    for d, n in enumerate(counts):
+        full = n - n % 16
+        for lo in range(0, full, _CHUNK_VALUES):
+            hi = min(full, lo + _CHUNK_VALUES)
+            plan.append((d, lo, hi, uniform + lo))
+        if n % 16:
+            tails.append((d, uniform + n))
+        uniform += n + (16 if n % 16 else 0)
+    positions = sorted({2 * u for *_, u in plan} | {2 * u for _, u in tails})
+    key, index = _states(seed, positions)
+    where = {p: i for i, p in enumerate(positions)}
+    outs = [torch.empty(n, dtype=torch.float64, pin_memory=pinned) for n in counts]
+
+    def chunk(d, lo, hi, u):
+        i = where[2 * u]
+        torch.randn(hi - lo, generator=_generator_at(seed, key[i], int(index[i])), dtype=torch.float64,
+                    out=outs[d][lo:hi])
+
+    futures = [chunk_pool.submit(chunk, *job) for job in plan]
+    tail_values = []
+    for d, u in tails:
+        i = where[2 * u]
+        tail_values.append((d, torch.randn(16, generator=_generator_at(seed, key[i], int(index[i])),
+                                           dtype=torch.float64)))
+    for f in futures:
+        f.result()
+    for d, values in tail_values:
+        outs[d][counts[d] - 16:] = values
+    return [o.view(s) for o, s in zip(outs, sizes)]
+
+
+def prefetch_draws(store: dict, seed: int, sequences) -> None:
+    """Start drawing each sequence of sizes (from a fresh generator seeded ``seed``) on host
+    threads, and register every prefix in ``store`` for :class:`SeededDraws` to pick up."""
+    draw_pool, _ = _pools()
     for seq in sequences:
         seq = tuple(tuple(s) for s in seq)
-        future = _draw_pool.submit(job, seq)
+        future = draw_pool.submit(parallel_randn, int(seed), seq)
         for i in range(len(seq)):
             store.setdefault((int(seed), seq[: i + 1]), (future, i))
 

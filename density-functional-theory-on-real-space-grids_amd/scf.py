@@ -112,18 +112,60 @@ class KohnShamStep:
         self.tmp_grid = torch.empty(g.n_points, **f)
         self.coefficients = torch.empty(len(self.charges), **f)
         self.rho_nuclei = torch.empty(len(self.charges), **f)
+        self.nucleus_stencils = self.q.stencils_at(self.positions)
+        # static inputs and outputs of the captured potentials / energy graphs
+        self.pot_in = torch.empty(g.n_points, **f)
+        self.pot_lumped = torch.empty((2, g.n_points), **f)
+        self.pot_effective = torch.empty(g.n_points, **f)
+        self.pot_scalars = torch.empty(3, **f)
+        self.graphs: dict = {}
+        self.graph_record = {"captures": 0, "replays": 0, "disabled": None}
+        self.use_graphs = True
+
+    def _replay(self, key, body) -> None:
+        """Run ``body`` eagerly on its first call (which allocates any lazy workspaces), capture it as
+        a HIP graph on its second, and replay the graph after that. A capture failure is recorded
+        and the body then runs eagerly."""
+        if not self.use_graphs or self.graph_record["disabled"] is not None:
+            body()
+            return
+        entry = self.graphs.get(key)
+        if entry is None:
+            self.graphs[key] = "eager"
+            body()
+            return
+        if entry == "eager":
+            try:
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph):
+                    body()
+                self.graphs[key] = graph
+                self.graph_record["captures"] += 1
+            except Exception as exc:  # noqa: BLE001 - recorded; the body then runs eagerly
+                self.graph_record["disabled"] = f"capture failed: {type(exc).__name__}: {exc}"
+                body()
+                return
+            entry = graph
+        entry.replay()
+        self.graph_record["replays"] += 1
 
     # ---- potentials ----
 
     def potentials(self, density: torch.Tensor) -> dict:
-        """Hartree (cusp split + FFT) and LDA lumped weights (reference ``scf/step.py:224-313``).
+        """Hartree (cusp split + FFT) and LDA lumped weights (reference ``scf/step.py:224-313``), one
+        captured graph over static buffers after the first call.
 
         Returns device tensors; the three energies sit in ``scalars = [E_H, E_xc, int n v_xc]``.
         """
+        self.pot_in.copy_(density.reshape(-1, self.grid.n_points)[0])
+        self._replay("potentials", self._potentials_body)
+        return {"effective": self.pot_effective.clone(), "scalars": self.pot_scalars.clone()}
+
+    def _potentials_body(self) -> None:
         q, g = self.q, self.grid
-        n_grid = density.reshape(-1, g.n_points)[0]
+        n_grid = self.pot_in
         ops.ewise(ops.EW_DIV, self.rho, n_grid, self.f2)
-        self.rho_nuclei.copy_(q.interpolate_at(self.positions, self.rho))
+        q.interpolate_with(self.nucleus_stencils, self.rho, out=self.rho_nuclei)
         ops.small_solve(self.overlap, self.nucleus_f2, self.rho_nuclei, self.coefficients)
         ops.atom_combine(n_grid, self.coefficients, self.cusp_grid, -1.0, self.n_smooth)
         self.poisson.solve(self.n_smooth, self.v_smooth)
@@ -144,16 +186,14 @@ class KohnShamStep:
         ops.ewise(ops.EW_MUL, self.grid_add[0], q.far_weights, self.tmp_grid)
         ops.ewise(ops.EW_MUL, self.tmp_grid, self.f2, self.v_grid)
         ops.ewise(ops.EW_MUL, self.grid_add[1], q.far_weights, self.tmp_grid)
-        lumped = q.lift(self.node_x, add=self.grid_add)
+        lumped = q.lift(self.node_x, add=self.grid_add, out=self.pot_lumped)
         hartree_weights, xc_weights = lumped[0], lumped[1]
-        scalars = torch.empty(3, dtype=torch.float64, device=g.device)
+        scalars = self.pot_scalars
         ops.reduce(ops.RED_DOT, hartree_weights, scalars, 0, b=self.rho, scale=0.5)
         ops.reduce(ops.RED_DOT, q.weights, scalars, 1, b=self.e_node)
         ops.reduce(ops.RED_DOT, q.far_weights, scalars, 1, b=self.e_grid, accumulate=True)
         ops.reduce(ops.RED_DOT, xc_weights, scalars, 2, b=self.rho)
-        effective = ops.ewise(ops.EW_SUMDIV, g.empty(), hartree_weights, xc_weights, q.mass_weights)
-        return {"effective": effective, "scalars": scalars, "v_hartree": self.v_h_grid.clone(),
-                "v_xc": self.v_grid.clone(), "coefficients": self.coefficients.clone()}
+        ops.ewise(ops.EW_SUMDIV, self.pot_effective, hartree_weights, xc_weights, q.mass_weights)
 
     def hamiltonian(self, pot: dict) -> CuspFactoredHamiltonian:
         return CuspFactoredHamiltonian(self.grid, self.factor, self.q, extra_potential=pot["effective"])
@@ -181,11 +221,28 @@ class KohnShamStep:
 
     def energy_scalars(self, phi: torch.Tensor, occ, rho_out: torch.Tensor, effective_in: torch.Tensor) -> torch.Tensor:
         """Device buffer ``[kinetic form (k x k, flattened) | cross | potential | dc_out]`` (reference
-        ``operators/hamiltonian.py:344-377``, ``scf/step.py:398-439``)."""
+        ``operators/hamiltonian.py:344-377``, ``scf/step.py:398-439``), one captured graph over
+        static buffers per block size after the first call."""
+        k = phi.shape[0]
+        key = ("energy", k, effective_in is None)
+        st = self.__dict__.setdefault("_energy_static", {})
+        if key not in st:
+            f = dict(dtype=torch.float64, device=self.grid.device)
+            st[key] = (torch.empty((k, self.grid.n_points), **f), torch.empty(self.grid.n_points, **f),
+                       None if effective_in is None else torch.empty(self.grid.n_points, **f),
+                       torch.empty(k * k + 3, **f))
+        phi_s, rho_s, eff_s, out = st[key]
+        phi_s.copy_(phi)
+        rho_s.copy_(rho_out)
+        if eff_s is not None:
+            eff_s.copy_(effective_in)
+        self._replay(key, lambda: self._energy_body(phi_s, rho_s, eff_s, out))
+        return out.clone()
+
+    def _energy_body(self, phi, rho_out, effective_in, out) -> None:
         g, q = self.grid, self.q
         k = phi.shape[0]
         kin = self.bare.kinetic_apply(phi)
-        out = torch.empty(k * k + 3, dtype=torch.float64, device=g.device)
         self.bare.measure.cross(phi, kin, out=out[: k * k].view(k, k))
         grad = g.gradient(rho_out)[0]  # (3, N)
         stacked = torch.empty((4, g.n_points), dtype=torch.float64, device=g.device)
@@ -204,7 +261,6 @@ class KohnShamStep:
         ops.reduce(ops.RED_DOT, q.far_weights, out, k * k + 1, b=pot_grid, accumulate=True)
         if effective_in is not None:
             ops.reduce(ops.RED_DOT, effective_in, out, k * k + 2, b=q.mass_weights, w=rho_out)
-        return out
 
 
 def energies_from_host(k, occ, evals, host_scalars, e_in, e_out, ion_ion):
@@ -363,14 +419,17 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
     started = time.perf_counter()
     store: dict = {}
     if geometry is None:
-        # The start vectors a solve draws (the cold block then its Lanczos start; a warm solve's
-        # Lanczos start) are known before the geometry exists: draw them on a host thread meanwhile.
+        # The start vectors a solve draws are known before the geometry exists: the cold block then
+        # its Lanczos start, and in a warm solve the vectors that pad the previous eigenvectors to
+        # the block then (when the bounds are not reused) the Lanczos start. Draw them on host
+        # threads meanwhile.
         counts = [int(round(length / numerics.spacing)) + 1 for length in numerics.box_lengths]
         npts = counts[0] * counts[1] * counts[2]
         n_states = max(1, int(-(-float(scenario.n_electrons) // 2.0)))
         cfg = numerics.eigen
         k = min(n_states + max(cfg.n_extra_states, int(cfg.extra_states_fraction * n_states)), npts)
-        prefetch_draws(store, numerics.seed, [((k, npts), (npts,)), ((npts,),)])
+        warm = ((k - n_states, npts), (npts,)) if k > n_states else ((npts,),)
+        prefetch_draws(store, numerics.seed, [((k, npts), (npts,)), warm])
     grid, factor, q = geometry if geometry is not None else build_geometry(scenario, numerics, device, anchor)
     step = KohnShamStep(scenario, numerics, grid, factor, q)
     step.draws = store
@@ -498,6 +557,7 @@ def solve_noninteracting(scenario, numerics, device, anchor=None, geometry=None)
     occ = aufbau(result.eigenvalues, scenario.n_electrons)
     step = KohnShamStep.__new__(KohnShamStep)
     step.grid, step.factor, step.q, step.bare = grid, factor, q, h
+    step.use_graphs, step.graph_record = False, {"captures": 0, "replays": 0, "disabled": "one evaluation"}
     rho = grid.empty()
     ops.density(result.eigenvectors, occ, None, rho, None)
     escal = KohnShamStep.energy_scalars(step, result.eigenvectors, occ, rho, None).cpu().numpy()

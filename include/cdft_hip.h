@@ -14,6 +14,8 @@
 #ifndef CDFT_HIP_H
 #define CDFT_HIP_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -227,6 +229,13 @@ int cdft_interp_transpose_axis_f64(const double* in, const int* dims, int axis, 
 /* (3) out = plain + interior (lumped - plain), interior = erfc((margin - d_face) / width) / 2. */
 int cdft_far_blend_f64(const int* shape, double h, const double* origin, double margin, double width,
                        const double* plain, const double* lumped, double* out, void* stream);
+
+/* Host only: the MT19937 state (key[624 i ..], index[i] = next word, 624 = twist due) of the
+ * stream seeded `seed` (init_genrand, low 32 bits) after positions[i] 32-bit outputs (ascending).
+ * torch's CPU generator runs this stream; from these states a float64 normal draw is split into
+ * chunks of 16k values drawn on separate host threads, bitwise equal to the serial draw. Serves the
+ * CPU draw of seeded_randn (src/cdft/precision.py:246-263). */
+int cdft_mt19937_states(uint64_t seed, int n, const int64_t* positions, uint32_t* key, int32_t* index);
 
 /* Central order-`order` first derivatives of nf fields (coef: DEVICE array of order+1 weights
  * divided by h), out (nf, 3, N), zero halo. Replaces gradient_from_padded

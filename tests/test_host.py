@@ -26,6 +26,37 @@ def test_seeded_draws_reproduce_torch_with_memo_and_prefetch(pkg):
             assert all(torch.equal(a, b) for a, b in zip(got, expect))
 
 
+def test_chunked_draws_are_the_serial_draws(pkg, monkeypatch):
+    """Draws split into chunks of whole 16-value groups, each drawn on a host thread from the
+    Mersenne-Twister state at its first uniform, equal the serial torch draw bit for bit: sizes
+    with and without a partial last group, chunks that straddle twists, seeds above 2^32."""
+    from cdft_amd import hamiltonian as hm
+
+    assert hm._layout_verified()  # otherwise parallel_randn would fall back to the serial draw
+    monkeypatch.setattr(hm, "_CHUNK_VALUES", 48)
+    for seed in (0, 5, 2**32 + 17):
+        sizes = [(3, 1001), (624,), (16,), (17,), (5, 320), (311,)]
+        got = hm.parallel_randn(seed, sizes)
+        expect = _torch_sequence(seed, sizes)
+        assert all(torch.equal(a, b) for a, b in zip(got, expect))
+    with_small = hm.parallel_randn(3, [(40,), (7,)])  # a size below 16 takes torch's scalar path
+    assert all(torch.equal(a, b) for a, b in zip(with_small, _torch_sequence(3, [(40,), (7,)])))
+
+
+def test_mersenne_states_match_the_stream(pkg):
+    """The states the library reports continue the stream exactly: numbers drawn from a generator
+    placed at position p equal the numbers after p outputs of the seeded generator."""
+    from cdft_amd import hamiltonian as hm
+
+    for seed, skip in ((1, 0), (1, 312), (1, 313), (99, 1000), (2**33 + 1, 4)):
+        ref = torch.Generator().manual_seed(seed)
+        torch.rand(skip, generator=ref, dtype=torch.float64)
+        key, index = hm._states(seed, [2 * skip])
+        placed = hm._generator_at(seed, key[0], int(index[0]))
+        assert torch.equal(torch.rand(700, generator=placed, dtype=torch.float64),
+                           torch.rand(700, generator=ref, dtype=torch.float64))
+
+
 def test_staggered_weights_closed_forms(pkg):
     from cdft_amd.stencil import staggered_coefficients
 

@@ -218,3 +218,36 @@ def test_h2_lda_scf_matches_oracle(pkg, device):
     o = ophys.solve_self_consistent(sc, num, anchor=anchor)
     assert r["scf_iterations"] == o["scf_iterations"]
     assert abs(r["energies"]["total"] - o["energies"]["total"]) < 1e-10
+
+
+def test_captured_potentials_and_energies_match_eager(pkg, device):
+    """The potentials and energy evaluations replayed from their captured graphs give bitwise the
+    eager results, on every replay, with eager work between replays and fresh inputs each time."""
+    from cdft_amd import scf
+
+    sc = ophys.Scenario((1.0, 1.0), ((-0.7, 0.05, 0.0), (0.7, 0.05, 0.0)), 2.0, "lda")
+    num = ophys.Numerics(box_lengths=(9.8, 8.4, 8.4), spacing=0.35)
+    grid, factor, quad = scf.build_geometry(sc, num, device)
+    captured = scf.KohnShamStep(sc, num, grid, factor, quad)
+    eager = scf.KohnShamStep(sc, num, grid, factor, quad)
+    eager.use_graphs = False
+    n0 = scf.initial_density(captured)
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    k = 3
+    for call in range(5):
+        bump = 1.0 + 0.05 * torch.rand(n0.shape, generator=gen, dtype=torch.float64).to(device)
+        density = n0 * bump
+        phi = torch.randn((k, grid.n_points), generator=gen, dtype=torch.float64).to(device)
+        a = captured.potentials(density)
+        b = eager.potentials(density)
+        torch.cuda.synchronize()
+        assert torch.equal(a["effective"], b["effective"]), call
+        assert torch.equal(a["scalars"], b["scalars"]), call
+        ea = captured.energy_scalars(phi, None, density[0], a["effective"])
+        eb = eager.energy_scalars(phi, None, density[0], b["effective"])
+        torch.cuda.synchronize()
+        assert torch.equal(ea, eb), call
+    assert captured.graph_record["disabled"] is None, captured.graph_record["disabled"]
+    assert captured.graph_record["captures"] == 2
+    assert captured.graph_record["replays"] == 2 * 4
+    assert eager.graph_record["replays"] == 0
