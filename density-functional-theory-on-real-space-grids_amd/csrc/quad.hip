@@ -238,39 +238,39 @@ __global__ void node_stencils_kernel(const double* __restrict__ pts, long m, Gri
   }
 }
 
-// 8 lanes per node (lane a = x offset); nf <= 4 fields.
+// One wavefront per node: lane 8 j1 + j2 takes the (y, z) offsets (j1, j2) of the 8^3 stencil and
+// walks the 8 x offsets, so every load instruction reads 8 rows of 8 consecutive values; the 64
+// partial sums are combined by a fixed butterfly (deterministic). nf <= 4 fields.
 __global__ __launch_bounds__(256) void interpolate_kernel(const double* __restrict__ field, int nf,
                                                           long nfield, const int* __restrict__ base,
                                                           const double* __restrict__ lw, long m,
                                                           GridDesc g, double* __restrict__ out) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long node = gid >> 3;
-  const int a = (int)(gid & 7);
+  const long node = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // wave-uniform
+  if (node >= m) return;
+  const int lane = (int)(threadIdx.x & 63);
+  const int j1 = lane >> 3, j2 = lane & 7;
+  const double* w = lw + node * 24;
+  const double wyz = w[8 + j1] * w[16 + j2];
+  const long plane = (long)g.n1 * g.n2;
+  const long b0 = base[3 * node] - 3;
+  const long col = (long)(base[3 * node + 1] - 3 + j1) * g.n2 + (base[3 * node + 2] - 3 + j2);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  if (node < m) {
-    const int b0 = base[3 * node] - 3 + a, b1 = base[3 * node + 1] - 3, b2 = base[3 * node + 2] - 3;
-    const double wx = lw[node * 24 + a];
-    const long plane = (long)g.n1 * g.n2;
-    for (int bb = 0; bb < 8; ++bb) {
-      const double wxy = wx * lw[node * 24 + 8 + bb];
-      const long row = (long)b0 * plane + (long)(b1 + bb) * g.n2 + b2;
-      for (int cc = 0; cc < 8; ++cc) {
-        const double w = wxy * lw[node * 24 + 16 + cc];
-        for (int f = 0; f < 4; ++f)
-          if (f < nf) acc[f] += w * field[f * nfield + row + cc];
-      }
-    }
-  }
-  for (int f = 0; f < 4; ++f) {
-    double v = acc[f];
-    v += __shfl_xor(v, 1, 64);
-    v += __shfl_xor(v, 2, 64);
-    v += __shfl_xor(v, 4, 64);
-    acc[f] = v;
-  }
-  if (node < m && a == 0)
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const double wa = w[a] * wyz;
+    const long idx = (b0 + a) * plane + col;
+#pragma unroll
     for (int f = 0; f < 4; ++f)
-      if (f < nf) out[f * m + node] = acc[f];
+      if (f < nf) acc[f] += wa * field[f * nfield + idx];
+  }
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    if (f >= nf) break;
+    double v = acc[f];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (lane == 0) out[f * m + node] = v;
+  }
 }
 
 struct CellBox {
@@ -628,7 +628,7 @@ extern "C" int cdft_interpolate_f64(const double* field, int nf, const int* base
     return fail(CDFT_E_ARG, "interpolate: bad arguments (nf=%d)", nf);
   GridDesc g = make_grid(shape, h, origin);
   const long n = (long)g.n0 * g.n1 * g.n2;
-  hipLaunchKernelGGL(interpolate_kernel, dim3(blocks_for(m * 8, 256)), dim3(256), 0, (hipStream_t)stream, field,
+  hipLaunchKernelGGL(interpolate_kernel, dim3(blocks_for(m * 64, 256)), dim3(256), 0, (hipStream_t)stream, field,
                      nf, n, base, lw, m, g, out);
   return check_launch("interpolate");
 }

@@ -412,6 +412,8 @@ class CuspQuadrature:
         bad = torch.zeros(1, dtype=torch.int32, device=self.grid.device)
         _lib.call("cdft_node_stencils_f64", pts.data_ptr(), p, self.grid.c_shape, self.grid.spacing, self.grid.c_origin,
                   base.data_ptr(), lw.data_ptr(), bad.data_ptr(), stream_of(pts))
+        if int(bad.item()):
+            raise ValueError("a point's interpolation stencil leaves the grid")
         return base, lw
 
     def interpolate_with(self, stencils: tuple[torch.Tensor, torch.Tensor], field: torch.Tensor,
