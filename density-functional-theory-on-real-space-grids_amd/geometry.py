@@ -188,6 +188,22 @@ class CuspFactor:
 # ------------------------------------------------------------------------------------------------
 
 
+@functools.lru_cache(maxsize=None)
+def sphere_directions(n_theta: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit directions (GL in cos theta x 2 n_theta uniform phi) and their solid-angle weights,
+    computed once per ``n_theta`` (read-only arrays)."""
+    n_phi = 2 * n_theta
+    cos_nodes, cos_weights = gauss_legendre(n_theta)
+    phis = 2.0 * math.pi * (np.arange(n_phi) + 0.5) / n_phi
+    sin_theta = np.sqrt(np.clip(1.0 - cos_nodes**2, 0.0, 1.0))
+    directions = np.stack([np.outer(sin_theta, np.cos(phis)).ravel(), np.outer(sin_theta, np.sin(phis)).ravel(),
+                           np.repeat(cos_nodes, n_phi)], axis=1)
+    weights = np.repeat(cos_weights, n_phi) * (2.0 * math.pi / n_phi)
+    directions.setflags(write=False)
+    weights.setflags(write=False)
+    return directions, weights
+
+
 def sphere_nodes(centre, radius_sphere, spacing, radial_order, node_spacing):
     """Composite GL radius x GL(cos theta) x uniform phi nodes and bare weights ``r^2 dr dOmega``
     (reference ``quadrature.py:129-180``). Host-side table construction."""
@@ -201,14 +217,9 @@ def sphere_nodes(centre, radius_sphere, spacing, radial_order, node_spacing):
             r = mid + half * node
             wr = half * weight * r * r
             n_theta = max(4, int(math.ceil(math.sqrt(2.0 * math.pi) * r / (node_spacing * spacing))))
-            n_phi = 2 * n_theta
-            cos_nodes, cos_weights = gauss_legendre(n_theta)
-            phis = 2.0 * math.pi * (np.arange(n_phi) + 0.5) / n_phi
-            sin_theta = np.sqrt(np.clip(1.0 - cos_nodes**2, 0.0, 1.0))
-            directions = np.stack([np.outer(sin_theta, np.cos(phis)).ravel(), np.outer(sin_theta, np.sin(phis)).ravel(),
-                                   np.repeat(cos_nodes, n_phi)], axis=1)
+            directions, angular = sphere_directions(n_theta)
             points.append(r * directions)
-            weights.append(wr * (np.repeat(cos_weights, n_phi) * (2.0 * math.pi / n_phi)))
+            weights.append(wr * angular)
     return np.concatenate(points) + np.asarray(centre), np.concatenate(weights)
 
 
@@ -309,9 +320,13 @@ class CuspQuadrature:
         bad = torch.zeros(1, dtype=torch.int32, device=dev)
         _lib.call("cdft_node_stencils_f64", self.points.data_ptr(), m, grid.c_shape, h, grid.c_origin,
                   self.base.data_ptr(), self.lw.data_ptr(), bad.data_ptr(), s)
-        base_host = self.base.cpu().numpy()  # geometry build: one read for the binning
-        if int(bad.item()) != 0:
+        # The kernel's base cells are floor((p - origin) / h) in the same IEEE operations as the host's
+        # ordering key above, so the binning is built from the host copy without a device read.
+        base_host = base_key[perm].astype(np.int32)
+        dims = np.asarray(grid.shape)
+        if (base_host - 3 < 0).any() or (base_host + 4 >= dims).any():
             raise ValueError("an interpolation stencil reaches outside the grid; the sphere radius bound is wrong")
+        self._base_host = base_host
         lo = base_host.min(axis=0)
         ext = base_host.max(axis=0) - lo + 1
         cell = ((base_host[:, 0] - lo[0]) * ext[1] + (base_host[:, 1] - lo[1])) * ext[2] + (base_host[:, 2] - lo[2])
@@ -352,28 +367,29 @@ class CuspQuadrature:
         nodes, with each chunk's z-cell range and the offset of its (64 x height) slab."""
         c2 = int(ext[2])
         ncols = int(ext[0] * ext[1])
-        col_start = cell_start[::c2][: ncols + 1] if cell_start.size == ncols * c2 + 1 else cell_start[np.arange(ncols + 1) * c2]
-        node, zlo, hz, col_chunk = [], [], [], [0]
-        b2 = base_host[:, 2] - lo[2]
-        for col in range(ncols):
-            s, e = int(col_start[col]), int(col_start[col + 1])
-            for a in range(s, e, self.lift_chunk):
-                b = min(e, a + self.lift_chunk)
-                node.append(a)
-                zlo.append(int(b2[a]))
-                hz.append(int(b2[b - 1]) - int(b2[a]) + 8)
-            col_chunk.append(len(node))
-        node.append(self.n_nodes)
-        off = np.zeros(len(hz) + 1, dtype=np.int64)
-        np.cumsum(64 * np.asarray(hz, dtype=np.int64), out=off[1:])
+        col_start = np.asarray(cell_start, dtype=np.int64)[np.arange(ncols + 1) * c2]
+        per_col = -(-(col_start[1:] - col_start[:-1]) // self.lift_chunk)  # chunks per column
+        col_chunk = np.zeros(ncols + 1, dtype=np.int64)
+        np.cumsum(per_col, out=col_chunk[1:])
+        n_chunks = int(col_chunk[-1])
+        # chunk j of column c starts at node col_start[c] + j * lift_chunk
+        col_of = np.repeat(np.arange(ncols), per_col)
+        first = col_start[col_of] + (np.arange(n_chunks) - col_chunk[col_of]) * self.lift_chunk
+        last = np.minimum(first + self.lift_chunk, col_start[col_of + 1]) - 1
+        b2 = (base_host[:, 2] - lo[2]).astype(np.int64)
+        zlo = b2[first]
+        hz = b2[last] - zlo + 8
+        node = np.append(first, self.n_nodes)
+        off = np.zeros(n_chunks + 1, dtype=np.int64)
+        np.cumsum(64 * hz, out=off[1:])
         if off[-1] >= 2**31:
             raise ValueError("the transpose's workspace exceeds 32-bit offsets")
-        self._lift_chunks = len(hz)
-        self._chunk_node = as_dev(np.asarray(node, dtype=np.int32), torch.int32)
-        self._chunk_zlo = as_dev(np.asarray(zlo, dtype=np.int32), torch.int32)
-        self._chunk_hz = as_dev(np.asarray(hz, dtype=np.int32), torch.int32)
+        self._lift_chunks = n_chunks
+        self._chunk_node = as_dev(node.astype(np.int32), torch.int32)
+        self._chunk_zlo = as_dev(zlo.astype(np.int32), torch.int32)
+        self._chunk_hz = as_dev(hz.astype(np.int32), torch.int32)
         self._chunk_off = as_dev(off.astype(np.int32), torch.int32)
-        self._col_chunk = as_dev(np.asarray(col_chunk, dtype=np.int32), torch.int32)
+        self._col_chunk = as_dev(col_chunk.astype(np.int32), torch.int32)
         self._lift_per_field = int(off[-1])
 
     def interpolate(self, field: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:

@@ -82,6 +82,8 @@ def test_quadrature_arrays(he_case):
     assert quad.n_nodes == quad_o.n_nodes
     perm = quad.node_order  # GPU nodes are numbered in base-cell order
     assert sorted(perm.tolist()) == list(range(quad.n_nodes))
+    # the transpose's binning is built from host base cells; they are the kernel's exactly
+    assert np.array_equal(quad.base.cpu().numpy(), quad._base_host)
     assert _rel(quad.points.cpu(), quad_o.points[perm]) < 1e-15
     assert _rel(quad.weights.cpu(), quad_o.weights[perm]) < 1e-13
     assert _rel(quad.f2.cpu(), quad_o.f2[perm]) < 1e-13
@@ -189,6 +191,7 @@ def test_lumped_far_weights_match_oracle(h2_case):
     quad = g.CuspQuadrature(grid, factor)
     assert quad.far_lumped and quad_o.far_lumped
     assert quad.n_nodes == quad_o.n_nodes
+    assert np.array_equal(quad.base.cpu().numpy(), quad._base_host)
     perm = quad.node_order
     assert _rel(quad.weights.cpu(), quad_o.weights[perm]) < 1e-13
     assert _rel(quad.lumped_far.cpu(), quad_o.lumped_far) < 1e-13
