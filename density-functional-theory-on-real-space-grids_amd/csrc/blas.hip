@@ -1,15 +1,14 @@
 // K3 / K4 and the small dense kernels of Rayleigh-Ritz and CheFSI, all float64 and deterministic.
 //
-//   wgram   : S_ij = scale * sum_p q_p a_ip b_jp   (block partials, combined in a fixed order)
+//   wgram   : S_ij = scale * sum_p q_p a_ip b_jp   (block partials in a fixed order, second pass)
 //   rotate  : Y = R X  (R is kout x kin on the device, row-major; or R^T)
 //   lincomb : out = sum_t c_t x_t with c_t a host constant optionally times / over a device scalar
 //   resnorm : r_i = sqrt(max(sum_p q_p (Av_ip - e_i v_ip)^2, 0))
 //   chol_inv: T = L^{-1}, S = L L^T, flag on a non-positive pivot
 //   eigh    : cyclic Jacobi on a symmetrised k x k matrix, eigenvalues ascending
 //
-// Reductions never use floating-point atomics: every block writes its partial sums, and the last
-// block to finish (an integer ticket in the workspace's final slot) sums the partials in block
-// order, so a repeat is bitwise identical and a reduction is one launch.
+// Reductions never use floating-point atomics: every block writes its partial sums, and one block
+// sums the partials in block order, so a repeat is bitwise identical.
 #include "common.h"
 
 namespace cdft {
@@ -40,56 +39,12 @@ __device__ __forceinline__ void block_sums(double (&acc)[COUNT], double* shared 
   }
 }
 
-// Whether this block is the last of the grid to have written its partials. Thread 0 of every block
-// has stored its partials before the call; the release fence orders them before the ticket, the
-// acquire fence in the last block orders the ticket before its reads of the other blocks' partials.
-// The last block resets the counter, so it reads zero at the next launch.
-__device__ __forceinline__ bool last_block(unsigned* counter) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (last) {
-    __threadfence();
-    if (threadIdx.x == 0) *counter = 0u;
-  }
-  return last;
-}
-
-// Entry e of a ka x kb tile from the block partials (stride KMAX^2): lane l sums partials l, l+64,
-// ... in order, then a fixed butterfly; post: 0 plain, 1 sqrt(max(x, 0)). One wavefront per entry.
-__device__ __forceinline__ void final_entry(const double* partials, int nblocks, int e, int kb, double scale,
-                                            int post, double* out, int ldo, int row0, int col0) {
-  const int lane = threadIdx.x & 63;
-  const int i = e / kb, j = e % kb;
-  double s = 0.0;
-  for (int blk = lane; blk < nblocks; blk += 64) s += partials[(long)blk * KMAX * KMAX + i * KMAX + j];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if (lane == 0) {
-    s *= scale;
-    if (post == 1) s = sqrt(s > 0.0 ? s : 0.0);
-    out[(row0 + i) * ldo + col0 + j] = s;
-  }
-}
-
-// The last block's pass over the partials: the wavefronts take the ka x kb entries in turn.
-__device__ __forceinline__ void final_entries(const double* partials, int nblocks, int ka, int kb, double scale,
-                                              int post, double* out, int ldo, int row0, int col0) {
-  const int wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
-  for (int e = wave; e < ka * kb; e += waves) final_entry(partials, nblocks, e, kb, scale, post, out, ldo, row0, col0);
-}
-
 // KA x KB accumulators per thread (compile-time; runtime ka <= KA, kb <= KB guard the loads).
 template <int KA, int KB>
 __global__ __launch_bounds__(RB) void wgram_partial_kernel(const double* __restrict__ a, int ka,
                                                            const double* __restrict__ b, int kb,
                                                            const double* __restrict__ q, long n,
-                                                           double* partials, unsigned* counter, double scale,
-                                                           int post, double* __restrict__ out, int ldo, int row0,
-                                                           int col0) {
+                                                           double* __restrict__ partials) {
   __shared__ double shared[4 * KA * KB];
   double acc[KA * KB];
 #pragma unroll
@@ -117,7 +72,26 @@ __global__ __launch_bounds__(RB) void wgram_partial_kernel(const double* __restr
       for (int j = 0; j < KB; ++j)
         if (i < ka && j < kb) partials[(long)blockIdx.x * KMAX * KMAX + i * KMAX + j] = acc[i * KB + j];
   }
-  if (last_block(counter)) final_entries(partials, gridDim.x, ka, kb, scale, post, out, ldo, row0, col0);
+}
+
+// One wavefront per output entry: lane l sums partials l, l+64, ... in order, then a fixed
+// butterfly. post: 0 plain, 1 sqrt(max(x, 0)).
+__global__ __launch_bounds__(64) void wgram_final_kernel(const double* __restrict__ partials, int nblocks,
+                                                         int ka, int kb, double scale, int post,
+                                                         double* __restrict__ out, int ldo, int row0,
+                                                         int col0) {
+  const int e = blockIdx.x;
+  if (e >= ka * kb) return;
+  const int i = e / kb, j = e % kb;
+  double s = 0.0;
+  for (int blk = threadIdx.x; blk < nblocks; blk += 64) s += partials[(long)blk * KMAX * KMAX + i * KMAX + j];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (threadIdx.x == 0) {
+    s *= scale;
+    if (post == 1) s = sqrt(s > 0.0 ? s : 0.0);
+    out[(row0 + i) * ldo + col0 + j] = s;
+  }
 }
 
 static int reduction_blocks(long n) {
@@ -187,8 +161,7 @@ __global__ __launch_bounds__(RB) void resnorm_partial_kernel(const double* __res
                                                              const double* __restrict__ v,
                                                              const double* __restrict__ evals,
                                                              const double* __restrict__ q, int k,
-                                                             long n, double* partials, unsigned* counter,
-                                                             double* __restrict__ norms) {
+                                                             long n, double* __restrict__ partials) {
   __shared__ double shared[4 * KMAX];
   double acc[KMAX];
   double e[KMAX];
@@ -214,8 +187,6 @@ __global__ __launch_bounds__(RB) void resnorm_partial_kernel(const double* __res
     for (int i = 0; i < KMAX; ++i)
       if (i < k) partials[(long)blockIdx.x * KMAX * KMAX + i * KMAX] = acc[i];
   }
-  // entries (i, 0) of each partial tile, one column, square-rooted
-  if (last_block(counter)) final_entries(partials, gridDim.x, k, 1, 1.0, 1, norms, 1, 0, 0);
 }
 
 // ---- small dense kernels (one thread: k is the subspace size, <= KROT) ----
@@ -473,10 +444,9 @@ extern "C" int cdft_wgram_f64(const double* a, int ka, const double* b, int kb, 
   if (ka < 1 || kb < 1 || n < 1) return fail(CDFT_E_ARG, "wgram: sizes must be positive");
   if (post != 0 && post != 1) return fail(CDFT_E_ARG, "wgram: post must be 0 or 1");
   const int blocks = reduction_blocks(n);
-  if (partials_len < (long)blocks * KMAX * KMAX + 1)
+  if (partials_len < (long)blocks * KMAX * KMAX)
     return fail(CDFT_E_CAPACITY, "wgram: partials workspace holds %ld doubles, needs %ld", partials_len,
-                (long)blocks * KMAX * KMAX + 1);
-  unsigned* counter = reinterpret_cast<unsigned*>(partials + partials_len - 1);
+                (long)blocks * KMAX * KMAX);
   hipStream_t s = (hipStream_t)stream;
   for (int i0 = 0; i0 < ka; i0 += KMAX) {
     for (int j0 = 0; j0 < kb; j0 += KMAX) {
@@ -486,20 +456,23 @@ extern "C" int cdft_wgram_f64(const double* a, int ka, const double* b, int kb, 
       const double* bp = b + (long)j0 * n;
       if (ti == tj && ti <= 8) {
         switch (ti) {
-          case 1: hipLaunchKernelGGL((wgram_partial_kernel<1, 1>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials, counter, scale, post, out, kb, i0, j0); break;
-          case 2: hipLaunchKernelGGL((wgram_partial_kernel<2, 2>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials, counter, scale, post, out, kb, i0, j0); break;
-          case 3: hipLaunchKernelGGL((wgram_partial_kernel<3, 3>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials, counter, scale, post, out, kb, i0, j0); break;
-          case 4: hipLaunchKernelGGL((wgram_partial_kernel<4, 4>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials, counter, scale, post, out, kb, i0, j0); break;
-          case 5: hipLaunchKernelGGL((wgram_partial_kernel<5, 5>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials, counter, scale, post, out, kb, i0, j0); break;
-          case 6: hipLaunchKernelGGL((wgram_partial_kernel<6, 6>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials, counter, scale, post, out, kb, i0, j0); break;
-          case 7: hipLaunchKernelGGL((wgram_partial_kernel<7, 7>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials, counter, scale, post, out, kb, i0, j0); break;
-          default: hipLaunchKernelGGL((wgram_partial_kernel<8, 8>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials, counter, scale, post, out, kb, i0, j0); break;
+          case 1: hipLaunchKernelGGL((wgram_partial_kernel<1, 1>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          case 2: hipLaunchKernelGGL((wgram_partial_kernel<2, 2>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          case 3: hipLaunchKernelGGL((wgram_partial_kernel<3, 3>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          case 4: hipLaunchKernelGGL((wgram_partial_kernel<4, 4>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          case 5: hipLaunchKernelGGL((wgram_partial_kernel<5, 5>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          case 6: hipLaunchKernelGGL((wgram_partial_kernel<6, 6>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          case 7: hipLaunchKernelGGL((wgram_partial_kernel<7, 7>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
+          default: hipLaunchKernelGGL((wgram_partial_kernel<8, 8>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials); break;
         }
       } else {
-        hipLaunchKernelGGL((wgram_partial_kernel<KMAX, KMAX>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n,
-                           partials, counter, scale, post, out, kb, i0, j0);
+        hipLaunchKernelGGL((wgram_partial_kernel<KMAX, KMAX>), dim3(blocks), dim3(RB), 0, s, ap, ti, bp, tj, q, n, partials);
       }
-      const int rc = check_launch("wgram");
+      int rc = check_launch("wgram_partial");
+      if (rc) return rc;
+      hipLaunchKernelGGL(wgram_final_kernel, dim3(ti * tj), dim3(64), 0, s, partials, blocks, ti, tj, scale,
+                         post, out, kb, i0, j0);
+      rc = check_launch("wgram_final");
       if (rc) return rc;
     }
   }
@@ -507,7 +480,7 @@ extern "C" int cdft_wgram_f64(const double* a, int ka, const double* b, int kb, 
 }
 
 extern "C" long cdft_reduction_workspace_doubles(long n) {
-  return (long)reduction_blocks(n) * KMAX * KMAX + 1;  // partial tiles, then the ticket slot
+  return (long)reduction_blocks(n) * KMAX * KMAX;
 }
 
 extern "C" int cdft_rotate_f64(const double* r, int kin, int kout, int transpose, const double* x,
@@ -550,11 +523,14 @@ extern "C" int cdft_resnorm_f64(const double* av, const double* v, const double*
   if (!av || !v || !evals || !norms || !partials) return fail(CDFT_E_ARG, "resnorm: null pointer argument");
   if (k < 1 || k > KMAX || n < 1) return fail(CDFT_E_ARG, "resnorm: k must be in 1..%d", KMAX);
   const int blocks = reduction_blocks(n);
-  if (partials_len < (long)blocks * KMAX * KMAX + 1) return fail(CDFT_E_CAPACITY, "resnorm: partials too small");
-  unsigned* counter = reinterpret_cast<unsigned*>(partials + partials_len - 1);
-  hipLaunchKernelGGL(resnorm_partial_kernel, dim3(blocks), dim3(RB), 0, (hipStream_t)stream, av, v, evals, q, k, n,
-                     partials, counter, norms);
-  return check_launch("resnorm");
+  if (partials_len < (long)blocks * KMAX * KMAX) return fail(CDFT_E_CAPACITY, "resnorm: partials too small");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(resnorm_partial_kernel, dim3(blocks), dim3(RB), 0, s, av, v, evals, q, k, n, partials);
+  int rc = check_launch("resnorm_partial");
+  if (rc) return rc;
+  // Reuse the Gram final pass: entries (i, 0) of each partial tile, one column.
+  hipLaunchKernelGGL(wgram_final_kernel, dim3(k), dim3(64), 0, s, partials, blocks, k, 1, 1.0, 1, norms, 1, 0, 0);
+  return check_launch("resnorm_final");
 }
 
 extern "C" int cdft_chol_inv_f64(const double* s, int k, double* t, int* flag, void* stream) {

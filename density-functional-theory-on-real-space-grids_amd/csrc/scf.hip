@@ -139,15 +139,12 @@ __global__ void small_solve_kernel(const double* __restrict__ a_in, const double
   }
 }
 
-// Reductions with block partials (no floating-point atomics): mode 0: sum a*b*w (w, b optional),
-// mode 1: sum w*|a|, mode 2: max |a - b|. The last block to finish (an integer ticket in the
-// workspace's final slot, reset by that block) combines the partials in block order:
-// out[slot] = scale * (combined) (+ out[slot] if accumulate). One launch, bitwise repeatable.
-__global__ __launch_bounds__(RBS) void reduce_kernel(int mode, const double* __restrict__ a,
-                                                     const double* __restrict__ b,
-                                                     const double* __restrict__ w, long len,
-                                                     double* partials, unsigned* counter, double scale,
-                                                     int accumulate, double* __restrict__ out, int slot) {
+// Reductions with block partials (no atomics): mode 0: sum a*b*w (w, b optional),
+// mode 1: sum w*|a|, mode 2: max |a - b|.
+__global__ __launch_bounds__(RBS) void reduce_partial_kernel(int mode, const double* __restrict__ a,
+                                                             const double* __restrict__ b,
+                                                             const double* __restrict__ w, long len,
+                                                             double* __restrict__ partials) {
   __shared__ double sh[RBS / 64];
   double acc = 0.0;
   const long stride = (long)gridDim.x * RBS;
@@ -170,21 +167,19 @@ __global__ __launch_bounds__(RBS) void reduce_kernel(int mode, const double* __r
   }
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
   __syncthreads();
-  __shared__ bool last;
   if (threadIdx.x == 0) {
     double r = sh[0];
     for (int i = 1; i < RBS / 64; ++i) r = (mode == 2) ? (sh[i] > r ? sh[i] : r) : r + sh[i];
     partials[blockIdx.x] = r;
-    __threadfence();  // release the partial before taking a ticket
-    last = atomicAdd(counter, 1u) == gridDim.x - 1;
   }
-  __syncthreads();
-  if (!last || threadIdx.x >= 64) return;
-  __threadfence();  // acquire: the other blocks' partials
-  if (threadIdx.x == 0) *counter = 0u;
-  // wavefront 0: lane l combines partials l, l+64, ... in order, then a fixed butterfly
-  const int nblocks = (int)gridDim.x;
-  double r = 0.0;
+}
+
+// out[slot] = scale * (reduction of partials) (+ out[slot] if accumulate).
+// One wavefront: lane l combines partials l, l+64, ... in order, then a fixed butterfly.
+__global__ __launch_bounds__(64) void reduce_final_kernel(int mode, const double* __restrict__ partials,
+                                                          int nblocks, double scale, int accumulate,
+                                                          double* __restrict__ out, int slot) {
+  double r = (mode == 2) ? 0.0 : 0.0;
   for (int i = threadIdx.x; i < nblocks; i += 64) r = (mode == 2) ? (partials[i] > r ? partials[i] : r) : r + partials[i];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -252,7 +247,7 @@ extern "C" int cdft_small_solve_f64(const double* a, const double* scale, const 
   return check_launch("small_solve");
 }
 
-extern "C" long cdft_reduce_workspace_doubles(long len) { return red_blocks(len) + 1; }  // + the ticket slot
+extern "C" long cdft_reduce_workspace_doubles(long len) { return red_blocks(len); }
 
 // out[slot] = scale * reduce(mode, a, b, w) (+= if accumulate). mode 0: sum a b w; 1: sum w |a|;
 // 2: max |a - b|.
@@ -262,9 +257,11 @@ extern "C" int cdft_reduce_f64(int mode, const double* a, const double* b, const
   if (!a || !out || !partials || len < 1 || mode < 0 || mode > 2) return fail(CDFT_E_ARG, "reduce: bad arguments");
   if ((mode == 1 && !w) || (mode == 2 && !b)) return fail(CDFT_E_ARG, "reduce: missing operand for mode %d", mode);
   const int blocks = red_blocks(len);
-  if (partials_len < blocks + 1) return fail(CDFT_E_CAPACITY, "reduce: partials workspace too small");
-  unsigned* counter = reinterpret_cast<unsigned*>(partials + partials_len - 1);
-  hipLaunchKernelGGL(reduce_kernel, dim3(blocks), dim3(RBS), 0, (hipStream_t)stream, mode, a, b, w, len, partials,
-                     counter, scale, accumulate, out, slot);
-  return check_launch("reduce");
+  if (partials_len < blocks) return fail(CDFT_E_CAPACITY, "reduce: partials workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(reduce_partial_kernel, dim3(blocks), dim3(RBS), 0, s, mode, a, b, w, len, partials);
+  int rc = check_launch("reduce_partial");
+  if (rc) return rc;
+  hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(64), 0, s, mode, partials, blocks, scale, accumulate, out, slot);
+  return check_launch("reduce_final");
 }

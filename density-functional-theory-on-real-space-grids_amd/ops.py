@@ -184,21 +184,16 @@ def cheb_filter(x: torch.Tensor, geom: StencilGeometry, table: torch.Tensor, deg
 # ------------------------------------------------------------------------------------------------
 
 _workspaces: dict[tuple, torch.Tensor] = {}
-_retired: list[torch.Tensor] = []
 
 
 def partials_for(n: int, device: torch.device) -> torch.Tensor:
-    """The block-partials workspace of the deterministic reductions for length ``n`` (one per
-    device, grown on demand). Its last slot is the reductions' block ticket, which must start at
-    zero (the library returns it to zero after every reduction), so it is allocated zeroed. A grown
-    workspace's predecessor is kept alive: captured graphs may still address it."""
+    """The block-partials workspace of the deterministic reductions for length ``n`` (cached per
+    device and size class; allocated once through torch)."""
     need = int(_lib.load().cdft_reduction_workspace_doubles(int(n)))
     key = ("partials", str(device))
     ws = _workspaces.get(key)
     if ws is None or ws.numel() < need:
-        if ws is not None:
-            _retired.append(ws)
-        ws = torch.zeros(max(need, 64 * 64 + 1), dtype=torch.float64, device=device)
+        ws = torch.empty(max(need, 64 * 64), dtype=torch.float64, device=device)
         _workspaces[key] = ws
     return ws
 
@@ -340,7 +335,7 @@ RED_DOT, RED_L1, RED_MAXDIFF = 0, 1, 2
 def reduce(mode: int, a: torch.Tensor, out: torch.Tensor, slot: int = 0, b: torch.Tensor | None = None,
            w: torch.Tensor | None = None, scale: float = 1.0, accumulate: bool = False) -> torch.Tensor:
     """``out[slot] = scale * reduction`` (``+=`` with ``accumulate``): 0 = sum a b w, 1 = sum w |a|,
-    2 = max |a - b|. Deterministic: block partials combined in block order by the last block."""
+    2 = max |a - b|. Deterministic two-pass, no atomics."""
     n = a.numel()
     need = int(_lib.load().cdft_reduce_workspace_doubles(n))
     ws = partials_for(max(n, 1), a.device)

@@ -99,9 +99,7 @@ int cdft_hdiv_apply_batched_f64(const void* desc, int n_grids, const int* launch
 
 /* ---- K3 / K4: float64 reductions and rotations of Rayleigh-Ritz (deterministic, no atomics) ---- */
 
-/* Doubles of block-partials workspace the reductions below need for length n. The workspace's
- * LAST double is the reductions' block ticket: zero it once before first use; every reduction
- * leaves it at zero. Reductions sharing a workspace must be ordered (one stream). */
+/* Doubles of block-partials workspace the reductions below need for length n. */
 long cdft_reduction_workspace_doubles(long n);
 
 /* out[i*kb + j] = scale * sum_p q[p] a[i*n + p] b[j*n + p] (q may be NULL for unit weights);
@@ -283,11 +281,10 @@ int cdft_atom_combine_f64(const double* base, const double* c, const double* fie
 int cdft_small_solve_f64(const double* a, const double* scale, const double* b, int na, double* x,
                          void* stream);
 
-/* Doubles of block-partials workspace cdft_reduce_f64 needs for len values (the last double is the
- * block ticket, zeroed before first use, as for cdft_reduction_workspace_doubles). */
+/* Doubles of block-partials workspace cdft_reduce_f64 needs for len values. */
 long cdft_reduce_workspace_doubles(long len);
 /* out[slot] = scale * R (or += with accumulate), R by mode: 0 sum a b w (b, w may be NULL);
- * 1 sum w |a|; 2 max |a - b|. Deterministic single-launch reduction. The integrals of
+ * 1 sum w |a|; 2 max |a - b|. Deterministic two-pass reduction. The integrals of
  * CuspQuadrature.integrate, the energy terms and the SCF residual norms
  * (src/cdft/operators/quadrature.py:701-709; src/cdft/scf/step.py:398-439;
  * src/cdft/scf/loop.py:345-412). */

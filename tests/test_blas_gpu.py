@@ -89,41 +89,3 @@ def test_cholesky_flags_indefinite(pkg, device):
     flag = torch.zeros(1, dtype=torch.int32, device=device)
     pkg.ops.chol_inv(s, t, flag)
     assert int(flag.item()) == 1
-
-
-def test_single_launch_reductions_repeat_and_reset_their_ticket(pkg, device):
-    """The last block to finish combines the block partials: results repeat bitwise across many
-    launches of different grid sizes (eager and replayed from a captured graph, with eager work
-    between replays) and the workspace's ticket slot is back at zero after every call."""
-    rng = np.random.default_rng(3)
-    n = 300001
-    a = torch.as_tensor(rng.standard_normal((5, n))).to(device)
-    q = torch.as_tensor(rng.uniform(0.1, 2.0, n)).to(device)
-    small = torch.as_tensor(rng.standard_normal(777)).to(device)
-    ref_gram = pkg.ops.wgram(a, None, q).clone()
-    out = torch.zeros(2, dtype=torch.float64, device=device)
-    pkg.ops.reduce(pkg.ops.RED_L1, a[0], out, 0, w=q)
-    pkg.ops.reduce(pkg.ops.RED_MAXDIFF, a[1], out, 1, b=a[2])
-    ref_out = out.clone()
-    ws = pkg.ops.partials_for(n, a.device)
-    for _ in range(20):
-        assert torch.equal(pkg.ops.wgram(a, None, q), ref_gram)
-        pkg.ops.reduce(pkg.ops.RED_DOT, small, out, 0, b=small)  # another grid size in between
-        pkg.ops.reduce(pkg.ops.RED_L1, a[0], out, 0, w=q)
-        pkg.ops.reduce(pkg.ops.RED_MAXDIFF, a[1], out, 1, b=a[2])
-        assert torch.equal(out, ref_out)
-    assert float(ws[-1].item()) == 0.0
-    gram = torch.empty(5, 5, dtype=torch.float64, device=device)
-    graph = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(graph):
-        pkg.ops.wgram(a, None, q, out=gram)
-        pkg.ops.reduce(pkg.ops.RED_L1, a[0], out, 0, w=q)
-    for _ in range(4):
-        gram.zero_()
-        out.zero_()
-        graph.replay()
-        torch.cuda.synchronize()
-        assert torch.equal(gram, ref_gram)
-        assert torch.equal(out[0], ref_out[0])
-        assert torch.equal(pkg.ops.wgram(a, None, q), ref_gram)
-    assert float(ws[-1].item()) == 0.0
