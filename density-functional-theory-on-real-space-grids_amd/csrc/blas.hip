@@ -50,6 +50,7 @@ __global__ __launch_bounds__(RB) void wgram_partial_kernel(const double* __restr
 #pragma unroll
   for (int i = 0; i < KA * KB; ++i) acc[i] = 0.0;
   const long stride = (long)gridDim.x * RB;
+  const bool same = a == b && ka == kb;  // a Gram matrix: each column is read once
   for (long p = (long)blockIdx.x * RB + threadIdx.x; p < n; p += stride) {
     const double w = q ? q[p] : 1.0;
     double bv[KB];
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(RB) void wgram_partial_kernel(const double* __restr
 #pragma unroll
     for (int i = 0; i < KA; ++i) {
       if (i < ka) {
-        const double av = a[(long)i * n + p] * w;
+        const double av = (same && i < KB ? bv[i] : a[(long)i * n + p]) * w;
 #pragma unroll
         for (int j = 0; j < KB; ++j) acc[i * KB + j] += av * bv[j];
       }
@@ -95,7 +96,8 @@ __global__ __launch_bounds__(64) void wgram_final_kernel(const double* __restric
 }
 
 static int reduction_blocks(long n) {
-  long blocks = (n + 1023) / 1024;
+  // about one point per thread: the reductions are latency-bound at a few blocks per CU
+  long blocks = (n + RB - 1) / RB;
   if (blocks < 1) blocks = 1;
   if (blocks > MAX_PARTIALS) blocks = MAX_PARTIALS;
   return (int)blocks;

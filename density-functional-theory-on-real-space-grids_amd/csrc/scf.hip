@@ -139,6 +139,66 @@ __global__ void small_solve_kernel(const double* __restrict__ a_in, const double
   }
 }
 
+// Pulay/DIIS coefficients on the device (reference src/cdft/scf/mixing.py:86-162): from the Gram
+// matrix G of the residual history (leading dimension ldg), the m active slots in age order
+// (oldest first), G_ij = gram[slot_i][slot_j] + 1e-12 max_k G_kk delta_ij, solve
+//   [G  -1] [c]   [0]
+//   [1^T 0] [l] = [1]
+// by Gaussian elimination with partial pivoting; coef[2i] = c_i, coef[2i+1] = alpha c_i.
+struct Slots {
+  int s[16];
+};
+__global__ void pulay_kernel(const double* __restrict__ gram, int ldg, Slots slots, int m, double alpha,
+                             double* __restrict__ coef) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int n = m + 1;
+  double a[17 * 17], b[17];
+  double scale = 0.0;
+  for (int i = 0; i < m; ++i) {
+    const double d = gram[slots.s[i] * ldg + slots.s[i]];
+    scale = d > scale ? d : scale;
+  }
+  if (scale == 0.0) scale = 1.0;
+  for (int i = 0; i < m; ++i) {
+    for (int j = 0; j < m; ++j) a[i * n + j] = gram[slots.s[i] * ldg + slots.s[j]];
+    a[i * n + i] += 1.0e-12 * scale;
+    a[i * n + m] = -1.0;
+    a[m * n + i] = 1.0;
+    b[i] = 0.0;
+  }
+  a[m * n + m] = 0.0;
+  b[m] = 1.0;
+  for (int col = 0; col < n; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < n; ++r)
+      if (fabs(a[r * n + col]) > fabs(a[piv * n + col])) piv = r;
+    if (piv != col) {
+      for (int j = 0; j < n; ++j) {
+        const double t = a[col * n + j];
+        a[col * n + j] = a[piv * n + j];
+        a[piv * n + j] = t;
+      }
+      const double t = b[col];
+      b[col] = b[piv];
+      b[piv] = t;
+    }
+    for (int r = col + 1; r < n; ++r) {
+      const double f = a[r * n + col] / a[col * n + col];
+      for (int j = col; j < n; ++j) a[r * n + j] -= f * a[col * n + j];
+      b[r] -= f * b[col];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double t = b[i];
+    for (int j = i + 1; j < n; ++j) t -= a[i * n + j] * b[j];
+    b[i] = t / a[i * n + i];
+  }
+  for (int i = 0; i < m; ++i) {
+    coef[2 * i] = b[i];
+    coef[2 * i + 1] = alpha * b[i];
+  }
+}
+
 // Reductions with block partials (no atomics): mode 0: sum a*b*w (w, b optional),
 // mode 1: sum w*|a|, mode 2: max |a - b|.
 __global__ __launch_bounds__(RBS) void reduce_partial_kernel(int mode, const double* __restrict__ a,
@@ -245,6 +305,20 @@ extern "C" int cdft_small_solve_f64(const double* a, const double* scale, const 
   if (!a || !scale || !b || !x || na < 1 || na > 8) return fail(CDFT_E_ARG, "small_solve: bad arguments");
   hipLaunchKernelGGL(small_solve_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, a, scale, b, na, x);
   return check_launch("small_solve");
+}
+
+// slots: host array of the m active history slots, oldest first (m <= 16, slots < ldg).
+extern "C" int cdft_pulay_coefficients_f64(const double* gram, int ldg, const int* slots, int m, double alpha,
+                                           double* coef, void* stream) {
+  if (!gram || !slots || !coef || m < 1 || m > 16 || ldg < m) return fail(CDFT_E_ARG, "pulay: bad arguments (m=%d)", m);
+  Slots sl;
+  for (int i = 0; i < 16; ++i) sl.s[i] = 0;
+  for (int i = 0; i < m; ++i) {
+    if (slots[i] < 0 || slots[i] >= ldg) return fail(CDFT_E_ARG, "pulay: slot %d out of range", slots[i]);
+    sl.s[i] = slots[i];
+  }
+  hipLaunchKernelGGL(pulay_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, gram, ldg, sl, m, alpha, coef);
+  return check_launch("pulay");
 }
 
 extern "C" long cdft_reduce_workspace_doubles(long len) { return red_blocks(len); }

@@ -21,7 +21,7 @@ import torch
 
 from . import ops
 from .chefsi import chebyshev_coefficients
-from .hamiltonian import seeded_randn
+from .hamiltonian import lanczos_enqueue, lanczos_finish, seeded_randn
 
 BOUND_REUSE_RITZ_FRACTION = 0.9
 
@@ -242,6 +242,25 @@ class ChebyshevFilteredSubspace:
             self.filter_hook("end", self, work.x32 if use32 else work.vectors, degree)
         self._replay(("ritz", work.k), lambda: self._ritz(work))
 
+    def _lanczos(self, hamiltonian, generator) -> tuple[float, float]:
+        """The operator's Lanczos enclosure (``hamiltonian.spectral_bounds``: the same seeded start,
+        recurrence and widening) run through a captured graph on the solver's static geometry,
+        which holds this operator's potential; one read of the recurrence scalars."""
+        cfg = self.config
+        steps = cfg.lanczos_steps
+        start = seeded_randn(self.grid.n_points, generator, self.grid.device)
+        buffers = self.__dict__.setdefault("_lanczos_buffers", {})
+        if steps not in buffers:
+            f = dict(dtype=torch.float64, device=self.grid.device)
+            buffers[steps] = (torch.empty(self.grid.n_points, **f), torch.zeros(2 * steps + 3, **f))
+        start_s, scal = buffers[steps]
+        start_s.copy_(start)
+        geom, measure = self._geom64, self._measure
+        self._replay(("lanczos", steps),
+                     lambda: lanczos_enqueue(lambda x, y: ops.hdiv_apply(x, geom, y), measure, start_s, steps, scal))
+        hamiltonian.note_applications(steps)
+        return lanczos_finish(scal.cpu().numpy(), steps, cfg.bound_safety)
+
     def _upload_table(self, work: RitzWork, lower, upper, smin) -> int:
         rows = chebyshev_coefficients(self.config.chebyshev_degree, lower, upper, smin)
         work.table_host.numpy()[:] = np.asarray(rows)
@@ -273,7 +292,7 @@ class ChebyshevFilteredSubspace:
         self._bind(hamiltonian)
 
         def lanczos():
-            return hamiltonian.spectral_bounds(steps=cfg.lanczos_steps, safety=cfg.bound_safety, generator=generator)
+            return self._lanczos(hamiltonian, generator)
 
         source, shift_used = "unset", 0.0
         if bounds is None:

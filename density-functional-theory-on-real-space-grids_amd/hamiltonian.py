@@ -274,10 +274,17 @@ def lanczos_launch(hamiltonian, start: torch.Tensor, steps: int) -> torch.Tensor
     """The device part of :func:`lanczos_bounds`: enqueue the run and return the device array whose
     first ``2 steps`` entries are the alphas and betas (read it when the bounds are needed, so
     several runs can share one transfer)."""
-    m = hamiltonian.measure
+    return lanczos_enqueue(lambda x, y: hamiltonian.apply(x, y), hamiltonian.measure, start, steps)
+
+
+def lanczos_enqueue(apply, m, start: torch.Tensor, steps: int, scal: torch.Tensor | None = None) -> torch.Tensor:
+    """The Lanczos recurrence of :func:`lanczos_launch` for an operator given by ``apply(x, y)``
+    (``(1, N)`` blocks) and a measure ``m``, into ``scal`` (``2 steps + 3`` doubles; every entry is
+    written before it is read). Only device work: it can be captured in a graph."""
     dev = start.device
     n = start.numel()
-    scal = torch.zeros(2 * steps + 3, dtype=torch.float64, device=dev)
+    if scal is None:
+        scal = torch.zeros(2 * steps + 3, dtype=torch.float64, device=dev)
     a_slot = lambda i: i  # noqa: E731
     b_slot = lambda i: steps + i  # noqa: E731
     s1, s2, s0 = 2 * steps, 2 * steps + 1, 2 * steps + 2
@@ -286,7 +293,7 @@ def lanczos_launch(hamiltonian, start: torch.Tensor, steps: int) -> torch.Tensor
     v_prev = torch.zeros_like(v)
     w = torch.empty_like(v)
     for i in range(steps):
-        hamiltonian.apply(v.view(1, -1), w.view(1, -1))
+        apply(v.view(1, -1), w.view(1, -1))
         m.inner_into(v, w, scal[a_slot(i):a_slot(i) + 1])
         terms = [(w, 1.0), (v, -1.0, 1, a_slot(i))]
         if i > 0:

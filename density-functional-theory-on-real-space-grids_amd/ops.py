@@ -184,15 +184,19 @@ def cheb_filter(x: torch.Tensor, geom: StencilGeometry, table: torch.Tensor, deg
 # ------------------------------------------------------------------------------------------------
 
 _workspaces: dict[tuple, torch.Tensor] = {}
+_retired: list[torch.Tensor] = []
 
 
 def partials_for(n: int, device: torch.device) -> torch.Tensor:
-    """The block-partials workspace of the deterministic reductions for length ``n`` (cached per
-    device and size class; allocated once through torch)."""
+    """The block-partials workspace of the deterministic reductions for length ``n`` (one per
+    device, grown on demand through torch). A grown workspace's predecessor is kept alive:
+    captured graphs may still address it."""
     need = int(_lib.load().cdft_reduction_workspace_doubles(int(n)))
     key = ("partials", str(device))
     ws = _workspaces.get(key)
     if ws is None or ws.numel() < need:
+        if ws is not None:
+            _retired.append(ws)
         ws = torch.empty(max(need, 64 * 64), dtype=torch.float64, device=device)
         _workspaces[key] = ws
     return ws
@@ -321,6 +325,19 @@ def atom_combine(base: torch.Tensor, c: torch.Tensor, fields: torch.Tensor, sign
     _lib.call("cdft_atom_combine_f64", base.data_ptr(), c.data_ptr(), fields.data_ptr(), fields.shape[0],
               base.numel(), float(sign), out.data_ptr(), stream_of(out))
     return out
+
+
+def pulay_coefficients(gram: torch.Tensor, slots, alpha: float, coef: torch.Tensor) -> torch.Tensor:
+    """Device Pulay/DIIS coefficients from the history Gram matrix ``gram`` (``(H, H)``) for the
+    active ``slots`` (oldest first): ``coef[2i] = c_i``, ``coef[2i+1] = alpha c_i``."""
+    h = gram.shape[0]
+    require(gram, (h, h), torch.float64, gram.device, "gram")
+    if coef.numel() < 2 * len(slots):
+        raise ValueError("pulay_coefficients: coef too small")
+    arr = (ctypes.c_int * len(slots))(*[int(v) for v in slots])
+    _lib.call("cdft_pulay_coefficients_f64", gram.data_ptr(), h, ctypes.addressof(arr), len(slots), float(alpha),
+              coef.data_ptr(), stream_of(gram))
+    return coef
 
 
 def small_solve(a: torch.Tensor, scale: torch.Tensor, b: torch.Tensor, x: torch.Tensor) -> torch.Tensor:

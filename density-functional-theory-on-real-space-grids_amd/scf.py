@@ -285,51 +285,46 @@ def energies_from_host(k, occ, evals, host_scalars, e_in, e_out, ion_ion):
 
 
 class PulayMixer:
-    """Periodic Pulay/DIIS with Tikhonov regularisation (reference ``scf/mixing.py:86-162``); the
-    residual products are device reductions read in one transfer, the small system is solved on the
-    host, the combination is one device kernel."""
+    """Periodic Pulay/DIIS with Tikhonov regularisation (reference ``scf/mixing.py:86-162``), all
+    on the device: the history lives in two ring buffers of ``history`` slots, the residual Gram
+    matrix is one reduction over the whole ring, the bordered system is solved by a one-thread
+    kernel and the combination reads its coefficients from device memory, so a mixing step never
+    waits for the GPU."""
 
     def __init__(self, alpha, history, period, inner_weights: torch.Tensor):
         self.alpha, self.history, self.period = alpha, history, period
+        if not 1 <= history <= 8:
+            raise ValueError("history must be 1..8 (the combination takes at most 16 terms)")
         self.w = inner_weights
-        self.inputs, self.residuals = [], []
+        self.inputs = self.residuals = None  # (history, N) rings
+        self.order: list[int] = []  # active slots, oldest first
+        self.coef = None
         self.name = f"periodic_pulay(alpha={alpha:g}, history={history}, period={period})"
 
     def mix(self, n_in: torch.Tensor, n_out: torch.Tensor, iteration: int) -> torch.Tensor:
-        residual = torch.empty_like(n_in)
-        ops.ewise(ops.EW_SUB, residual.view(-1), n_out.reshape(-1), n_in.reshape(-1))
-        self.inputs.append(n_in.clone())
-        self.residuals.append(residual)
-        if len(self.inputs) > self.history:
-            self.inputs.pop(0)
-            self.residuals.pop(0)
-        m = len(self.inputs)
+        n = n_in.numel()
+        if self.inputs is None:
+            f = dict(dtype=torch.float64, device=n_in.device)
+            self.inputs = torch.zeros((self.history, n), **f)
+            self.residuals = torch.zeros((self.history, n), **f)
+            self.coef = torch.zeros(2 * self.history, **f)
+        slot = len(self.order) if len(self.order) < self.history else self.order.pop(0)
+        self.order.append(slot)
+        self.inputs[slot].copy_(n_in.reshape(-1))
+        residual = self.residuals[slot]
+        ops.ewise(ops.EW_SUB, residual, n_out.reshape(-1), n_in.reshape(-1))
+        m = len(self.order)
         out = torch.empty_like(n_in)
         if m < 2 or iteration % self.period != 0:
-            ops.lincomb(out.view(-1), [(n_in.reshape(-1), 1.0), (residual.reshape(-1), self.alpha)])
+            ops.lincomb(out.view(-1), [(n_in.reshape(-1), 1.0), (residual, self.alpha)])
             return out
-        pairs = [(i, j) for i in range(m) for j in range(i, m)]
-        vals = torch.empty(len(pairs), dtype=torch.float64, device=n_in.device)
-        for slot, (i, j) in enumerate(pairs):
-            ops.reduce(ops.RED_DOT, self.residuals[i].reshape(-1), vals, slot, b=self.residuals[j].reshape(-1), w=self.w)
-        host = vals.cpu().numpy()
-        gram = np.zeros((m, m))
-        for (i, j), v in zip(pairs, host):
-            gram[i, j] = gram[j, i] = v
-        scale = float(gram.diagonal().max()) or 1.0
-        gram = gram + 1.0e-12 * scale * np.eye(m)
-        system = np.zeros((m + 1, m + 1))
-        system[:m, :m] = gram
-        system[:m, m] = -1.0
-        system[m, :m] = 1.0
-        rhs = np.zeros(m + 1)
-        rhs[m] = 1.0
-        coeffs = np.linalg.solve(system, rhs)[:m]
+        gram = ops.wgram(self.residuals, None, self.w)
+        ops.pulay_coefficients(gram, self.order, self.alpha, self.coef)
         terms = []
-        for c, si, sr in zip(coeffs, self.inputs, self.residuals):
-            terms.append((si.reshape(-1), float(c)))
-            terms.append((sr.reshape(-1), float(c) * self.alpha))
-        ops.lincomb(out.view(-1), terms)
+        for i, s in enumerate(self.order):
+            terms.append((self.inputs[s], 1.0, 1, 2 * i))
+            terms.append((self.residuals[s], 1.0, 1, 2 * i + 1))
+        ops.lincomb(out.view(-1), terms, self.coef)
         return out
 
 

@@ -278,6 +278,13 @@ int cdft_atom_combine_f64(const double* base, const double* c, const double* fie
 
 /* x = A^-1 (b * scale) for a small dense system on the device (na <= 8, partial pivoting). The
  * cusp coefficients c = S^-1 n(R_a) (src/cdft/scf/step.py:239-250). */
+/* Pulay/DIIS coefficients on the device: gram (ldg x ldg, the residual history's Gram matrix in the
+ * mixing inner product), slots (HOST, m <= 16 active history slots, oldest first); solves the
+ * Tikhonov-regularised (1e-12 max diag) bordered system and writes coef[2i] = c_i,
+ * coef[2i+1] = alpha c_i. Replaces the host solve of PeriodicPulayMixer.mix
+ * (src/cdft/scf/mixing.py:86-162). */
+int cdft_pulay_coefficients_f64(const double* gram, int ldg, const int* slots, int m, double alpha, double* coef,
+                                void* stream);
 int cdft_small_solve_f64(const double* a, const double* scale, const double* b, int na, double* x,
                          void* stream);
 
