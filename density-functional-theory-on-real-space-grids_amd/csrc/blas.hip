@@ -96,8 +96,9 @@ __global__ __launch_bounds__(64) void wgram_final_kernel(const double* __restric
 }
 
 static int reduction_blocks(long n) {
-  // about one point per thread: the reductions are latency-bound at a few blocks per CU
-  long blocks = (n + RB - 1) / RB;
+  // four points per thread: the block-sum epilogue (k^2 wavefront reductions) outweighs the
+  // loads at one point per thread
+  long blocks = (n + 4 * RB - 1) / (4 * RB);
   if (blocks < 1) blocks = 1;
   if (blocks > MAX_PARTIALS) blocks = MAX_PARTIALS;
   return (int)blocks;
