@@ -51,17 +51,39 @@ __global__ __launch_bounds__(RB) void wgram_partial_kernel(const double* __restr
   for (int i = 0; i < KA * KB; ++i) acc[i] = 0.0;
   const long stride = (long)gridDim.x * RB;
   const bool same = a == b && ka == kb;  // a Gram matrix: each column is read once
-  for (long p = (long)blockIdx.x * RB + threadIdx.x; p < n; p += stride) {
-    const double w = q ? q[p] : 1.0;
-    double bv[KB];
+  // U points per pass, all loads issued before the products (one pass covers a thread's points at
+  // the usual launch); the points are accumulated in ascending order, as a plain loop would.
+  constexpr int U = KA * KB <= 25 ? 4 : (KA * KB <= 36 ? 2 : 1);  // register budget
+  for (long p0 = (long)blockIdx.x * RB + threadIdx.x; p0 < n; p0 += U * stride) {
+    double w[U], bv[U][KB];
 #pragma unroll
-    for (int j = 0; j < KB; ++j) bv[j] = (j < kb) ? b[(long)j * n + p] : 0.0;
+    for (int u = 0; u < U; ++u) {
+      const long p = p0 + u * stride;
+      const bool in = p < n;
+      w[u] = in ? (q ? q[p] : 1.0) : 0.0;
 #pragma unroll
-    for (int i = 0; i < KA; ++i) {
-      if (i < ka) {
-        const double av = (same && i < KB ? bv[i] : a[(long)i * n + p]) * w;
+      for (int j = 0; j < KB; ++j) bv[u][j] = (in && j < kb) ? b[(long)j * n + p] : 0.0;
+    }
+    double avs[U][KA];
+    if (!same) {
 #pragma unroll
-        for (int j = 0; j < KB; ++j) acc[i * KB + j] += av * bv[j];
+      for (int u = 0; u < U; ++u) {
+        const long p = p0 + u * stride;
+#pragma unroll
+        for (int i = 0; i < KA; ++i) avs[u][i] = (p < n && i < ka) ? a[(long)i * n + p] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (p0 + u * stride < n) {
+#pragma unroll
+        for (int i = 0; i < KA; ++i) {
+          if (i < ka) {
+            const double av = (same && i < KB ? bv[u][i] : avs[u][i]) * w[u];
+#pragma unroll
+            for (int j = 0; j < KB; ++j) acc[i * KB + j] += av * bv[u][j];
+          }
+        }
       }
     }
   }
