@@ -140,3 +140,33 @@ def test_sternheimer_gradient_matches_finite_differences(pkg, device):
         fd = float(lp - lm) / (2 * step)
         an = float((grad * d).sum())
         assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-30), (seed, fd, an)
+
+
+def test_sternheimer_gradient_pointwise_finite_differences(pkg, device):
+    """dL/dV at 64 seeded grid points (inside the region where the density lives) against central
+    differences of the GPU forward map with V perturbed at that one point (step 1e-3)."""
+    from cdft_amd.adjoint import KohnShamDensity, KohnShamProblem, density_loss
+
+    grid_o, ham_o, grid, geom, q = _small_case(pkg, device, n=17)
+    pts = grid_o.points()
+    weights = torch.as_tensor(ham_o.measure.quadrature * ham_o.factor.weight).to(device)
+    problem = KohnShamProblem(grid, geom, q, [2.0])
+    rho_t, _ = problem.forward(torch.as_tensor(_bumps(pts, seed=1)).to(device))
+    v = torch.zeros(grid.n_points, dtype=torch.float64, device=device, requires_grad=True)
+    loss = density_loss(KohnShamDensity.apply(v, problem), rho_t, weights)
+    loss.backward()
+    grad = v.grad.detach().cpu().numpy()
+    scale = np.abs(grad).max()
+    inner = np.nonzero(np.linalg.norm(pts, axis=1) < 2.5)[0]
+    picks = np.random.default_rng(7).choice(inner, size=64, replace=False)
+    step = 1e-3
+    worst = 0.0
+    for p in picks:
+        d = torch.zeros(grid.n_points, dtype=torch.float64, device=device)
+        d[int(p)] = 1.0
+        lp = density_loss(problem.forward(step * d)[0], rho_t, weights)
+        lm = density_loss(problem.forward(-step * d)[0], rho_t, weights)
+        fd = float(lp - lm) / (2 * step)
+        worst = max(worst, abs(fd - grad[p]))
+    print(f"pointwise FD: worst |fd - grad| = {worst:.3e}, max |grad| = {scale:.3e}", flush=True)
+    assert worst <= 1e-6 * scale
