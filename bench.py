@@ -306,6 +306,15 @@ def batched_scan(device, count=64, seed=0, spacing=0.25):
     torch.cuda.synchronize()
     t_solo = time.perf_counter() - t2
     diffs = [abs(b["eigenvalues"][0] - s["eigenvalues"][0]) for b, s in zip(batched, solo)]
+    # the same batch with float32 filters while every active grid's residual is above 1e-5
+    BatchedCheFSI(hams, nums[0].eigen, n_states=1, filter_dtype=torch.float32).solve()  # warm-up (capture)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    msolver = BatchedCheFSI(hams, nums[0].eigen, n_states=1, filter_dtype=torch.float32)
+    mixed = msolver.solve()
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t3
+    mixed_diff = max(abs(m["eigenvalues"][0] - b["eigenvalues"][0]) for m, b in zip(mixed, batched))
     shapes = [h.grid.shape for h in hams]
     return {"workload": f"{count} bare H2+ geometries, R = 0.8 + 5.2 U(0,1) (seed {seed}), box (R+14, 14, 14) at h={spacing}, "
                         "float64, CheFSI k=5 degree 20 from the reference's seeded start",
@@ -320,7 +329,13 @@ def batched_scan(device, count=64, seed=0, spacing=0.25):
             "max_abs_eigenvalue_diff_vs_sequential": max(diffs),
             "iterations_equal": all(b["n_iterations"] == s["n_iterations"] for b, s in zip(batched, solo)),
             "eigenvalue_range": [float(min(b["eigenvalues"][0] for b in batched)),
-                                 float(max(b["eigenvalues"][0] for b in batched))]}
+                                 float(max(b["eigenvalues"][0] for b in batched))],
+            "mixed": {"batched_solve_s": t_mixed, "speedup_vs_float64_batch": t_batch / t_mixed,
+                      "lockstep_iterations": msolver.iterations,
+                      "f32_passes": msolver.filter_precisions.count("f32"),
+                      "f64_passes": msolver.filter_precisions.count("f64"),
+                      "all_converged": all(m["converged"] for m in mixed),
+                      "max_abs_eigenvalue_diff_vs_float64_batch": mixed_diff}}
 
 
 def grid_points(grid):

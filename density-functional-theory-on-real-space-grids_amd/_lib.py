@@ -34,6 +34,9 @@ SIGNATURES: dict[str, list] = {
     "cdft_batch_plan_f64": [I, P, I, I, P, P, P, L, P],
     "cdft_cheb_filter_batched_f64": [P, I, P, I, I, P],
     "cdft_hdiv_apply_batched_f64": [P, I, P, I, P],
+    "cdft_batch_plan_f32": [I, P, I, I, P, P, P, L, P],
+    "cdft_cheb_filter_batched_f32": [P, I, P, I, I, P],
+    "cdft_hdiv_apply_batched_f32": [P, I, P, I, P],
     "cdft_hdiv_set_segment": [I],
     "cdft_hdiv_launch_shape": [I, I, I, I, I, P],
     "cdft_wgram_f64": [P, I, P, I, P, L, D, I, P, L, P, P],

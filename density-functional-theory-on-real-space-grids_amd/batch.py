@@ -43,15 +43,18 @@ class BatchPlan:
         if n < 1:
             raise ValueError("a batch needs at least one grid")
         order = grids[0][0].order
+        dtype = grids[0][0].dtype
+        if dtype not in (torch.float64, torch.float32):
+            raise ValueError("a batch runs in float64 or float32")
         device = grids[0][1].device
         shapes = (ctypes.c_int * (3 * n))()
         coefs = (ctypes.c_double * (order * n))()
         ptrs = (ctypes.c_void_p * (10 * n))()
         for i, (geom, x, out, w1, w2, table) in enumerate(grids):
-            if geom.order != order or geom.dtype != torch.float64:
-                raise ValueError("a batch needs one stencil order and float64 geometry")
+            if geom.order != order or geom.dtype != dtype:
+                raise ValueError("a batch needs one stencil order and one geometry dtype")
             for t, name in ((x, "x"), (out, "out"), (w1, "work1"), (w2, "work2")):
-                ops.require(t, (k, geom.n_points), torch.float64, device, name)
+                ops.require(t, (k, geom.n_points), dtype, device, name)
             ops.require(table, (table.shape[0], 3), torch.float64, device, "table")
             shapes[3 * i: 3 * i + 3] = list(geom.shape)
             for j in range(order):
@@ -62,7 +65,8 @@ class BatchPlan:
         nbytes = int(_lib.load().cdft_batch_desc_bytes(n))
         host = (ctypes.c_uint8 * nbytes)()
         launch = (ctypes.c_int * 3)()
-        _lib.call("cdft_batch_plan_f64", n, ctypes.addressof(shapes), k, order, ctypes.addressof(coefs),
+        self.suffix = "f64" if dtype == torch.float64 else "f32"
+        _lib.call(f"cdft_batch_plan_{self.suffix}", n, ctypes.addressof(shapes), k, order, ctypes.addressof(coefs),
                   ctypes.addressof(ptrs), ctypes.addressof(host), nbytes, ctypes.addressof(launch))
         self.desc = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
         self.launch = launch
@@ -73,12 +77,12 @@ class BatchPlan:
         self._keep = grids
 
     def filter(self, degree: int, stream_tensor: torch.Tensor) -> None:
-        _lib.call("cdft_cheb_filter_batched_f64", self.desc.data_ptr(), self.n, ctypes.addressof(self.launch),
-                  self.order, int(degree), stream_of(stream_tensor))
+        _lib.call(f"cdft_cheb_filter_batched_{self.suffix}", self.desc.data_ptr(), self.n,
+                  ctypes.addressof(self.launch), self.order, int(degree), stream_of(stream_tensor))
 
     def apply(self, stream_tensor: torch.Tensor) -> None:
-        _lib.call("cdft_hdiv_apply_batched_f64", self.desc.data_ptr(), self.n, ctypes.addressof(self.launch),
-                  self.order, stream_of(stream_tensor))
+        _lib.call(f"cdft_hdiv_apply_batched_{self.suffix}", self.desc.data_ptr(), self.n,
+                  ctypes.addressof(self.launch), self.order, stream_of(stream_tensor))
 
 
 def _seeded_start(seed: int, k: int, n: int) -> tuple[torch.Tensor, torch.Tensor]:
@@ -107,9 +111,18 @@ class _Instance:
 class BatchedCheFSI:
     """Lockstep CheFSI for the lowest states of many independent operators (one per grid)."""
 
+    #: Mixed precision (as the single-grid solver): float32 filters while every active grid's worst
+    #: residual is above this and no float32 pass has stalled, float64 after that.
+    switch_residual: float = 1.0e-5
+    stall_ratio: float = 0.7
+    fp64_tail: int = 1
+
     def __init__(self, hamiltonians: list, config: EigenConfig, n_states: int = 1, ritz_stop: bool = True,
-                 seed: int = 0, use_graphs: bool = True, autotune: bool = True):
+                 seed: int = 0, use_graphs: bool = True, autotune: bool = True, filter_dtype=torch.float64):
         self.h = hamiltonians
+        self.filter_dtype = filter_dtype
+        self.mixed = filter_dtype != torch.float64
+        self.filter_precisions: list[str] = []
         self.config = config
         self.n_states = n_states
         self.ritz_stop = ritz_stop
@@ -120,9 +133,16 @@ class BatchedCheFSI:
         self.autotune = autotune
         self.tuning: dict | None = None
 
-    def _iteration_body(self, insts, plan_filter, plan_rr, plan_res, degree):
+    def _iteration_body(self, insts, plan_filter, plan_rr, plan_res, degree, use32=False):
         cfg = self.config
-        plan_filter.filter(degree, insts[0].work.vectors)
+        if use32:  # float32 filter of the float64 Ritz vectors, back to float64 for Rayleigh-Ritz
+            for it in insts:
+                it.work.x32.copy_(it.work.vectors)
+            plan_filter.filter(degree, insts[0].work.vectors)
+            for it in insts:
+                it.work.filtered.copy_(it.work.y32)
+        else:
+            plan_filter.filter(degree, insts[0].work.vectors)
         for it in insts:  # orthonormalise and rotate into the basis
             w, m = it.work, it.h.measure
             m.gram(w.filtered, out=w.S)
@@ -156,7 +176,7 @@ class BatchedCheFSI:
         tables_host = torch.zeros((n, degree, 3), dtype=torch.float64).pin_memory()
         insts = []
         for i, h in enumerate(self.h):
-            work = RitzWork(k, h.grid.n_points, degree, dev, mixed=False)
+            work = RitzWork(k, h.grid.n_points, degree, dev, mixed=self.mixed)
             work.scalars = scal[i]
             work.flag = flags[i: i + 1]
             work.table = tables[i]
@@ -217,8 +237,18 @@ class BatchedCheFSI:
             ops.hdiv_set_segment(0)
         plan_rr = BatchPlan([(g, w.basis, w.applied, w.work1, w.work2, w.table) for g, w in geoms], k)
         plan_res = BatchPlan([(g, w.vectors, w.applied, w.work1, w.work2, w.table) for g, w in geoms], k)
+        plan32 = None
+        if self.mixed:
+            ops.hdiv_set_segment(segment)
+            try:
+                plan32 = BatchPlan([(it.h.geometry_in(self.filter_dtype), it.work.x32, it.work.y32, it.work.a32,
+                                     it.work.b32, it.work.table) for it in insts], k)
+            finally:
+                ops.hdiv_set_segment(0)
         self.blocks_per_launch = plan_filter.blocks
-        graph = None
+        graphs: dict = {}
+        stalled32 = False
+        worst_before = max(it.worst for it in insts)
         self.timings["plans_s"] = time.perf_counter() - t0 - self.timings["start_s"]
         weak_form = self.ritz_stop
         eigenvalue_tol = 0.1 * cfg.cross_check_tol
@@ -230,20 +260,34 @@ class BatchedCheFSI:
                 lower = min(max(it.next_lower, it.smin + 1e-8), it.smax - 1e-8)
                 tables_host[i] = torch.as_tensor(chebyshev_coefficients(degree, lower, it.smax, it.smin))
             tables.copy_(tables_host, non_blocking=True)
+            active_worst = max(it.worst for it in insts if not it.done)
+            use32 = (self.mixed and not stalled32 and active_worst > self.switch_residual
+                     and iteration <= cfg.max_iterations - self.fp64_tail)
+            self.filter_precisions.append("f32" if use32 else "f64")
+            body_filter = plan32 if use32 else plan_filter
             if self.use_graphs:
+                graph = graphs.get(use32)
                 if graph is None:
                     graph = torch.cuda.CUDAGraph()
                     with torch.cuda.graph(graph):
-                        self._iteration_body(insts, plan_filter, plan_rr, plan_res, degree)
+                        self._iteration_body(insts, body_filter, plan_rr, plan_res, degree, use32)
+                    graphs[use32] = graph
                     self.graph_record["captures"] += 1
                 graph.replay()
                 self.graph_record["replays"] += 1
             else:
-                self._iteration_body(insts, plan_filter, plan_rr, plan_res, degree)
+                self._iteration_body(insts, body_filter, plan_rr, plan_res, degree, use32)
             fl = flags.to("cpu", non_blocking=True)
             host = scal.cpu().numpy()
             if int(fl.max()) != 0:
                 raise RuntimeError("Rayleigh-Ritz: a filtered block's Gram matrix is not positive definite")
+            if use32:
+                # a float32 pass that no longer shrinks the worst active residual has reached the
+                # float32 floor: the rest of the solve filters in float64
+                now = max(float(host[i, k:k + self.n_states].max()) for i, it in enumerate(insts) if not it.done)
+                if now > self.stall_ratio * worst_before:
+                    stalled32 = True
+                worst_before = now
             for i, it in enumerate(insts):
                 if it.done:
                     continue
@@ -255,10 +299,11 @@ class BatchedCheFSI:
                 it.next_lower = float(evals[k - 1])
                 it.previous = evals
                 stop = None
-                if it.worst < cfg.residual_tol:
+                if it.worst < cfg.residual_tol and not use32:
                     stop = "residual below residual_tol"
                 else:
-                    it.stable = it.stable + 1 if (weak_form and drift < eigenvalue_tol) else 0
+                    # the weak-form stop counts float64 passes only (float32 Ritz values jitter)
+                    it.stable = it.stable + 1 if (weak_form and drift < eigenvalue_tol and not use32) else 0
                     if it.stable >= 2:
                         stop = "Ritz values stable on the weak-form path"
                 if stop is not None:

@@ -665,23 +665,30 @@ static int check_batch_launch(const void* desc, int n_grids, const int* launch, 
 
 }  // namespace cdft
 
+static_assert(sizeof(BatchGrid<double>) == sizeof(BatchGrid<float>), "one descriptor size for both precisions");
+
 extern "C" long cdft_batch_desc_bytes(int n_grids) {
   return n_grids < 1 ? 0L : (long)n_grids * (long)sizeof(BatchGrid<double>);
 }
 
-extern "C" int cdft_batch_plan_f64(int n_grids, const int* shapes, int k, int order, const double* coefs,
-                                   const void* const* ptrs, void* desc, long desc_bytes, int* launch) {
+namespace cdft {
+
+// The descriptor table of a batch (the element type only changes the pointer types; the layout is
+// the same for float and double).
+template <typename T>
+static int batch_plan(int n_grids, const int* shapes, int k, int order, const double* coefs,
+                      const void* const* ptrs, void* desc, long desc_bytes, int* launch) {
   if (n_grids < 1 || !shapes || !coefs || !ptrs || !desc || !launch) return fail(CDFT_E_ARG, "batch_plan: null argument");
   if (desc_bytes < cdft_batch_desc_bytes(n_grids)) return fail(CDFT_E_ARG, "batch_plan: descriptor buffer too small");
   if (order != 2 && order != 4 && order != 6 && order != 8) return fail(CDFT_E_ARG, "batch_plan: bad order %d", order);
   if (k < 1 || k > 65535) return fail(CDFT_E_ARG, "batch_plan: bad block width %d", k);
-  BatchGrid<double>* g = (BatchGrid<double>*)desc;
+  BatchGrid<T>* g = (BatchGrid<T>*)desc;
   unsigned long start = 0;
   int threads = 64, tz = 0;
   for (int i = 0; i < n_grids; ++i) {
     const int n0 = shapes[3 * i], n1 = shapes[3 * i + 1], n2 = shapes[3 * i + 2];
     if (n0 < 1 || n1 < 1 || n2 < 1) return fail(CDFT_E_ARG, "batch_plan: grid %d has a bad shape", i);
-    if ((long)(n0 + order) * (n1 + order) * (n2 + order) * 8L >= 0x7fffffffL)
+    if ((long)(n0 + order) * (n1 + order) * (n2 + order) * (long)sizeof(T) >= 0x7fffffffL)
       return fail(CDFT_E_ARG, "batch_plan: grid %d too large for 32-bit offsets", i);
     const March d = march_dims(n0, n1, n2, k, order - 1);
     if (d.threads > NTMAX) return fail(CDFT_E_ARG, "batch_plan: grid %d needs %d threads", i, d.threads);
@@ -691,15 +698,15 @@ extern "C" int cdft_batch_plan_f64(int n_grids, const int* shapes, int k, int or
     const void* const* p = ptrs + 10 * i;
     for (int j = 0; j < 10; ++j)
       if (!p[j]) return fail(CDFT_E_ARG, "batch_plan: grid %d pointer %d is null", i, j);
-    g[i].x = (const double*)p[0];
-    g[i].out = (double*)p[1];
-    g[i].work1 = (double*)p[2];
-    g[i].work2 = (double*)p[3];
-    g[i].w0 = (const double*)p[4];
-    g[i].w1 = (const double*)p[5];
-    g[i].w2 = (const double*)p[6];
-    g[i].invw = (const double*)p[7];
-    g[i].pot = (const double*)p[8];
+    g[i].x = (const T*)p[0];
+    g[i].out = (T*)p[1];
+    g[i].work1 = (T*)p[2];
+    g[i].work2 = (T*)p[3];
+    g[i].w0 = (const T*)p[4];
+    g[i].w1 = (const T*)p[5];
+    g[i].w2 = (const T*)p[6];
+    g[i].invw = (const T*)p[7];
+    g[i].pot = (const T*)p[8];
     g[i].table = (const double*)p[9];
     for (int j = 0; j < 8; ++j) g[i].taps[j] = j < order ? coefs[order * i + j] : 0.0;
     g[i].n0 = n0;
@@ -716,6 +723,18 @@ extern "C" int cdft_batch_plan_f64(int n_grids, const int* shapes, int k, int or
   launch[1] = threads;
   launch[2] = tz;
   return CDFT_OK;
+}
+
+}  // namespace cdft
+
+extern "C" int cdft_batch_plan_f64(int n_grids, const int* shapes, int k, int order, const double* coefs,
+                                   const void* const* ptrs, void* desc, long desc_bytes, int* launch) {
+  return batch_plan<double>(n_grids, shapes, k, order, coefs, ptrs, desc, desc_bytes, launch);
+}
+
+extern "C" int cdft_batch_plan_f32(int n_grids, const int* shapes, int k, int order, const double* coefs,
+                                   const void* const* ptrs, void* desc, long desc_bytes, int* launch) {
+  return batch_plan<float>(n_grids, shapes, k, order, coefs, ptrs, desc, desc_bytes, launch);
 }
 
 extern "C" int cdft_cheb_filter_batched_f64(const void* desc, int n_grids, const int* launch, int order, int degree,
@@ -735,4 +754,23 @@ extern "C" int cdft_hdiv_apply_batched_f64(const void* desc, int n_grids, const 
   int rc = check_batch_launch(desc, n_grids, launch, order);
   if (rc) return rc;
   return launch_batched<double, 0>(desc, n_grids, launch, order, 0, 1, (hipStream_t)stream);
+}
+
+extern "C" int cdft_cheb_filter_batched_f32(const void* desc, int n_grids, const int* launch, int order, int degree,
+                                            void* stream) {
+  int rc = check_batch_launch(desc, n_grids, launch, order);
+  if (rc) return rc;
+  if (degree < 1) return fail(CDFT_E_ARG, "cheb_filter_batched: degree must be >= 1");
+  for (int s = 0; s < degree; ++s) {
+    rc = s == 0 ? launch_batched<float, 1>(desc, n_grids, launch, order, s, degree, (hipStream_t)stream)
+                : launch_batched<float, 2>(desc, n_grids, launch, order, s, degree, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return CDFT_OK;
+}
+
+extern "C" int cdft_hdiv_apply_batched_f32(const void* desc, int n_grids, const int* launch, int order, void* stream) {
+  int rc = check_batch_launch(desc, n_grids, launch, order);
+  if (rc) return rc;
+  return launch_batched<float, 0>(desc, n_grids, launch, order, 0, 1, (hipStream_t)stream);
 }

@@ -96,6 +96,14 @@ int cdft_cheb_filter_batched_f64(const void* desc, int n_grids, const int* launc
                                  void* stream);
 /* K1 for every grid of a batch: out = A x. */
 int cdft_hdiv_apply_batched_f64(const void* desc, int n_grids, const int* launch, int order, void* stream);
+/* The float32 batch: the same descriptor layout and size, planned by cdft_batch_plan_f32 over float32
+ * blocks and geometry (the coefficients and Chebyshev tables stay float64); its filter and apply are
+ * the float32 single-grid kernels' steps, bitwise. */
+int cdft_batch_plan_f32(int n_grids, const int* shapes, int k, int order, const double* coefs,
+                        const void* const* ptrs, void* desc, long desc_bytes, int* launch);
+int cdft_cheb_filter_batched_f32(const void* desc, int n_grids, const int* launch, int order, int degree,
+                                 void* stream);
+int cdft_hdiv_apply_batched_f32(const void* desc, int n_grids, const int* launch, int order, void* stream);
 
 /* ---- K3 / K4: float64 reductions and rotations of Rayleigh-Ritz (deterministic, no atomics) ---- */
 

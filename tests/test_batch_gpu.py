@@ -67,3 +67,50 @@ def test_batched_chefsi_matches_single_solves(pkg, device):
     order = np.argsort([r for r, _, _ in scan])
     evs = np.array([b["eigenvalues"][0] for b in batched])[order]
     assert np.all(np.diff(evs) > 0)
+
+
+def test_float32_batched_filter_matches_single_grid(pkg, device):
+    """The float32 batch runs the float32 single-grid filter's steps, bitwise."""
+    from cdft_amd.batch import BatchPlan, bond_scan_geometries
+    from cdft_amd.chefsi import chebyshev_coefficients
+    from cdft_amd.contract import Numerics
+    from cdft_amd.hamiltonian import CuspFactoredHamiltonian
+    from cdft_amd.scf import build_geometry
+
+    rows = []
+    gen = torch.Generator().manual_seed(1)
+    k, degree = 5, 20
+    for r, sc, box in bond_scan_geometries(count=3, seed=4, spacing=0.5):
+        grid, factor, q = build_geometry(sc, Numerics(box_lengths=box, spacing=0.5), device)
+        geom = CuspFactoredHamiltonian(grid, factor, q).geometry_in(torch.float32)
+        x = (torch.rand((k, grid.n_points), generator=gen, dtype=torch.float32) * 2 - 1).to(device)
+        table = torch.tensor(chebyshev_coefficients(degree, -0.3, 30.0, -1.2), dtype=torch.float64, device=device)
+        rows.append((geom, x, torch.empty_like(x), torch.empty_like(x), torch.empty_like(x), table))
+    plan = BatchPlan(rows, k)
+    plan.filter(degree, rows[0][1])
+    for geom, x, out, _, _, table in rows:
+        assert torch.equal(out, pkg.ops.cheb_filter(x, geom, table, degree))
+    plan.apply(rows[0][1])
+    for geom, x, out, _, _, _ in rows:
+        assert torch.equal(out, pkg.ops.hdiv_apply(x, geom))
+
+
+def test_mixed_batched_chefsi_matches_float64_batch(pkg, device):
+    """Mixed precision in the lockstep batch: float32 filters while every active grid's residual is
+    above 1e-5, float64 after; the eigenvalues agree with the float64 batch to 1e-12 relative."""
+    from cdft_amd.batch import BatchedCheFSI, bond_scan_geometries
+    from cdft_amd.contract import Numerics
+    from cdft_amd.eigen import EigenConfig
+    from cdft_amd.hamiltonian import CuspFactoredHamiltonian
+    from cdft_amd.scf import build_geometry
+
+    hams = []
+    for r, sc, box in bond_scan_geometries(count=4, seed=0, spacing=0.5):
+        hams.append(CuspFactoredHamiltonian(*build_geometry(sc, Numerics(box_lengths=box, spacing=0.5), device)))
+    f64 = BatchedCheFSI(hams, EigenConfig(), n_states=1).solve()
+    solver = BatchedCheFSI(hams, EigenConfig(), n_states=1, filter_dtype=torch.float32)
+    mixed = solver.solve()
+    assert "f32" in solver.filter_precisions and solver.filter_precisions[-1] == "f64"
+    for a, b in zip(mixed, f64):
+        assert a["converged"]
+        assert abs(a["eigenvalues"][0] - b["eigenvalues"][0]) <= 1e-12 * abs(b["eigenvalues"][0])
