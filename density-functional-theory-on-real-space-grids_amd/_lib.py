@@ -46,6 +46,7 @@ SIGNATURES: dict[str, list] = {
     "cdft_chol_inv_f64": [P, I, P, P, P],
     "cdft_eigh_f64": [P, I, P, P, P],
     "cdft_orth_error_f64": [P, I, P, P],
+    "cdft_cheb_table_f64": [P, I, P, I, P, P],
     "cdft_cusp_grid_fields_f64": [I, P, P, P, D, P, D, P, P, D, P, P, P, P, P],
     "cdft_face_weights_f64": [I, P, P, P, D, P, I, P, P, P, P, P],
     "cdft_node_fields_f64": [P, P, P, L, I, P, P, P, D, P, P, P, P, P, P],

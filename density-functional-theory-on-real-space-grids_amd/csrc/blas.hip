@@ -446,6 +446,35 @@ __global__ void eigh_small_kernel(const double* __restrict__ m_in, double* __res
   }
 }
 
+// The scaled Chebyshev rows of a CheFSI filter (reference src/cdft/eigen/chefsi.py:116-136) built on
+// the device: lower = evals[index] clamped into (smin + 1e-8, smax - 1e-8), upper = smax, smin =
+// bounds[0], smax = bounds[1]; table row s = (c, a_s, b_s). The operations are those of the host
+// routine in the same order, uncontracted, so the table is bitwise the host's.
+__global__ void cheb_table_kernel(const double* __restrict__ evals, int index, const double* __restrict__ bounds,
+                                  int degree, double* __restrict__ table) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double smin = bounds[0], upper = bounds[1];
+  double lower = evals[index];
+  const double lo = smin + 1e-8, hi = upper - 1e-8;
+  lower = lower >= lo ? lower : lo;
+  lower = lower <= hi ? lower : hi;
+  const double e = 0.5 * (upper - lower);
+  const double c = 0.5 * (upper + lower);
+  double sigma = e / (smin - c);
+  const double sigma1 = sigma;
+  table[0] = c;
+  table[1] = sigma1 / e;
+  table[2] = 0.0;
+  for (int s = 1; s < degree; ++s) {
+    const double sigma2 = 1.0 / (2.0 / sigma1 - sigma);
+    table[3 * s] = c;
+    table[3 * s + 1] = 2.0 * sigma2 / e;
+    table[3 * s + 2] = sigma * sigma2;
+    sigma = sigma2;
+  }
+}
+
 // max |M - I| for a k x k matrix (orthonormality error), written to out[0].
 __global__ void max_abs_minus_identity_kernel(const double* __restrict__ m, int k, double* __restrict__ out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -590,6 +619,13 @@ extern "C" int cdft_eigh_f64(const double* m, int k, double* evals, double* c, v
     default: hipLaunchKernelGGL(eigh_kernel, dim3(1), dim3(64), 0, st, m, k, evals, c);
   }
   return check_launch("eigh");
+}
+
+extern "C" int cdft_cheb_table_f64(const double* evals, int index, const double* bounds, int degree, double* table,
+                                   void* stream) {
+  if (!evals || !bounds || !table || index < 0 || degree < 1) return fail(CDFT_E_ARG, "cheb_table: bad arguments");
+  hipLaunchKernelGGL(cheb_table_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, evals, index, bounds, degree, table);
+  return check_launch("cheb_table");
 }
 
 extern "C" int cdft_orth_error_f64(const double* m, int k, double* out, void* stream) {

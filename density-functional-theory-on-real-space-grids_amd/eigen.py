@@ -20,7 +20,6 @@ import numpy as np
 import torch
 
 from . import ops
-from .chefsi import chebyshev_coefficients
 from .hamiltonian import lanczos_enqueue, lanczos_finish, seeded_randn
 
 BOUND_REUSE_RITZ_FRACTION = 0.9
@@ -82,7 +81,11 @@ class RitzWork:
         self.work1 = torch.empty((k, n), **f)
         self.work2 = torch.empty((k, n), **f)
         self.table = torch.zeros((degree, 3), **f)
-        self.table_host = torch.zeros((degree, 3), dtype=torch.float64).pin_memory()
+        # [spectrum_min, upper] of the filter interval: the device builds the table from these and the
+        # last Ritz value, so only a change of bounds crosses the bus
+        self.bounds = torch.zeros(2, **f)
+        self.bounds_host = torch.zeros(2, dtype=torch.float64).pin_memory()
+        self.bounds_set: tuple | None = None
         if mixed:
             g = dict(dtype=torch.float32, device=device)
             self.x32 = torch.empty((k, n), **g)
@@ -123,14 +126,16 @@ def residual_norms(geometry, measure, work: RitzWork) -> None:
 
 
 def _read(work: RitzWork) -> tuple[np.ndarray, np.ndarray, float]:
-    flag = work.flag.to("cpu", non_blocking=True)
-    host = work.scalars.cpu().numpy()  # the iteration's one synchronising read (the flag rides along)
-    if int(flag[0]) != 0:
+    host = work.scalars.cpu().numpy()  # the iteration's one synchronising read
+    k = work.k
+    orth = float(host[2 * k])
+    # A failed Cholesky zeroes its inverse, so the Ritz vectors vanish and max|G - I| is 1: only then
+    # is the flag worth a second read.
+    if not orth < 0.5 and int(work.flag.cpu()[0]) != 0:
         raise RuntimeError(
             "Rayleigh-Ritz: the filtered block's Gram matrix is not positive definite (the block has "
             "collapsed in rank); the eigh fallback of the reference that drops vectors is not implemented")
-    k = work.k
-    return host[:k].copy(), host[k:2 * k].copy(), float(host[2 * k])
+    return host[:k].copy(), host[k:2 * k].copy(), orth
 
 
 def _fraction(value, low, high):
@@ -210,6 +215,8 @@ class ChebyshevFilteredSubspace:
     def _ritz(self, work: RitzWork) -> None:
         rayleigh_ritz(self._geom64, self._measure, work.filtered, work)
         residual_norms(self._geom64, self._measure, work)
+        # the next filter's table, from this pass's last Ritz value and the current bounds
+        ops.cheb_table(work.evals, work.k - 1, work.bounds, self.config.chebyshev_degree, work.table)
 
     def _replay(self, key: tuple, body) -> None:
         """Run ``body`` through its captured graph (captured on first use), or eagerly when capture
@@ -261,11 +268,17 @@ class ChebyshevFilteredSubspace:
         hamiltonian.note_applications(steps)
         return lanczos_finish(scal.cpu().numpy(), steps, cfg.bound_safety)
 
-    def _upload_table(self, work: RitzWork, lower, upper, smin) -> int:
-        rows = chebyshev_coefficients(self.config.chebyshev_degree, lower, upper, smin)
-        work.table_host.numpy()[:] = np.asarray(rows)
-        work.table.copy_(work.table_host, non_blocking=True)
-        return len(rows)
+    def _set_bounds(self, work: RitzWork, smin: float, smax: float) -> int:
+        """Point the filter at ``[smin, smax]``. The table itself is built on the device from the
+        last Ritz value (at the end of every Rayleigh-Ritz graph); a change of bounds uploads them
+        and rebuilds the table at once. Returns the filter degree."""
+        degree = self.config.chebyshev_degree
+        if work.bounds_set != (smin, smax):
+            work.bounds_set = (smin, smax)
+            work.bounds_host[0], work.bounds_host[1] = smin, smax
+            work.bounds.copy_(work.bounds_host, non_blocking=True)
+            ops.cheb_table(work.evals, work.k - 1, work.bounds, degree, work.table)
+        return degree
 
     # ---- the solver ----
 
@@ -301,6 +314,7 @@ class ChebyshevFilteredSubspace:
         rayleigh_ritz(self._geom64, self._measure, block, work)
         residual_norms(self._geom64, self._measure, work)
         hamiltonian.note_applications(2)
+        work.bounds_set = None  # new Ritz values: the first pass rebuilds its table
         evals, norms, _ = _read(work)
         worst = float(norms[:n_states].max())
         stalled32 = False
@@ -326,7 +340,6 @@ class ChebyshevFilteredSubspace:
                 source = f"lanczos (reuse rejected at iteration {iteration})"
                 fraction = _fraction(lower, spectrum_min, spectrum_max)
             top_fraction = max(top_fraction, fraction)
-            lower = min(max(lower, spectrum_min + 1e-8), spectrum_max - 1e-8)
             if precision == "f32":
                 use32 = self.mixed
             elif precision == "f64":
@@ -334,7 +347,9 @@ class ChebyshevFilteredSubspace:
             else:
                 use32 = (self.mixed and not stalled32 and worst > self.switch_residual
                          and iteration <= cfg.max_iterations - self.fp64_tail)
-            degree = self._upload_table(work, lower, spectrum_max, spectrum_min)
+            # the device-built table takes the last Ritz value (clamped into the bounds) as its lower
+            # edge; the host copy of it above decides only the bound reuse
+            degree = self._set_bounds(work, spectrum_min, spectrum_max)
             self.filter_precisions.append("f32" if use32 else "f64")
             self._run_iteration(work, use32, degree)
             hamiltonian.note_applications(degree + 2)

@@ -327,6 +327,18 @@ def atom_combine(base: torch.Tensor, c: torch.Tensor, fields: torch.Tensor, sign
     return out
 
 
+def cheb_table(evals: torch.Tensor, index: int, bounds: torch.Tensor, degree: int, table: torch.Tensor) -> torch.Tensor:
+    """The filter's ``(degree, 3)`` Chebyshev rows on the device from ``evals[index]`` (the lower
+    edge, clamped into the bounds) and ``bounds = [spectrum_min, upper]`` (bitwise the host rows)."""
+    require(table, (degree, 3), torch.float64, evals.device, "table")
+    require(bounds, (2,), torch.float64, evals.device, "bounds")
+    if evals.dtype != torch.float64 or not 0 <= index < evals.numel():
+        raise ValueError("cheb_table: evals must be float64 and hold the index")
+    _lib.call("cdft_cheb_table_f64", evals.data_ptr(), int(index), bounds.data_ptr(), int(degree), table.data_ptr(),
+              stream_of(evals))
+    return table
+
+
 def pulay_coefficients(gram: torch.Tensor, slots, alpha: float, coef: torch.Tensor) -> torch.Tensor:
     """Device Pulay/DIIS coefficients from the history Gram matrix ``gram`` (``(H, H)``) for the
     active ``slots`` (oldest first): ``coef[2i] = c_i``, ``coef[2i+1] = alpha c_i``."""

@@ -146,6 +146,12 @@ int cdft_eigh_f64(const double* m, int k, double* evals, double* c, void* stream
 
 /* out[0] = max |M - I| over a k x k matrix (src/cdft/eigen/rayleigh_ritz.py:44-48). */
 int cdft_orth_error_f64(const double* m, int k, double* out, void* stream);
+/* The (degree, 3) scaled-Chebyshev table of a filter built on the device, bitwise the host rows of
+ * chebyshev_coefficients (src/cdft/eigen/chefsi.py:116-136): lower = evals[index] clamped into
+ * (bounds[0] + 1e-8, bounds[1] - 1e-8), upper = bounds[1], spectrum_min = bounds[0] (device arrays). A
+ * filter graph that starts with it needs no per-pass table upload. */
+int cdft_cheb_table_f64(const double* evals, int index, const double* bounds, int degree, double* table,
+                        void* stream);
 
 /* ---- stencil launch control (no reference counterpart: tuning of the K1/K2 launch) ---- */
 

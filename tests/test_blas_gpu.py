@@ -89,3 +89,26 @@ def test_cholesky_flags_indefinite(pkg, device):
     flag = torch.zeros(1, dtype=torch.int32, device=device)
     pkg.ops.chol_inv(s, t, flag)
     assert int(flag.item()) == 1
+
+
+def test_device_chebyshev_table_is_the_host_table(pkg, device):
+    """The filter's table built on the device equals the host rows bitwise, including the clamp of
+    the lower edge into the bounds."""
+    from cdft_amd.chefsi import chebyshev_coefficients
+
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        smin = float(rng.uniform(-3.0, -0.5))
+        smax = float(rng.uniform(20.0, 400.0))
+        evals = np.sort(rng.uniform(smin - 1.0, 5.0, 5))
+        if trial == 0:
+            evals[-1] = smax + 3.0  # clamped from above
+        if trial == 1:
+            evals[-1] = smin - 2.0  # clamped from below
+        degree = int(rng.integers(1, 25))
+        table = torch.empty((degree, 3), dtype=torch.float64, device=device)
+        pkg.ops.cheb_table(torch.as_tensor(evals).to(device), 4,
+                           torch.tensor([smin, smax], dtype=torch.float64, device=device), degree, table)
+        lower = min(max(float(evals[4]), smin + 1e-8), smax - 1e-8)
+        expect = np.asarray(chebyshev_coefficients(degree, lower, smax, smin))
+        assert np.array_equal(table.cpu().numpy(), expect), trial
