@@ -311,8 +311,10 @@ class ChebyshevFilteredSubspace:
         if bounds is None:
             spectrum_min, spectrum_max = lanczos()
             source = "lanczos"
-        rayleigh_ritz(self._geom64, self._measure, block, work)
-        residual_norms(self._geom64, self._measure, work)
+        # the first Rayleigh-Ritz is the iteration's: the start block goes where a filter leaves its
+        # output, and the captured Rayleigh-Ritz graph runs on it
+        work.filtered.copy_(block)
+        self._replay(("ritz", work.k), lambda: self._ritz(work))
         hamiltonian.note_applications(2)
         work.bounds_set = None  # new Ritz values: the first pass rebuilds its table
         evals, norms, _ = _read(work)
