@@ -327,11 +327,9 @@ class CuspQuadrature:
         if (base_host - 3 < 0).any() or (base_host + 4 >= dims).any():
             raise ValueError("an interpolation stencil reaches outside the grid; the sphere radius bound is wrong")
         self._base_host = base_host
-        lo = base_host.min(axis=0)
-        ext = base_host.max(axis=0) - lo + 1
-        cell = ((base_host[:, 0] - lo[0]) * ext[1] + (base_host[:, 1] - lo[1])) * ext[2] + (base_host[:, 2] - lo[2])
-        order = np.argsort(cell, kind="stable").astype(np.int32)
-        if not np.array_equal(order, np.arange(m, dtype=np.int32)):
+        lo, ext = lo_key, ext_key  # the same cells, so the same cell box
+        cell = key[perm]  # the cell index of every node, in node order
+        if np.any(cell[1:] < cell[:-1]):
             raise RuntimeError("quadrature nodes are not in base-cell order")
         counts = np.bincount(cell, minlength=int(ext.prod()))
         start = np.zeros(counts.size + 1, dtype=np.int32)
