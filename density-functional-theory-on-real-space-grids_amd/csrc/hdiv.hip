@@ -67,7 +67,9 @@ struct March {
 
 // Epilogue modes: 0 = A phi; 1 = (A y - c y) a; 2 = (A y - c y) a - b prev, where (c, a, b) is row
 // `step` of the Chebyshev coefficient table; 3 = the transpose A^T x = 0.5 sum_d D_d^T[w_d D_d
-// (invw x)] + V x, with phi = invw x and prev = x. TZ is the tile extent along z (compile-time, so the LDS
+// (invw x)] + V x, with phi = invw x and prev = x; 4 = the residual norm: nothing is stored, the
+// block accumulates sum q (A phi - e_v phi)^2 over its points (prev = q, table = the eigenvalues e)
+// and writes one partial to out[0] (the kernel points out at the block's slot). TZ is the tile extent along z (compile-time, so the LDS
 // pitches below are constants and neighbouring reads pair into ds_read2). Threads are numbered
 // row-major over the tile and every LDS row pitch is congruent to TZ modulo 16 doubles, so any 16
 // consecutive lanes touch 16 consecutive bank pairs: the plane-tile, y-face and z-face accesses are
@@ -126,7 +128,7 @@ __device__ __forceinline__ void stencil_block(
   const long vbase = (long)v * plane * n0;
   const int vbytes = plane * n0 * S;
   const auto r_phi = make_rsrc(phi + vbase, vbytes);
-  const auto r_prev = make_rsrc(MODE >= 2 ? prev + vbase : phi, vbytes);
+  const auto r_prev = make_rsrc(MODE == 4 ? prev : (MODE >= 2 ? prev + vbase : phi), vbytes);
   const auto r_out = make_rsrc(out + vbase, vbytes);
   const auto r_w0 = make_rsrc(w0, (n0 + 2 * P) * plane * S);
   const auto r_w1 = make_rsrc(w1, n0 * pl1 * S);
@@ -139,11 +141,13 @@ __device__ __forceinline__ void stencil_block(
 #pragma unroll
   for (int i = 0; i < 2 * P; ++i) c[i] = (T)taps.c[i];
   T cc = 0, ca = 0, cb = 0;
-  if (MODE >= 1) {
+  if (MODE == 1 || MODE == 2) {
     cc = (T)table[3 * step];
     ca = (T)table[3 * step + 1];
     cb = (T)table[3 * step + 2];
   }
+  const T ev = MODE == 4 ? (T)table[v] : T(0);
+  T racc = 0;  // MODE 4: this thread's sum of q r^2 over its planes
 
   // Halo slots of this thread (the same in every plane): in-plane source byte offset (NONE =
   // outside the grid, staged as zero) and LDS destination (-1 = no slot).
@@ -301,7 +305,12 @@ __device__ __forceinline__ void stencil_block(
       T val = MODE == 3 ? lap * T(0.5) + cur.pt * cur.pr : lap * T(0.5) * cur.iw + pv * cur.pt;
       if (MODE == 1) val = (val - cc * pv) * ca;
       if (MODE == 2) val = (val - cc * pv) * ca - cb * cur.pr;
-      bstore<T>(val, r_out, col, m * plane * S);
+      if (MODE == 4) {
+        const T r = val - ev * pv;
+        racc += (r * r) * cur.pr;
+      } else {
+        bstore<T>(val, r_out, col, m * plane * S);
+      }
     }
 
     // advance the x queues and the plane pipeline
@@ -312,6 +321,20 @@ __device__ __forceinline__ void stencil_block(
     }
     qphi[2 * P - 1] = cur.phn;
     cur = nxt;
+  }
+  if constexpr (MODE == 4) {
+    // fixed-order block sum: a butterfly per wavefront, then the wavefronts in order
+    T sum = racc;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    __syncthreads();  // every thread is done with the face tiles, which now hold the wave sums
+    if ((t & 63) == 0) s_yf[t >> 6] = sum;
+    __syncthreads();
+    if (t == 0) {
+      T total = s_yf[0];
+      for (int w = 1; w < (nt + 63) / 64; ++w) total += s_yf[w];
+      out[0] = total;
+    }
   }
 }
 
@@ -325,8 +348,8 @@ __global__ __launch_bounds__(NTMAX) void hdiv_kernel(
   const unsigned lin = xcd_tile(blockIdx.x, gridDim.x);
   const unsigned tile = lin / (unsigned)k;
   const int v = (int)(lin - tile * (unsigned)k);
-  stencil_block<T, P, MODE, TZ>(phi, prev, out, w0, w1, w2, invw, pot, n0, n1, n2, taps, table, step, md, tile, v,
-                                lds);
+  stencil_block<T, P, MODE, TZ>(phi, prev, MODE == 4 ? out + lin : out, w0, w1, w2, invw, pot, n0, n1, n2, taps, table,
+                                step, md, tile, v, lds);
 }
 
 // ---- batched launches: many grids (one descriptor each) in one launch ----
@@ -605,6 +628,38 @@ extern "C" int cdft_hdiv_apply_adjoint_f32(const float* x, const float* w0, cons
                                            const float* invw, const float* pot, float* out, float* work, int k,
                                            int n0, int n1, int n2, const double* coef, int order, void* stream) {
   return adjoint_impl<float>(x, w0, w1, w2, invw, pot, out, work, k, n0, n1, n2, coef, order, (hipStream_t)stream);
+}
+
+namespace cdft {
+// ||A phi_v - e_v phi_v||_q from the stencil blocks' partials: one wavefront per vector, lane l sums the
+// partials of tiles l, l+64, ... in order, then a fixed butterfly.
+__global__ __launch_bounds__(64) void resnorm_tiles_kernel(const double* __restrict__ partials, int tiles, int k,
+                                                           double* __restrict__ norms) {
+  const int v = blockIdx.x;
+  double s = 0.0;
+  for (int t = threadIdx.x; t < tiles; t += 64) s += partials[(long)t * k + v];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (threadIdx.x == 0) norms[v] = sqrt(s > 0.0 ? s : 0.0);
+}
+}  // namespace cdft
+
+extern "C" int cdft_hdiv_resnorm_f64(const double* phi, const double* w0, const double* w1, const double* w2,
+                                     const double* invw, const double* pot, const double* evals, const double* q, int k,
+                                     int n0, int n1, int n2, const double* coef, int order, double* partials,
+                                     long partials_len, double* norms, void* stream) {
+  int rc = validate(phi, partials, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order);
+  if (rc) return rc;
+  if (!evals || !q || !norms) return fail(CDFT_E_ARG, "hdiv_resnorm: null evals, q or norms");
+  const March d = march_dims(n0, n1, n2, k, order - 1);
+  const long blocks = (long)d.tiles * k;
+  if (partials_len < blocks)
+    return fail(CDFT_E_CAPACITY, "hdiv_resnorm: partials hold %ld doubles, the launch needs %ld", partials_len, blocks);
+  hipStream_t s = (hipStream_t)stream;
+  rc = launch<double, 4>(phi, q, partials, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order, evals, 0, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(resnorm_tiles_kernel, dim3(k), dim3(64), 0, s, partials, (int)d.tiles, k, norms);
+  return check_launch("resnorm_tiles");
 }
 
 extern "C" int cdft_hdiv_set_segment(int planes) {

@@ -120,9 +120,8 @@ def rayleigh_ritz(geometry, measure, block: torch.Tensor, work: RitzWork) -> Non
 
 def residual_norms(geometry, measure, work: RitzWork) -> None:
     """``||A phi_i - eps_i phi_i||`` in the measure into ``work.norms`` (a fresh apply, as the
-    reference's ``residual_norms``)."""
-    ops.hdiv_apply(work.vectors, geometry, work.applied)
-    ops.resnorm(work.applied, work.vectors, work.evals, measure.q, out=work.norms)
+    reference's ``residual_norms``; the apply keeps only the norm partials, nothing is stored)."""
+    ops.hdiv_resnorm(work.vectors, geometry, work.evals, measure.q, work.norms)
 
 
 def _read(work: RitzWork) -> tuple[np.ndarray, np.ndarray, float]:

@@ -150,6 +150,35 @@ def hdiv_launch_shape(shape, k: int, order: int = 8) -> dict:
     return dict(zip(("tile_y", "tile_z", "segment", "segments", "threads", "blocks"), list(buf)))
 
 
+_stencil_partials: dict[str, torch.Tensor] = {}
+
+
+def hdiv_resnorm(x: torch.Tensor, geom: StencilGeometry, evals: torch.Tensor, q: torch.Tensor,
+                 out: torch.Tensor) -> torch.Tensor:
+    """``out[v] = ||A x_v - evals[v] x_v||_q`` in one stencil launch plus a small combine (float64)."""
+    k = x.shape[0]
+    if geom.dtype != torch.float64:
+        raise ValueError("hdiv_resnorm: float64 only")
+    require(x, (k, geom.n_points), torch.float64, geom.device, "x")
+    require(q, (geom.n_points,), torch.float64, geom.device, "q")
+    require(out, (k,), torch.float64, geom.device, "norms")
+    if evals.dtype != torch.float64 or evals.numel() < k or not evals.is_contiguous():
+        raise ValueError("hdiv_resnorm: evals must hold k contiguous float64 values")
+    blocks = hdiv_launch_shape(geom.shape, k, geom.order)["blocks"]
+    key = str(geom.device)
+    ws = _stencil_partials.get(key)
+    if ws is None or ws.numel() < blocks:
+        if ws is not None:
+            _retired.append(ws)  # a captured graph may still address it
+        ws = torch.empty(max(blocks, 4096), dtype=torch.float64, device=geom.device)
+        _stencil_partials[key] = ws
+    n0, n1, n2 = geom.shape
+    _lib.call("cdft_hdiv_resnorm_f64", x.data_ptr(), geom.w0.data_ptr(), geom.w1.data_ptr(), geom.w2.data_ptr(),
+              geom.invw.data_ptr(), geom.potential.data_ptr(), evals.data_ptr(), q.data_ptr(), k, n0, n1, n2,
+              ctypes.addressof(geom.taps), geom.order, ws.data_ptr(), ws.numel(), out.data_ptr(), stream_of(x))
+    return out
+
+
 def cheb_step(y: torch.Tensor, prev: torch.Tensor | None, geom: StencilGeometry, table: torch.Tensor, step: int,
               out: torch.Tensor) -> torch.Tensor:
     """One fused recurrence step ``(A y - c y) a - b prev`` with ``(c, a, b) = table[step]``."""

@@ -52,6 +52,15 @@ int cdft_hdiv_apply_adjoint_f32(const float* x, const float* w0, const float* w1
                                 const float* invw, const float* pot, float* out, float* work, int k,
                                 int n0, int n1, int n2, const double* coef, int order, void* stream);
 
+/* K1 fused with the residual norms of Rayleigh-Ritz: norms[v] = ||A phi_v - evals[v] phi_v||_q
+ * (sqrt of sum_p q_p r_p^2), with nothing stored but per-block partials (partials: at least the launch's
+ * block count, cdft_hdiv_launch_shape(..)[5] doubles) combined in a fixed order. Replaces the apply and
+ * the norm of residual_norms (src/cdft/eigen/rayleigh_ritz.py:105-115). */
+int cdft_hdiv_resnorm_f64(const double* phi, const double* w0, const double* w1, const double* w2,
+                          const double* invw, const double* pot, const double* evals, const double* q, int k,
+                          int n0, int n1, int n2, const double* coef, int order, double* partials, long partials_len,
+                          double* norms, void* stream);
+
 /* K2: one fused scaled-Chebyshev step, out = (A y - c y) a - b prev with (c, a, b) = table[step]
  * read on the device (table is a (degree, 3) float64 device array), prev ignored at step 0.
  * Replaces chebyshev_update (src/cdft/operators/fused.py:391-417) and the body of

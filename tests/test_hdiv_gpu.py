@@ -109,3 +109,32 @@ def test_invalid_arguments_raise(pkg, device):
     with pytest.raises(pkg._lib.HipLibraryError):
         pkg._lib.call("cdft_hdiv_apply_f64", phi.data_ptr(), 0, 0, 0, 0, 0, phi.data_ptr(), 2, 9, 9, 9,
                       0, 8, 0)
+
+
+@pytest.mark.parametrize("order,shape,k,segment", [(8, (65, 65, 65), 5, 0), (8, (29, 25, 25), 3, 0),
+                                                   (6, (17, 9, 40), 2, 4), (8, (6, 5, 7), 5, 0),
+                                                   (4, (30, 31, 32), 4, 11)])
+def test_fused_residual_norms_match_apply_then_norm(pkg, device, order, shape, k, segment):
+    """K1 with the residual-norm epilogue (nothing stored, block partials combined in fixed order)
+    against the apply followed by the weighted norm of ``A x - e x``, with the oracle's numpy norm
+    beside them; every segment split gives the same norms to 1e-13."""
+    rng = np.random.default_rng(order + k)
+    n = int(np.prod(shape))
+    faces = [rng.uniform(0.5, 1.5, size=tuple(s + (order if d == a else 0) for d, s in enumerate(shape))) for a in range(3)]
+    geom = pkg.geometry_from_arrays(shape, order, 0.31, faces, rng.uniform(0.5, 2.0, n), rng.uniform(-1, 1, n), device)
+    x = torch.as_tensor(rng.uniform(-1, 1, size=(k, n))).to(device)
+    q = torch.as_tensor(rng.uniform(0.1, 1.5, size=n)).to(device)
+    evals = torch.as_tensor(rng.uniform(-2, 2, size=k)).to(device)
+    pkg.ops.hdiv_set_segment(segment)
+    try:
+        fused = pkg.ops.hdiv_resnorm(x, geom, evals, q, torch.empty(k, dtype=torch.float64, device=device))
+    finally:
+        pkg.ops.hdiv_set_segment(0)
+    ax = pkg.ops.hdiv_apply(x, geom)
+    two_pass = pkg.ops.resnorm(ax, x, evals, q, out=torch.empty(k, dtype=torch.float64, device=device))
+    r = ax.cpu().numpy() - evals.cpu().numpy()[:, None] * x.cpu().numpy()
+    numpy_norm = np.sqrt((r * r * q.cpu().numpy()).sum(axis=1))
+    assert np.abs(fused.cpu().numpy() - two_pass.cpu().numpy()).max() <= 1e-13 * numpy_norm.max()
+    assert np.abs(fused.cpu().numpy() - numpy_norm).max() <= 1e-13 * numpy_norm.max()
+    again = pkg.ops.hdiv_resnorm(x, geom, evals, q, torch.empty(k, dtype=torch.float64, device=device))
+    assert torch.equal(again, pkg.ops.hdiv_resnorm(x, geom, evals, q, torch.empty(k, dtype=torch.float64, device=device)))
