@@ -359,32 +359,54 @@ __global__ __launch_bounds__(64) void lift_chunks_kernel(const double* __restric
 }
 
 // Stage 2: each column's chunk slabs summed (chunks in node order) into one column slab
-// Zcol[f][col][t][iz], iz in [0, c2 + 7): one block per column.
+// Zcol[f][col][t][iz], iz in [0, c2 + 7): one thread per (column, t, iz) entry, blocks of 256 entries
+// of one column (blockIdx.y). A column's chunks start at increasing z-cells, so the chunks reaching
+// iz are a prefix of the column's chunk list (less those already past iz); they are visited four at
+// a time with all loads issued before the in-order sums.
 template <int NF>
-__global__ void lift_columns_kernel(const double* __restrict__ z, CellBox cb, const int* __restrict__ chunk_zlo,
-                                    const int* __restrict__ chunk_hz, const int* __restrict__ chunk_off,
-                                    const int* __restrict__ col_chunk, long total, double* __restrict__ zcol) {
-  const int col = blockIdx.x;
+__global__ __launch_bounds__(256) void lift_columns_kernel(const double* __restrict__ z, CellBox cb,
+                                                           const int* __restrict__ chunk_zlo,
+                                                           const int* __restrict__ chunk_hz,
+                                                           const int* __restrict__ chunk_off,
+                                                           const int* __restrict__ col_chunk, long total,
+                                                           double* __restrict__ zcol) {
+  const int col = blockIdx.y;
   const int ez = cb.c2 + 7;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= 64 * ez) return;
   const long ncols = (long)cb.c0 * cb.c1;
   const int c0 = col_chunk[col], c1 = col_chunk[col + 1];
-  for (int e = threadIdx.x; e < 64 * ez; e += blockDim.x) {
-    const int t = e / ez, iz = e - (e / ez) * ez;
-    double acc[NF];
+  const int t = e / ez, iz = e - (e / ez) * ez;
+  double acc[NF];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) acc[f] = 0.0;
-    for (int c = c0; c < c1; ++c) {
-      const int zlo = chunk_zlo[c];
-      if (zlo > iz) break;  // chunks of a column start at increasing z-cells
-      const int r = iz - zlo, hz = chunk_hz[c];
-      if (r >= hz) continue;
-      const long idx = (long)chunk_off[c] + (long)t * hz + r;
+  for (int f = 0; f < NF; ++f) acc[f] = 0.0;
+  for (int c = c0; c < c1; c += 4) {
+    double v[4][NF];
+    bool in[4];
+    bool more = true;
 #pragma unroll
-      for (int f = 0; f < NF; ++f) acc[f] += z[f * total + idx];
+    for (int u = 0; u < 4; ++u) {
+      const int cc = c + u;
+      in[u] = false;
+      if (cc < c1) {
+        const int zlo = chunk_zlo[cc];
+        if (zlo > iz) more = false;
+        const int r = iz - zlo, hz = chunk_hz[cc];
+        in[u] = zlo <= iz && r < hz;
+        const long idx = in[u] ? (long)chunk_off[cc] + (long)t * hz + r : 0L;  // in range either way
+#pragma unroll
+        for (int f = 0; f < NF; ++f) v[u][f] = in[u] ? z[f * total + idx] : 0.0;
+      }
     }
 #pragma unroll
-    for (int f = 0; f < NF; ++f) zcol[((f * ncols + col) * 64 + t) * ez + iz] = acc[f];
+    for (int u = 0; u < 4; ++u)
+      if (in[u])
+#pragma unroll
+        for (int f = 0; f < NF; ++f) acc[f] += v[u][f];
+    if (!more) break;
   }
+#pragma unroll
+  for (int f = 0; f < NF; ++f) zcol[((f * ncols + col) * 64 + t) * ez + iz] = acc[f];
 }
 
 // Stage 3: out[f][p] = add[f][p] + sum_{j0, j1} Zcol[f][col(i0 - j0 + 3, i1 - j1 + 3)][8 j0 + j1][iz]
@@ -647,6 +669,9 @@ extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* base, c
       !col_chunk || nf < 1 || nf > 4 || n_chunks < 1)
     return fail(CDFT_E_ARG, "lift: bad arguments (nf=%d, chunks=%d)", nf, n_chunks);
   const long col_doubles = (long)cell_box[3] * cell_box[4] * 64 * (cell_box[5] + 7);
+  if ((long)cell_box[3] * cell_box[4] > 65535)
+    return fail(CDFT_E_ARG, "lift: %d x %d node columns exceed one launch's second grid dimension", cell_box[3],
+                cell_box[4]);
   if (work_len < (work_per_field + col_doubles) * nf)
     return fail(CDFT_E_CAPACITY, "lift: workspace holds %ld doubles, needs %ld", work_len,
                 (work_per_field + col_doubles) * nf);
@@ -666,7 +691,8 @@ extern "C" int cdft_lift_f64(const double* x, int nf, long m, const int* base, c
                      chunk_node, chunk_zlo, chunk_hz, chunk_off, work_per_field, work);                        \
   rc = check_launch("lift_chunks");                                                                          \
   if (rc) return rc;                                                                                         \
-  hipLaunchKernelGGL(lift_columns_kernel<NFV>, dim3((unsigned)(cb.c0 * cb.c1)), dim3(256), 0, s, work, cb,     \
+  hipLaunchKernelGGL(lift_columns_kernel<NFV>, dim3(blocks_for(64L * (cb.c2 + 7), 256), (unsigned)(cb.c0 * cb.c1)), \
+                     dim3(256), 0, s, work, cb,                                                                \
                      chunk_zlo, chunk_hz, chunk_off, col_chunk, work_per_field, zcol);                        \
   rc = check_launch("lift_columns");                                                                         \
   if (rc) return rc;                                                                                         \
