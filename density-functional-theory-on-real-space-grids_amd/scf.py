@@ -178,14 +178,11 @@ class KohnShamStep:
         ops.lda_vwn(n_grid, self.e_grid, self.v_grid)
         ops.lda_vwn(self.n_node, self.e_node, self.v_node)
         # node parts W P (f2 v), grid parts far_w (f2 v): Hartree in row 0, XC in row 1
-        ops.ewise(ops.EW_MUL, self.tmp_node, q.f2, self.v_h_node)
-        ops.ewise(ops.EW_MUL, self.node_x[0], q.weights, self.tmp_node)
-        ops.ewise(ops.EW_MUL, self.tmp_node, q.f2, self.v_node)
-        ops.ewise(ops.EW_MUL, self.node_x[1], q.weights, self.tmp_node)
-        ops.ewise(ops.EW_MUL, self.tmp_grid, self.f2, self.v_h_grid)
-        ops.ewise(ops.EW_MUL, self.grid_add[0], q.far_weights, self.tmp_grid)
-        ops.ewise(ops.EW_MUL, self.tmp_grid, self.f2, self.v_grid)
-        ops.ewise(ops.EW_MUL, self.grid_add[1], q.far_weights, self.tmp_grid)
+        # (f2 v) w in one pass each: the same products as w (f2 v), so the same bits
+        ops.ewise(ops.EW_MUL3, self.node_x[0], q.f2, self.v_h_node, q.weights)
+        ops.ewise(ops.EW_MUL3, self.node_x[1], q.f2, self.v_node, q.weights)
+        ops.ewise(ops.EW_MUL3, self.grid_add[0], self.f2, self.v_h_grid, q.far_weights)
+        ops.ewise(ops.EW_MUL3, self.grid_add[1], self.f2, self.v_grid, q.far_weights)
         lumped = q.lift(self.node_x, add=self.grid_add, out=self.pot_lumped)
         hartree_weights, xc_weights = lumped[0], lumped[1]
         scalars = self.pot_scalars
