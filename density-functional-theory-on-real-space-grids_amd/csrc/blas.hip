@@ -154,6 +154,67 @@ __global__ __launch_bounds__(256) void rotate_kernel(const double* __restrict__ 
   }
 }
 
+// The rotation y = R^T x (k x k, R on the device) fused with the weighted Gram matrix of its output,
+// sum_p q_p y_ip y_jp: the rotation is rotate_kernel's (same sums), the products and block partials
+// are wgram_partial_kernel's for a Gram matrix at the same launch (same points per thread, same
+// order), so y and the Gram matrix are bitwise those of the two separate kernels, with y read once.
+template <int K>
+__global__ __launch_bounds__(RB) void rotate_gram_kernel(const double* __restrict__ r, const double* __restrict__ x,
+                                                         double* __restrict__ y, const double* __restrict__ q, long n,
+                                                         double* __restrict__ partials) {
+  __shared__ double sr[KROT * KROT];
+  __shared__ double shared[4 * K * K];
+  for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
+    const int i = e / K, j = e % K;  // output i, input j
+    sr[i * KROT + j] = r[j * K + i];
+  }
+  __syncthreads();
+  double acc[K * K];
+#pragma unroll
+  for (int i = 0; i < K * K; ++i) acc[i] = 0.0;
+  const long stride = (long)gridDim.x * RB;
+  constexpr int U = K * K <= 25 ? 4 : (K * K <= 36 ? 2 : 1);
+  for (long p0 = (long)blockIdx.x * RB + threadIdx.x; p0 < n; p0 += U * stride) {
+    double w[U], yv[U][K];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long p = p0 + u * stride;
+      const bool in = p < n;
+      w[u] = in ? (q ? q[p] : 1.0) : 0.0;
+      double xv[KROT];
+#pragma unroll
+      for (int j = 0; j < KROT; ++j) xv[j] = (in && j < K) ? x[(long)j * n + p] : 0.0;
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        double sum = 0.0;
+#pragma unroll
+        for (int j = 0; j < KROT; ++j)
+          if (j < K) sum += sr[i * KROT + j] * xv[j];
+        yv[u][i] = sum;
+        if (in) y[(long)i * n + p] = sum;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (p0 + u * stride < n) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          const double av = yv[u][i] * w[u];
+#pragma unroll
+          for (int j = 0; j < K; ++j) acc[i * K + j] += av * yv[u][j];
+        }
+      }
+    }
+  }
+  block_sums<K * K>(acc, shared);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+#pragma unroll
+      for (int j = 0; j < K; ++j) partials[(long)blockIdx.x * KMAX * KMAX + i * KMAX + j] = acc[i * K + j];
+  }
+}
+
 // ---- linear combination ----
 
 struct Terms {
@@ -531,6 +592,32 @@ extern "C" int cdft_wgram_f64(const double* a, int ka, const double* b, int kb, 
     }
   }
   return CDFT_OK;
+}
+
+// y = R^T x and gram = sum_p q_p y_ip y_jp (k <= 8; q may be NULL): bitwise cdft_rotate_f64 (transpose)
+// followed by cdft_wgram_f64(y, k, y, k, q, n, 1, 0, ..).
+extern "C" int cdft_rotate_gram_f64(const double* r, int k, const double* x, double* y, const double* q, long n,
+                                    double* partials, long partials_len, double* gram, void* stream) {
+  if (!r || !x || !y || !partials || !gram) return fail(CDFT_E_ARG, "rotate_gram: null pointer argument");
+  if (k < 1 || k > KMAX || n < 1) return fail(CDFT_E_ARG, "rotate_gram: k must be in 1..%d", KMAX);
+  if (x == y) return fail(CDFT_E_ARG, "rotate_gram: y must not alias x");
+  const int blocks = reduction_blocks(n);
+  if (partials_len < (long)blocks * KMAX * KMAX) return fail(CDFT_E_CAPACITY, "rotate_gram: partials too small");
+  hipStream_t s = (hipStream_t)stream;
+  switch (k) {
+    case 1: hipLaunchKernelGGL(rotate_gram_kernel<1>, dim3(blocks), dim3(RB), 0, s, r, x, y, q, n, partials); break;
+    case 2: hipLaunchKernelGGL(rotate_gram_kernel<2>, dim3(blocks), dim3(RB), 0, s, r, x, y, q, n, partials); break;
+    case 3: hipLaunchKernelGGL(rotate_gram_kernel<3>, dim3(blocks), dim3(RB), 0, s, r, x, y, q, n, partials); break;
+    case 4: hipLaunchKernelGGL(rotate_gram_kernel<4>, dim3(blocks), dim3(RB), 0, s, r, x, y, q, n, partials); break;
+    case 5: hipLaunchKernelGGL(rotate_gram_kernel<5>, dim3(blocks), dim3(RB), 0, s, r, x, y, q, n, partials); break;
+    case 6: hipLaunchKernelGGL(rotate_gram_kernel<6>, dim3(blocks), dim3(RB), 0, s, r, x, y, q, n, partials); break;
+    case 7: hipLaunchKernelGGL(rotate_gram_kernel<7>, dim3(blocks), dim3(RB), 0, s, r, x, y, q, n, partials); break;
+    default: hipLaunchKernelGGL(rotate_gram_kernel<8>, dim3(blocks), dim3(RB), 0, s, r, x, y, q, n, partials); break;
+  }
+  int rc = check_launch("rotate_gram");
+  if (rc) return rc;
+  hipLaunchKernelGGL(wgram_final_kernel, dim3(k * k), dim3(64), 0, s, partials, blocks, k, k, 1.0, 0, gram, k, 0, 0);
+  return check_launch("rotate_gram_final");
 }
 
 extern "C" long cdft_reduction_workspace_doubles(long n) {

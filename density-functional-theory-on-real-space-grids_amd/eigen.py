@@ -113,8 +113,11 @@ def rayleigh_ritz(geometry, measure, block: torch.Tensor, work: RitzWork) -> Non
     ops.hdiv_apply(work.basis, geometry, work.applied)
     measure.cross(work.basis, work.applied, out=work.M)
     ops.eigh(work.M, work.evals, work.C)
-    ops.rotate(work.C, work.basis, transpose=True, out=work.vectors)
-    measure.gram(work.vectors, out=work.G)
+    if k <= 8:
+        ops.rotate_gram(work.C, work.basis, work.vectors, measure.q, work.G)
+    else:
+        ops.rotate(work.C, work.basis, transpose=True, out=work.vectors)
+        measure.gram(work.vectors, out=work.G)
     ops.orth_error(work.G, work.scalars[2 * k: 2 * k + 1])
 
 

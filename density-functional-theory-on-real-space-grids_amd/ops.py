@@ -270,6 +270,22 @@ def rotate(r: torch.Tensor, x: torch.Tensor, transpose: bool = False, out: torch
     return out
 
 
+def rotate_gram(r: torch.Tensor, x: torch.Tensor, y: torch.Tensor, q: torch.Tensor | None,
+                gram: torch.Tensor) -> torch.Tensor:
+    """``y = r^T x`` and ``gram = sum_p q_p y_ip y_jp`` in one pass over ``y`` (bitwise ``rotate`` with
+    ``transpose=True`` then ``wgram(y, None, q)``); returns ``y``."""
+    k, n = x.shape
+    require(r, (k, k), torch.float64, x.device, "r")
+    require(y, (k, n), torch.float64, x.device, "y")
+    require(gram, (k, k), torch.float64, x.device, "gram")
+    if q is not None:
+        require(q, (n,), torch.float64, x.device, "q")
+    ws = partials_for(n, x.device)
+    _lib.call("cdft_rotate_gram_f64", r.data_ptr(), k, x.data_ptr(), y.data_ptr(), _ptr(q), n, ws.data_ptr(), ws.numel(),
+              gram.data_ptr(), stream_of(x))
+    return y
+
+
 def lincomb(out: torch.Tensor, terms: list, dev: torch.Tensor | None = None) -> torch.Tensor:
     """``out = sum_t c_t x_t``; each term is ``(x, c)`` or ``(x, c, mode, idx)`` with mode 1 multiplying
     and mode 2 dividing by the device scalar ``dev[idx]``. ``out`` may alias a term."""

@@ -126,6 +126,12 @@ int cdft_wgram_f64(const double* a, int ka, const double* b, int kb, const doubl
                    double scale, int post, double* partials, long partials_len, double* out,
                    void* stream);
 
+/* y = R^T x (k x k, k <= 8) fused with gram = sum_p q_p y_ip y_jp (q may be NULL), bitwise cdft_rotate_f64
+ * (transpose = 1) followed by cdft_wgram_f64 of y with itself: the rotation of the Ritz vectors and
+ * their orthonormality Gram matrix (src/cdft/eigen/rayleigh_ritz.py:95-102). */
+int cdft_rotate_gram_f64(const double* r, int k, const double* x, double* y, const double* q, long n,
+                         double* partials, long partials_len, double* gram, void* stream);
+
 /* y[i] = sum_j R[i][j] x[j] (R kout x kin row-major on the device; transpose = 1 uses R^T),
  * kin, kout <= 16. Replaces the rotations `rotation.T @ basis` and the triangular solve of
  * orthonormalise (src/cdft/eigen/rayleigh_ritz.py:51-102, precision.py:292-308). */

@@ -112,3 +112,19 @@ def test_device_chebyshev_table_is_the_host_table(pkg, device):
         lower = min(max(float(evals[4]), smin + 1e-8), smax - 1e-8)
         expect = np.asarray(chebyshev_coefficients(degree, lower, smax, smin))
         assert np.array_equal(table.cpu().numpy(), expect), trial
+
+
+@pytest.mark.parametrize("k,n", [(5, 274625), (3, 1001), (8, 4097), (1, 77)])
+def test_rotate_gram_is_rotate_then_gram(pkg, device, k, n):
+    """The fused rotation + weighted Gram gives bitwise the rotation and the Gram matrix of the two
+    separate kernels."""
+    rng = np.random.default_rng(k + n)
+    x = torch.as_tensor(rng.standard_normal((k, n))).to(device)
+    r = torch.as_tensor(rng.standard_normal((k, k))).to(device)
+    q = torch.as_tensor(rng.uniform(0.1, 2.0, n)).to(device)
+    y = torch.empty_like(x)
+    gram = torch.empty((k, k), dtype=torch.float64, device=device)
+    pkg.ops.rotate_gram(r, x, y, q, gram)
+    y2 = pkg.ops.rotate(r, x, transpose=True)
+    assert torch.equal(y, y2)
+    assert torch.equal(gram, pkg.ops.wgram(y2, None, q))
