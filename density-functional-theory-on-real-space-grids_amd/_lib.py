@@ -72,7 +72,7 @@ SIGNATURES: dict[str, list] = {
     "cdft_poisson_create": [I, I, I, I, I, I, P],
     "cdft_poisson_destroy": [P],
     "cdft_poisson_kernel_f64": [P, D, D, P, P],
-    "cdft_poisson_solve_f64": [P, P, P, P, P, P, P, P],
+    "cdft_poisson_solve_f64": [P, P, P, P, P, P, P, P, P],
 }
 
 #: Entry points that return a value rather than a status.

@@ -1,12 +1,20 @@
 // K5: the Coulomb-cutoff free-space Poisson solve on a zero-padded box (reference
-// src/cdft/operators/poisson.py:27-84): rocFFT (through hipFFT) D2Z / Z2D plans, plus hand-written
-// kernels for the Green's function K(G) = 4 pi (1 - cos G R)/G^2, K(0) = 2 pi R^2 (stored divided by
-// the transform size, so the inverse needs no separate normalisation pass), the pad-scatter of the
-// density into the padded box and the gather of the interior potential.
+// src/cdft/operators/poisson.py:27-84): rocFFT (through hipFFT) plans, plus hand-written kernels for
+// the Green's function K(G) = 4 pi (1 - cos G R)/G^2, K(0) = 2 pi R^2 (stored divided by the
+// transform size, so the inverse needs no separate normalisation pass), the pad-scatter of the
+// density and the gather of the interior potential.
 //
-// Buffers are owned by the caller: `padded_in` (P real, zero outside the box, never written by the
-// transforms), `spectrum` (P0 x P1 x (P2/2+1) complex), `padded_out` (P real). The padded input
-// region outside the box is zeroed once at setup; every solve rewrites only the interior.
+// The transforms are pruned to the box. Only the first n0 of the p0 x-slabs of the padded density are
+// non-zero, and only the first n0 x-slabs of the potential are kept, so the (y, z) transforms run on
+// n0 slabs and only the x transform covers the whole padded spectrum:
+//   forward: 2-D D2Z over (y, z) of the n0 slabs -> `slabs` (p0 x p1 x (p2/2+1) complex, slabs >= n0
+//            permanently zero), then a strided 1-D Z2Z along x of every (ky, kz) column -> `spectrum`;
+//   multiply by the kernel;
+//   inverse: the 1-D Z2Z along x in place on `spectrum`, then a 2-D Z2D over (y, z) of its first n0
+//            slabs -> `padded_out` (n0 x p1 x p2 real), of which the (n1, n2) corner is the potential.
+// The zero rows y >= n1 and columns z >= n2 inside those slabs stay in the (y, z) transforms. The
+// result is the full 3-D transform's up to rounding. Buffers are owned by the caller: `padded_in` (n0
+// p1 p2 real, zero outside the box, never written by the transforms), `slabs`, `spectrum`, `padded_out`.
 #include "common.h"
 
 #include <hipfft/hipfft.h>
@@ -14,10 +22,12 @@
 namespace cdft {
 
 struct PoissonPlan {
-  hipfftHandle forward;
-  hipfftHandle inverse;
-  int n0, n1, n2;  // box
-  int p0, p1, p2;  // padded
+  hipfftHandle yz_forward;  // 2-D D2Z over (p1, p2), n0 slabs
+  hipfftHandle yz_inverse;  // 2-D Z2D over (p1, p2), n0 slabs
+  hipfftHandle x_forward;   // 1-D Z2Z along x, stride p1 (p2/2+1), out of place
+  hipfftHandle x_inverse;   // 1-D Z2Z along x, in place
+  int n0, n1, n2;           // box
+  int p0, p1, p2;           // padded
 };
 
 __global__ void green_kernel(int p0, int p1, int p2, double h, double radius, double norm,
@@ -90,14 +100,33 @@ extern "C" int cdft_poisson_create(int n0, int n1, int n2, int p0, int p1, int p
   PoissonPlan* plan = new PoissonPlan();
   plan->n0 = n0; plan->n1 = n1; plan->n2 = n2;
   plan->p0 = p0; plan->p1 = p1; plan->p2 = p2;
-  if (hipfftPlan3d(&plan->forward, p0, p1, p2, HIPFFT_D2Z) != HIPFFT_SUCCESS) {
-    delete plan;
-    return fail(CDFT_E_LIBRARY, "hipfftPlan3d D2Z (%d, %d, %d) failed", p0, p1, p2);
+  const int q2 = p2 / 2 + 1;
+  int yz[2] = {p1, p2};
+  int yz_real[2] = {p1, p2};
+  int yz_complex[2] = {p1, q2};
+  int x[1] = {p0};
+  int x_embed[1] = {p0};
+  const int column = p1 * q2;
+  int made = 0;
+  hipfftHandle* handles[4] = {&plan->yz_forward, &plan->yz_inverse, &plan->x_forward, &plan->x_inverse};
+  if (hipfftPlanMany(&plan->yz_forward, 2, yz, yz_real, 1, p1 * p2, yz_complex, 1, column, HIPFFT_D2Z, n0) == HIPFFT_SUCCESS) {
+    ++made;
+    if (hipfftPlanMany(&plan->yz_inverse, 2, yz, yz_complex, 1, column, yz_real, 1, p1 * p2, HIPFFT_Z2D, n0) ==
+        HIPFFT_SUCCESS) {
+      ++made;
+      if (hipfftPlanMany(&plan->x_forward, 1, x, x_embed, column, 1, x_embed, column, 1, HIPFFT_Z2Z, column) ==
+          HIPFFT_SUCCESS) {
+        ++made;
+        if (hipfftPlanMany(&plan->x_inverse, 1, x, x_embed, column, 1, x_embed, column, 1, HIPFFT_Z2Z, column) ==
+            HIPFFT_SUCCESS)
+          ++made;
+      }
+    }
   }
-  if (hipfftPlan3d(&plan->inverse, p0, p1, p2, HIPFFT_Z2D) != HIPFFT_SUCCESS) {
-    hipfftDestroy(plan->forward);
+  if (made < 4) {
+    for (int i = 0; i < made; ++i) hipfftDestroy(*handles[i]);
     delete plan;
-    return fail(CDFT_E_LIBRARY, "hipfftPlan3d Z2D (%d, %d, %d) failed", p0, p1, p2);
+    return fail(CDFT_E_LIBRARY, "hipfftPlanMany for the (%d, %d, %d) padded box failed (plan %d)", p0, p1, p2, made);
   }
   *handle = plan;
   return CDFT_OK;
@@ -106,8 +135,10 @@ extern "C" int cdft_poisson_create(int n0, int n1, int n2, int p0, int p1, int p
 extern "C" int cdft_poisson_destroy(void* handle) {
   if (!handle) return CDFT_OK;
   PoissonPlan* plan = (PoissonPlan*)handle;
-  hipfftDestroy(plan->forward);
-  hipfftDestroy(plan->inverse);
+  hipfftDestroy(plan->yz_forward);
+  hipfftDestroy(plan->yz_inverse);
+  hipfftDestroy(plan->x_forward);
+  hipfftDestroy(plan->x_inverse);
   delete plan;
   return CDFT_OK;
 }
@@ -123,10 +154,10 @@ extern "C" int cdft_poisson_kernel_f64(void* handle, double h, double radius, do
   return check_launch("poisson_kernel");
 }
 
-extern "C" int cdft_poisson_solve_f64(void* handle, const double* density, double* padded_in,
-                                      void* spectrum, double* padded_out, const double* kernel,
-                                      double* potential, void* stream) {
-  if (!handle || !density || !padded_in || !spectrum || !padded_out || !kernel || !potential)
+extern "C" int cdft_poisson_solve_f64(void* handle, const double* density, double* padded_in, void* slabs,
+                                      void* spectrum, double* padded_out, const double* kernel, double* potential,
+                                      void* stream) {
+  if (!handle || !density || !padded_in || !slabs || !spectrum || !padded_out || !kernel || !potential)
     return fail(CDFT_E_ARG, "poisson_solve: null argument");
   PoissonPlan* pl = (PoissonPlan*)handle;
   hipStream_t s = (hipStream_t)stream;
@@ -136,15 +167,21 @@ extern "C" int cdft_poisson_solve_f64(void* handle, const double* density, doubl
                      pl->p2, padded_in);
   int rc = check_launch("poisson pad_scatter");
   if (rc) return rc;
-  if (hipfftSetStream(pl->forward, s) != HIPFFT_SUCCESS || hipfftSetStream(pl->inverse, s) != HIPFFT_SUCCESS)
-    return fail(CDFT_E_LIBRARY, "hipfftSetStream failed");
-  if (hipfftExecD2Z(pl->forward, padded_in, (hipfftDoubleComplex*)spectrum) != HIPFFT_SUCCESS)
-    return fail(CDFT_E_LIBRARY, "hipfftExecD2Z failed");
+  for (hipfftHandle h : {pl->yz_forward, pl->yz_inverse, pl->x_forward, pl->x_inverse})
+    if (hipfftSetStream(h, s) != HIPFFT_SUCCESS) return fail(CDFT_E_LIBRARY, "hipfftSetStream failed");
+  hipfftDoubleComplex* sl = (hipfftDoubleComplex*)slabs;
+  hipfftDoubleComplex* sp = (hipfftDoubleComplex*)spectrum;
+  if (hipfftExecD2Z(pl->yz_forward, padded_in, sl) != HIPFFT_SUCCESS)
+    return fail(CDFT_E_LIBRARY, "hipfftExecD2Z (y, z slabs) failed");
+  if (hipfftExecZ2Z(pl->x_forward, sl, sp, HIPFFT_FORWARD) != HIPFFT_SUCCESS)
+    return fail(CDFT_E_LIBRARY, "hipfftExecZ2Z (x, forward) failed");
   hipLaunchKernelGGL(spectrum_multiply_kernel, dim3(nb(total)), dim3(256), 0, s, (double2*)spectrum, kernel, total);
   rc = check_launch("poisson spectrum_multiply");
   if (rc) return rc;
-  if (hipfftExecZ2D(pl->inverse, (hipfftDoubleComplex*)spectrum, padded_out) != HIPFFT_SUCCESS)
-    return fail(CDFT_E_LIBRARY, "hipfftExecZ2D failed");
+  if (hipfftExecZ2Z(pl->x_inverse, sp, sp, HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
+    return fail(CDFT_E_LIBRARY, "hipfftExecZ2Z (x, inverse) failed");
+  if (hipfftExecZ2D(pl->yz_inverse, sp, padded_out) != HIPFFT_SUCCESS)
+    return fail(CDFT_E_LIBRARY, "hipfftExecZ2D (y, z slabs) failed");
   hipLaunchKernelGGL(gather_kernel, dim3(nb(n)), dim3(256), 0, s, padded_out, pl->n0, pl->n1, pl->n2, pl->p1, pl->p2,
                      potential);
   return check_launch("poisson gather");

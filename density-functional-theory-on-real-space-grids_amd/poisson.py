@@ -4,7 +4,8 @@ Mirrors ``CoulombCutoffPoisson`` (reference ``src/cdft/operators/poisson.py:27-1
 ``fft_friendly(ceil(pad * n))`` per axis, cutoff radius ``R = min(P_d h)/2``, kernel
 ``K(G) = 4 pi (1 - cos G R)/G^2`` with ``K(0) = 2 pi R^2``. The plan and the Green's-function table
 are built once per grid and spacing (and shared by later solvers of the same grid); a solve is
-pad-scatter, D2Z, multiply, Z2D, gather (K5).
+pad-scatter, (y, z) D2Z on the occupied x-slabs, x transform, multiply, x transform, (y, z) Z2D on the
+kept x-slabs, gather (K5).
 """
 
 from __future__ import annotations
@@ -41,8 +42,10 @@ class _Plan:
         handle = ctypes.c_void_p()
         _lib.call("cdft_poisson_create", n0, n1, n2, p0, p1, p2, ctypes.addressof(handle))
         self.handle = handle
-        self.padded_in = torch.zeros((p0, p1, p2), dtype=torch.float64, device=device)  # padding stays zero
-        self.padded_out = torch.empty((p0, p1, p2), dtype=torch.float64, device=device)
+        # the transforms are pruned to the n0 occupied x-slabs (see cdft_poisson_solve_f64)
+        self.padded_in = torch.zeros((n0, p1, p2), dtype=torch.float64, device=device)  # padding stays zero
+        self.padded_out = torch.empty((n0, p1, p2), dtype=torch.float64, device=device)
+        self.slabs = torch.zeros((p0, p1, p2 // 2 + 1, 2), dtype=torch.float64, device=device)  # slabs >= n0 stay zero
         self.spectrum = torch.empty((p0, p1, p2 // 2 + 1, 2), dtype=torch.float64, device=device)
         self.kernel = torch.empty((p0, p1, p2 // 2 + 1), dtype=torch.float64, device=device)
         _lib.call("cdft_poisson_kernel_f64", handle, spacing, radius, self.kernel.data_ptr(), stream_of(self.kernel))
@@ -78,6 +81,7 @@ class CoulombCutoffPoisson:
         self._handle = plan.handle
         self.padded_in = plan.padded_in
         self.padded_out = plan.padded_out
+        self.slabs = plan.slabs
         self.spectrum = plan.spectrum
         self.kernel = plan.kernel
 
@@ -87,7 +91,7 @@ class CoulombCutoffPoisson:
             raise ValueError("expected a total density of shape (n_points,)")
         out = torch.empty_like(density) if out is None else out
         _lib.call("cdft_poisson_solve_f64", self._handle, density.data_ptr(), self.padded_in.data_ptr(),
-                  self.spectrum.data_ptr(), self.padded_out.data_ptr(), self.kernel.data_ptr(), out.data_ptr(),
-                  stream_of(density))
+                  self.slabs.data_ptr(), self.spectrum.data_ptr(), self.padded_out.data_ptr(), self.kernel.data_ptr(),
+                  out.data_ptr(), stream_of(density))
         return out
 

@@ -337,12 +337,13 @@ int cdft_poisson_destroy(void* handle);
 /* kernel (p0 x p1 x (p2/2+1)) = K(G) / (p0 p1 p2), K = 4 pi (1 - cos G R) / G^2, K(0) = 2 pi R^2.
  * Replaces CoulombCutoffPoisson._build_kernel (src/cdft/operators/poisson.py:43-64). */
 int cdft_poisson_kernel_f64(void* handle, double h, double radius, double* kernel, void* stream);
-/* potential = the cut-off Coulomb potential of density; padded_in/padded_out (p0 p1 p2 doubles)
- * and spectrum (p0 p1 (p2/2+1) complex doubles) are caller workspace, padded_in's padding must be
- * zero. Replaces CoulombCutoffPoisson.solve (src/cdft/operators/poisson.py:66-84). */
-int cdft_poisson_solve_f64(void* handle, const double* density, double* padded_in, void* spectrum,
-                           double* padded_out, const double* kernel, double* potential,
-                           void* stream);
+/* potential = the cut-off Coulomb potential of density, with the transforms pruned to the box: (y, z)
+ * transforms on the n0 occupied x-slabs only, the x transform on the whole padded spectrum. Caller
+ * workspace: padded_in and padded_out (n0 p1 p2 doubles; padded_in zero outside the box), slabs and
+ * spectrum (p0 p1 (p2/2+1) complex doubles; slabs zero beyond its first n0 x-slabs); the zero parts
+ * are never written. Replaces CoulombCutoffPoisson.solve (src/cdft/operators/poisson.py:66-84). */
+int cdft_poisson_solve_f64(void* handle, const double* density, double* padded_in, void* slabs, void* spectrum,
+                           double* padded_out, const double* kernel, double* potential, void* stream);
 
 #ifdef __cplusplus
 }

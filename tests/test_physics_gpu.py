@@ -129,6 +129,14 @@ def test_poisson_matches_oracle_and_gaussian(he_case, device):
     assert _rel(v, vo) < 1e-12
     energy = 0.5 * grid_o.integrate(rho * v)
     assert abs(energy - 1.0 / (2 * np.sqrt(np.pi) * sigma)) < 1e-6  # Q^2/(2 sqrt(pi) sigma), grid-limited
+    # further solves through the same buffers: an off-centre, asymmetric density (the pruned (y, z)
+    # transforms and the untouched zero slabs), then the first density again, bitwise as before
+    rng = np.random.default_rng(3)
+    rho2 = np.exp(-((pts - [1.3, -0.4, 2.1]) ** 2).sum(-1) / 1.7) * (1.0 + 0.2 * rng.uniform(-1, 1, len(pts)))
+    v2 = solver.solve(torch.as_tensor(rho2).to(device)).cpu().numpy()
+    assert _rel(v2, oracle.solve(rho2)) < 1e-12
+    v3 = solver.solve(torch.as_tensor(rho).to(device)).cpu().numpy()
+    assert np.array_equal(v3, v)
 
 
 def test_lda_matches_oracle(pkg, device):
