@@ -419,6 +419,7 @@ def main(argv=None):
                    "grid": list(shape), "block": 5},
         "energy_total": r["energies"]["total"],
         "hip_graphs": r["graphs"],
+        "speculative_filters": r["speculative_filters"],
         "energy_reference_cpu": -2.8348195294207272,
         "scf_iterations": r["scf_iterations"],
         "block_applies": int(sum(r["applications"])),

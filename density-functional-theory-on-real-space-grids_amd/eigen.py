@@ -86,6 +86,10 @@ class RitzWork:
         self.bounds = torch.zeros(2, **f)
         self.bounds_host = torch.zeros(2, dtype=torch.float64).pin_memory()
         self.bounds_set: tuple | None = None
+        # the per-pass scalars land in pinned host memory behind an event, so the host can wait for
+        # them while a speculative filter runs
+        self.scalars_host = torch.zeros(2 * k + 1, dtype=torch.float64).pin_memory()
+        self.scalars_ready = torch.cuda.Event() if torch.cuda.is_available() else None
         if mixed:
             g = dict(dtype=torch.float32, device=device)
             self.x32 = torch.empty((k, n), **g)
@@ -140,9 +144,33 @@ def _read(work: RitzWork) -> tuple[np.ndarray, np.ndarray, float]:
     return host[:k].copy(), host[k:2 * k].copy(), orth
 
 
+def _read_async(work: RitzWork) -> None:
+    """Start the pass's scalar read: a copy into pinned host memory and an event behind it."""
+    work.scalars_host.copy_(work.scalars, non_blocking=True)
+    work.scalars_ready.record()
+
+
+def _collect(work: RitzWork) -> tuple[np.ndarray, np.ndarray, float]:
+    """Wait for the read started by :func:`_read_async` (not for later work on the stream)."""
+    work.scalars_ready.synchronize()
+    host = work.scalars_host.numpy()
+    k = work.k
+    orth = float(host[2 * k])
+    if not orth < 0.5 and int(work.flag.cpu()[0]) != 0:
+        raise RuntimeError(
+            "Rayleigh-Ritz: the filtered block's Gram matrix is not positive definite (the block has "
+            "collapsed in rank); the eigh fallback of the reference that drops vectors is not implemented")
+    return host[:k].copy(), host[k:2 * k].copy(), orth
+
+
 def _fraction(value, low, high):
     width = high - low
     return (value - low) / width if width > 0.0 else float("inf")
+
+
+#: Speculate the next filter only while the last residual is above this many times the tolerance (a
+#: pass cuts the residual by far less than this at the filter degrees used).
+SPECULATE_RESIDUAL = 1.0e3
 
 
 class ChebyshevFilteredSubspace:
@@ -154,6 +182,9 @@ class ChebyshevFilteredSubspace:
     fp64_tail: int = 1
     #: A float32 pass whose residual is above this fraction of the previous one has stalled.
     stall_ratio: float = 0.7
+    #: Enqueue the next pass's filter before reading a pass's scalars when that pass cannot stop
+    #: the solve (see ``SPECULATE_RESIDUAL``); results are the same either way.
+    speculate: bool = True
 
     def __init__(self, grid, config: EigenConfig, ritz_stop: bool = True, filter_dtype=torch.float64,
                  filter_hook=None, use_graphs: bool = True):
@@ -166,6 +197,7 @@ class ChebyshevFilteredSubspace:
         self.graph_record: dict = {"captures": 0, "replays": 0, "disabled": None}
         self.history: list[dict] = []
         self.filter_precisions: list[str] = []
+        self.speculation = {"used": 0, "redone": 0, "wasted": 0}
         self._work: dict[int, RitzWork] = {}
         self._graphs: dict[tuple, torch.cuda.CUDAGraph] = {}
         n = grid.n_points
@@ -241,14 +273,19 @@ class ChebyshevFilteredSubspace:
                 return
         body()
 
-    def _run_iteration(self, work: RitzWork, use32: bool, degree: int) -> None:
-        """Filter, then Rayleigh-Ritz and residuals: two captured graphs, with the timing hook's
-        events (if any) around the filter graph on the same stream."""
+    def _filter(self, work: RitzWork, use32: bool, degree: int) -> None:
+        """The filter graph, with the timing hook's events (if any) around it on the same stream."""
         if self.filter_hook is not None:
             self.filter_hook("start", self, work.x32 if use32 else work.vectors, degree)
         self._replay(("filter", work.k, use32), lambda: self._filter_into(work, use32))
         if self.filter_hook is not None:
             self.filter_hook("end", self, work.x32 if use32 else work.vectors, degree)
+
+    def _run_iteration(self, work: RitzWork, use32: bool, degree: int, filtered: bool = False) -> None:
+        """Filter (unless a speculative filter already did, with these settings), then Rayleigh-Ritz
+        and residuals: two captured graphs."""
+        if not filtered:
+            self._filter(work, use32, degree)
         self._replay(("ritz", work.k), lambda: self._ritz(work))
 
     def _lanczos(self, hamiltonian, generator) -> tuple[float, float]:
@@ -270,17 +307,17 @@ class ChebyshevFilteredSubspace:
         hamiltonian.note_applications(steps)
         return lanczos_finish(scal.cpu().numpy(), steps, cfg.bound_safety)
 
-    def _set_bounds(self, work: RitzWork, smin: float, smax: float) -> int:
+    def _set_bounds(self, work: RitzWork, smin: float, smax: float) -> bool:
         """Point the filter at ``[smin, smax]``. The table itself is built on the device from the
         last Ritz value (at the end of every Rayleigh-Ritz graph); a change of bounds uploads them
-        and rebuilds the table at once. Returns the filter degree."""
-        degree = self.config.chebyshev_degree
+        and rebuilds the table at once. Returns whether the bounds changed."""
         if work.bounds_set != (smin, smax):
             work.bounds_set = (smin, smax)
             work.bounds_host[0], work.bounds_host[1] = smin, smax
             work.bounds.copy_(work.bounds_host, non_blocking=True)
-            ops.cheb_table(work.evals, work.k - 1, work.bounds, degree, work.table)
-        return degree
+            ops.cheb_table(work.evals, work.k - 1, work.bounds, self.config.chebyshev_degree, work.table)
+            return True
+        return False
 
     # ---- the solver ----
 
@@ -336,6 +373,7 @@ class ChebyshevFilteredSubspace:
         converged = False
         drift = float("inf")
         iteration = 0
+        speculative = None  # precision of a speculative filter of the coming pass, if one is enqueued
         for iteration in range(1, cfg.max_iterations + 1):
             lower = next_lower
             fraction = _fraction(lower, spectrum_min, spectrum_max)
@@ -353,11 +391,27 @@ class ChebyshevFilteredSubspace:
                          and iteration <= cfg.max_iterations - self.fp64_tail)
             # the device-built table takes the last Ritz value (clamped into the bounds) as its lower
             # edge; the host copy of it above decides only the bound reuse
-            degree = self._set_bounds(work, spectrum_min, spectrum_max)
+            degree = cfg.chebyshev_degree
+            changed = self._set_bounds(work, spectrum_min, spectrum_max)
             self.filter_precisions.append("f32" if use32 else "f64")
-            self._run_iteration(work, use32, degree)
+            # A speculative filter of this pass (enqueued before the previous pass's read) stands
+            # unless the bounds or the precision have changed since; it then runs again.
+            ready = speculative == use32 and not changed
+            if speculative is not None:
+                self.speculation["used" if ready else "redone"] += 1
+            self._run_iteration(work, use32, degree, filtered=ready)
+            _read_async(work)
+            # Speculate the next pass's filter while the host waits for this pass's scalars, when
+            # this pass cannot be the last: not at the iteration limit, the residual stop far away
+            # (the last residual above SPECULATE_RESIDUAL x tol) and the weak-form stop impossible
+            # (it needs two stable passes in a row and the last pass was not stable).
+            speculative = None
+            if (self.speculate and iteration < cfg.max_iterations and worst > SPECULATE_RESIDUAL * cfg.residual_tol
+                    and stable == 0):
+                self._filter(work, use32, degree)
+                speculative = use32
+            evals, norms, orth = _collect(work)
             hamiltonian.note_applications(degree + 2)
-            evals, norms, orth = _read(work)
             before = worst
             worst = float(norms[:n_states].max())
             # A float32 pass that no longer shrinks the residual has reached float32's floor
@@ -378,6 +432,8 @@ class ChebyshevFilteredSubspace:
                 converged = True
                 stop_reason = "Ritz values stable on the weak-form path"
                 break
+        if speculative is not None and (converged or iteration == cfg.max_iterations):
+            self.speculation["wasted"] += 1
         info = {"spectral_bounds": (spectrum_min, spectrum_max), "bounds_source": source, "bounds_shift": shift_used,
                 "top_ritz_fraction": top_fraction, "stop_reason": stop_reason, "eigenvalue_drift": drift,
                 "graphs": dict(self.graph_record)}

@@ -531,6 +531,7 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
         "fallback_events": events,
         "filter_precisions": list(solver.filter_precisions),
         "graphs": dict(solver.graph_record),
+        "speculative_filters": dict(solver.speculation),
         "wall_seconds": time.perf_counter() - started,
         "step": step,
         "effective_potential": pin["effective"],
