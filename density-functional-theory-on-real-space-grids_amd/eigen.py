@@ -57,7 +57,24 @@ class SpectralBoundsHint:
 
     lower: float
     upper: float
-    shift: float = 0.0
+    shift: float = 0.0  # or a DeviceScalar, read when the solve first needs it
+
+
+class DeviceScalar:
+    """A device scalar copied to pinned host memory behind an event as soon as it is made; ``float()``
+    waits only for that copy. Lets a caller hand the solver a value still being computed without a
+    synchronising read of its own (the solver reads it after its first Rayleigh-Ritz, by which time
+    the copy has long finished)."""
+
+    def __init__(self, value: torch.Tensor):
+        self._host = torch.empty(1, dtype=torch.float64).pin_memory()
+        self._host.copy_(value.reshape(1), non_blocking=True)
+        self._ready = torch.cuda.Event()
+        self._ready.record()
+
+    def __float__(self) -> float:
+        self._ready.synchronize()
+        return float(self._host[0])
 
 
 class RitzWork:
@@ -292,20 +309,33 @@ class ChebyshevFilteredSubspace:
         """The operator's Lanczos enclosure (``hamiltonian.spectral_bounds``: the same seeded start,
         recurrence and widening) run through a captured graph on the solver's static geometry,
         which holds this operator's potential; one read of the recurrence scalars."""
+        return self._lanczos_end(self._lanczos_begin(hamiltonian, generator))
+
+    def _lanczos_begin(self, hamiltonian, generator):
+        """Enqueue the Lanczos run of :meth:`_lanczos` and the copy of its scalars to pinned host
+        memory; :meth:`_lanczos_end` waits for that copy and finishes on the host."""
         cfg = self.config
         steps = cfg.lanczos_steps
         start = seeded_randn(self.grid.n_points, generator, self.grid.device)
         buffers = self.__dict__.setdefault("_lanczos_buffers", {})
         if steps not in buffers:
             f = dict(dtype=torch.float64, device=self.grid.device)
-            buffers[steps] = (torch.empty(self.grid.n_points, **f), torch.zeros(2 * steps + 3, **f))
-        start_s, scal = buffers[steps]
+            buffers[steps] = (torch.empty(self.grid.n_points, **f), torch.zeros(2 * steps + 3, **f),
+                              torch.zeros(2 * steps + 3, dtype=torch.float64).pin_memory(), torch.cuda.Event())
+        start_s, scal, host, ready = buffers[steps]
         start_s.copy_(start)
         geom, measure = self._geom64, self._measure
         self._replay(("lanczos", steps),
                      lambda: lanczos_enqueue(lambda x, y: ops.hdiv_apply(x, geom, y), measure, start_s, steps, scal))
         hamiltonian.note_applications(steps)
-        return lanczos_finish(scal.cpu().numpy(), steps, cfg.bound_safety)
+        host.copy_(scal, non_blocking=True)
+        ready.record()
+        return steps, host, ready
+
+    def _lanczos_end(self, pending) -> tuple[float, float]:
+        steps, host, ready = pending
+        ready.synchronize()
+        return lanczos_finish(host.numpy().copy(), steps, self.config.bound_safety)
 
     def _set_bounds(self, work: RitzWork, smin: float, smax: float) -> bool:
         """Point the filter at ``[smin, smax]``. The table itself is built on the device from the
@@ -347,16 +377,18 @@ class ChebyshevFilteredSubspace:
             return self._lanczos(hamiltonian, generator)
 
         source, shift_used = "unset", 0.0
-        if bounds is None:
-            spectrum_min, spectrum_max = lanczos()
-            source = "lanczos"
+        pending = self._lanczos_begin(hamiltonian, generator) if bounds is None else None
         # the first Rayleigh-Ritz is the iteration's: the start block goes where a filter leaves its
-        # output, and the captured Rayleigh-Ritz graph runs on it
+        # output, and the captured Rayleigh-Ritz graph runs on it (behind the Lanczos run, whose
+        # scalars are read with its own)
         work.filtered.copy_(block)
         self._replay(("ritz", work.k), lambda: self._ritz(work))
         hamiltonian.note_applications(2)
         work.bounds_set = None  # new Ritz values: the first pass rebuilds its table
         evals, norms, _ = _read(work)
+        if pending is not None:
+            spectrum_min, spectrum_max = self._lanczos_end(pending)
+            source = "lanczos"
         worst = float(norms[:n_states].max())
         stalled32 = False
         previous = evals.copy()
@@ -366,9 +398,9 @@ class ChebyshevFilteredSubspace:
         eigenvalue_tol = 0.1 * cfg.cross_check_tol
         next_lower = float(evals[min(size, evals.shape[0]) - 1])
         if bounds is not None:
-            spectrum_min, spectrum_max = bounds.lower - bounds.shift, bounds.upper + bounds.shift
-            source = "reused"
             shift_used = float(bounds.shift)
+            spectrum_min, spectrum_max = bounds.lower - shift_used, bounds.upper + shift_used
+            source = "reused"
         top_fraction = float("-inf")
         converged = False
         drift = float("inf")

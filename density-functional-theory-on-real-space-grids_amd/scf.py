@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import _lib, ops
-from .eigen import ChebyshevFilteredSubspace, EigenConfig, SpectralBoundsHint
+from .eigen import ChebyshevFilteredSubspace, DeviceScalar, EigenConfig, SpectralBoundsHint
 from .geometry import CuspFactor, CuspQuadrature, UniformGrid
 from .hamiltonian import CuspFactoredHamiltonian, SeededDraws, prefetch_draws
 from .ops import stream_of
@@ -457,7 +457,7 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
         hint = None
         if reason is None and previous_effective is not None:
             ops.reduce(ops.RED_MAXDIFF, pin["effective"], norm_buf, 2, b=previous_effective)
-            hint = SpectralBoundsHint(policy.bounds[0], policy.bounds[1], float(norm_buf[2].item()))
+            hint = SpectralBoundsHint(policy.bounds[0], policy.bounds[1], DeviceScalar(norm_buf[2:3]))
         elif reason is None:
             reason = "no previous potential"
         # Mixed precision: the warm passes of an iteration filter in float32; the first (cold) solve
