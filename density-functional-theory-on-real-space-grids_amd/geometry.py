@@ -292,11 +292,13 @@ class CuspQuadrature:
         # Number the nodes in the order of their interpolation base cell (stable), so the nodes a
         # grid point gathers from in the transpose are contiguous in memory. Only the order of the
         # node sums changes.
-        base_key = np.floor((points - origin) / h).astype(np.int64)
+        base_key = np.floor((points - origin) / h).astype(np.int32)
         lo_key = base_key.min(axis=0)
         ext_key = base_key.max(axis=0) - lo_key + 1
-        key = ((base_key[:, 0] - lo_key[0]) * ext_key[1] + (base_key[:, 1] - lo_key[1])) * ext_key[2] + (base_key[:, 2] - lo_key[2])
-        perm = np.argsort(key, kind="stable")
+        rel = base_key - lo_key
+        key = (rel[:, 0].astype(np.int64) * int(ext_key[1]) + rel[:, 1]) * int(ext_key[2]) + rel[:, 2]
+        # the same stable order; 16-bit keys (cell boxes up to 65,536 cells) take numpy's radix sort
+        perm = np.argsort(key.astype(np.uint16) if int(ext_key.prod()) <= 65536 else key, kind="stable")
         points, owners, bare_weights = points[perm], owners[perm], bare_weights[perm]
         self.n_nodes = m = points.shape[0]
         as_dev = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)  # noqa: E731
@@ -322,22 +324,18 @@ class CuspQuadrature:
                   self.base.data_ptr(), self.lw.data_ptr(), bad.data_ptr(), s)
         # The kernel's base cells are floor((p - origin) / h) in the same IEEE operations as the host's
         # ordering key above, so the binning is built from the host copy without a device read.
-        base_host = base_key[perm].astype(np.int32)
         dims = np.asarray(grid.shape)
-        if (base_host - 3 < 0).any() or (base_host + 4 >= dims).any():
+        if (lo_key - 3 < 0).any() or (lo_key + ext_key - 1 + 4 >= dims).any():
             raise ValueError("an interpolation stencil reaches outside the grid; the sphere radius bound is wrong")
-        self._base_host = base_host
+        self._base_key, self._perm = base_key, perm
         lo, ext = lo_key, ext_key  # the same cells, so the same cell box
-        cell = key[perm]  # the cell index of every node, in node order
-        if np.any(cell[1:] < cell[:-1]):
-            raise RuntimeError("quadrature nodes are not in base-cell order")
-        counts = np.bincount(cell, minlength=int(ext.prod()))
+        counts = np.bincount(key, minlength=int(ext.prod()))  # nodes per cell (order-free)
         start = np.zeros(counts.size + 1, dtype=np.int32)
         np.cumsum(counts, out=start[1:])
         self.cell_start = as_dev(start, torch.int32)
         self.node_order = perm  # GPU node i is node perm[i] of the sphere-by-sphere enumeration
         self._cell_box = _ints([lo[0], lo[1], lo[2], ext[0], ext[1], ext[2]])
-        self._build_lift_chunks(base_host, start, lo, ext, as_dev)
+        self._build_lift_chunks(base_key[perm, 2], start, lo, ext, as_dev)
         # Far part of the rule.
         self.far = grid.empty()
         _lib.call("cdft_far_factor_f64", grid.c_shape, h, grid.c_origin, n_atoms, ctypes.addressof(self._spheres_c),
@@ -360,7 +358,12 @@ class CuspQuadrature:
     #: Nodes per chunk of the transpose's column walk (one LDS staging of node records).
     lift_chunk = 64
 
-    def _build_lift_chunks(self, base_host, cell_start, lo, ext, as_dev) -> None:
+    @property
+    def _base_host(self) -> np.ndarray:
+        """The nodes' base cells (``(m, 3)`` int32, node order), as the stencil kernel computes them."""
+        return self._base_key[self._perm]
+
+    def _build_lift_chunks(self, base_z, cell_start, lo, ext, as_dev) -> None:
         """Split every z-column of the cell box into chunks of at most ``lift_chunk`` consecutive
         nodes, with each chunk's z-cell range and the offset of its (64 x height) slab."""
         c2 = int(ext[2])
@@ -374,7 +377,7 @@ class CuspQuadrature:
         col_of = np.repeat(np.arange(ncols), per_col)
         first = col_start[col_of] + (np.arange(n_chunks) - col_chunk[col_of]) * self.lift_chunk
         last = np.minimum(first + self.lift_chunk, col_start[col_of + 1]) - 1
-        b2 = (base_host[:, 2] - lo[2]).astype(np.int64)
+        b2 = base_z.astype(np.int64) - int(lo[2])
         zlo = b2[first]
         hz = b2[last] - zlo + 8
         node = np.append(first, self.n_nodes)
