@@ -60,6 +60,21 @@ class SpectralBoundsHint:
     shift: float = 0.0  # or a DeviceScalar, read when the solve first needs it
 
 
+class DeviceArray:
+    """A device array copied to pinned host memory behind an event as soon as it is made;
+    :meth:`numpy` waits only for that copy, not for work enqueued after it."""
+
+    def __init__(self, value: torch.Tensor):
+        self._host = torch.empty(value.numel(), dtype=value.dtype).pin_memory()
+        self._host.copy_(value.reshape(-1), non_blocking=True)
+        self._ready = torch.cuda.Event()
+        self._ready.record()
+
+    def numpy(self) -> np.ndarray:
+        self._ready.synchronize()
+        return self._host.numpy()
+
+
 class DeviceScalar:
     """A device scalar copied to pinned host memory behind an event as soon as it is made; ``float()``
     waits only for that copy. Lets a caller hand the solver a value still being computed without a
@@ -149,16 +164,9 @@ def residual_norms(geometry, measure, work: RitzWork) -> None:
 
 
 def _read(work: RitzWork) -> tuple[np.ndarray, np.ndarray, float]:
-    host = work.scalars.cpu().numpy()  # the iteration's one synchronising read
-    k = work.k
-    orth = float(host[2 * k])
-    # A failed Cholesky zeroes its inverse, so the Ritz vectors vanish and max|G - I| is 1: only then
-    # is the flag worth a second read.
-    if not orth < 0.5 and int(work.flag.cpu()[0]) != 0:
-        raise RuntimeError(
-            "Rayleigh-Ritz: the filtered block's Gram matrix is not positive definite (the block has "
-            "collapsed in rank); the eigh fallback of the reference that drops vectors is not implemented")
-    return host[:k].copy(), host[k:2 * k].copy(), orth
+    """The pass's scalars (eigenvalues, residual norms, orthonormality error), read now."""
+    _read_async(work)
+    return _collect(work)
 
 
 def _read_async(work: RitzWork) -> None:

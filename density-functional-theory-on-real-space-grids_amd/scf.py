@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import _lib, ops
-from .eigen import ChebyshevFilteredSubspace, DeviceScalar, EigenConfig, SpectralBoundsHint
+from .eigen import ChebyshevFilteredSubspace, DeviceArray, DeviceScalar, EigenConfig, SpectralBoundsHint
 from .geometry import CuspFactor, CuspQuadrature, UniformGrid
 from .hamiltonian import CuspFactoredHamiltonian, SeededDraws, prefetch_draws
 from .ops import stream_of
@@ -449,8 +449,11 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
     iteration = 0
     norm_buf = torch.empty(3, dtype=torch.float64, device=g.device)
     tmp = g.empty()
+    pin_next = None  # the next iteration's input potentials, enqueued before this one's read
+    speculated = {"used": 0, "discarded": 0}
     for iteration in range(1, numerics.max_scf + 1):
-        pin = step.potentials(n_in)
+        pin = pin_next if pin_next is not None else step.potentials(n_in)
+        pin_next = None
         h = step.hamiltonian(pin)
         steps = None if initial_block is None else FILTER_STEPS_PER_ITERATION
         reason = policy.decide(worsened > 0, rung_changed)
@@ -480,7 +483,12 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
         ops.ewise(ops.EW_SUB, tmp, n_next[0], n_in[0])
         ops.ewise(ops.EW_DIV, tmp, tmp, step.f2)
         ops.reduce(ops.RED_L1, tmp, norm_buf, 1, w=q.mass_weights)
-        host = torch.cat([escal, pin["scalars"], pout["scalars"], norm_buf[:2]]).cpu().numpy()  # one read
+        read = DeviceArray(torch.cat([escal, pin["scalars"], pout["scalars"], norm_buf[:2]]))  # one read
+        # The next iteration's potentials of the mixed density go on the stream before the read;
+        # they are dropped if the solve stops here or the mixing rung changes.
+        if iteration < numerics.max_scf:
+            pin_next = step.potentials(n_next)
+        host = read.numpy()
         k = eigen.eigenvectors.shape[0]
         ne = k * k + 3
         e_in, e_out = host[ne:ne + 3], host[ne + 3:ne + 6]
@@ -492,6 +500,7 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
         de = abs(energies[-1] - energies[-2]) if len(energies) > 1 else float("inf")
         if iteration >= numerics.min_scf and de < numerics.energy_tol and res_norm < numerics.density_tol:
             converged = True
+            speculated["discarded"] += pin_next is not None
             break
         worsened = worsened + 1 if res_norm > previous_residual else 0
         previous_residual = res_norm
@@ -503,6 +512,9 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
             events.append(f"iteration {iteration}: residual grew three times, rung {rung}")
             n_next = torch.empty_like(n_in)
             ops.lincomb(n_next.view(-1), [(n_in.reshape(-1), 1.0), (residual, 0.5 * numerics.alpha)])
+            speculated["discarded"] += pin_next is not None
+            pin_next = None
+        speculated["used"] += pin_next is not None
         n_in = n_next
     h_final = step.hamiltonian(pin)
     # The final diagonalisation is float64 throughout; a mixed solve refines its float32-filtered
@@ -532,6 +544,7 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
         "filter_precisions": list(solver.filter_precisions),
         "graphs": dict(solver.graph_record),
         "speculative_filters": dict(solver.speculation),
+        "speculative_potentials": speculated,
         "wall_seconds": time.perf_counter() - started,
         "step": step,
         "effective_potential": pin["effective"],
