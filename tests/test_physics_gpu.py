@@ -262,3 +262,24 @@ def test_captured_potentials_and_energies_match_eager(pkg, device):
     assert captured.graph_record["captures"] == 2
     assert captured.graph_record["replays"] == 2 * 4
     assert eager.graph_record["replays"] == 0
+
+
+def test_speculative_filters_change_nothing(pkg, device, monkeypatch):
+    """The CheFSI solve enqueues the next pass's filter before reading a pass when that pass cannot
+    stop the solve; with the speculation off, an SCF gives bitwise the same energies, eigenvalues and
+    iteration counts (He LDA at h = 0.5, bound reuse and float32 warm passes included)."""
+    from cdft_amd import eigen, scf
+
+    sc = ophys.Scenario((2.0,), ((0.0, 0.0, 0.0),), 2.0, "lda")
+    num = ophys.Numerics(box_lengths=(16.0, 16.0, 16.0), spacing=0.5)
+    runs = {}
+    for label, flag, dtype in (("on", True, torch.float64), ("off", False, torch.float64),
+                               ("on32", True, torch.float32), ("off32", False, torch.float32)):
+        monkeypatch.setattr(eigen.ChebyshevFilteredSubspace, "speculate", flag)
+        runs[label] = scf.solve_self_consistent(sc, num, device, filter_dtype=dtype)
+    for a, b in (("on", "off"), ("on32", "off32")):
+        assert runs[a]["energy_history"] == runs[b]["energy_history"]
+        assert np.array_equal(runs[a]["eigenvalues"], runs[b]["eigenvalues"])
+        assert runs[a]["applications"] == runs[b]["applications"]
+    assert runs["on"]["speculative_filters"]["used"] > 0
+    assert runs["off"]["speculative_filters"]["used"] == 0
