@@ -213,7 +213,13 @@ __device__ __forceinline__ void stencil_block(
   auto load_plane = [&](int m, Plane& L) {
     const int base = m * plane * S;
     L.wxf = bload<T>(r_w0, col, (m + 2 * P - 1) * plane * S);
-    L.phn = (m + 2 * P < n0) ? bload<T>(r_phi, col, (m + 2 * P) * plane * S) : T(0);
+    {
+      // loaded unconditionally (a clamped plane, then a select): a load on only one side of a
+      // branch makes the compiler's wait counting assume the worst and wait for every load
+      const int mp = m + 2 * P < n0 ? m + 2 * P : n0 - 1;
+      const T y = bload<T>(r_phi, col, mp * plane * S);
+      L.phn = m + 2 * P < n0 ? y : T(0);
+    }
 #pragma unroll
     for (int s = 0; s < MAXH; ++s) L.hal[s] = bload<T>(r_phi, hsrc[s], base);
     L.wy0 = bload<T>(r_w1, wy0_off, m * pl1 * S);
@@ -225,14 +231,7 @@ __device__ __forceinline__ void stencil_block(
     L.pr = MODE >= 2 ? bload<T>(r_prev, col, base) : T(0);
   };
 
-  Plane cur;
-  load_plane(X0, cur);
-  for (int m = X0; m < X1; ++m) {
-    Plane nxt;
-    if (m + 1 < X1)
-      load_plane(m + 1, nxt);
-    else
-      nxt = cur;
+  auto body = [&](int m, const Plane& cur) __attribute__((always_inline)) {
 
     // newest x-face F(m+P-1) and the x part of the divergence
     {
@@ -320,7 +319,19 @@ __device__ __forceinline__ void stencil_block(
       qphi[j] = qphi[j + 1];
     }
     qphi[2 * P - 1] = cur.phn;
-    cur = nxt;
+  };
+  // Two plane buffers used in turn, the loop unrolled over them: the next plane's loads land in the
+  // other buffer and no register with a load in flight is ever copied (a copy waits for the load).
+  // The prefetch past the segment's last plane reloads that plane, so every iteration issues the
+  // same loads.
+  Plane pa, pb;
+  load_plane(X0, pa);
+  for (int m = X0; m < X1; m += 2) {
+    load_plane(m + 1 < X1 ? m + 1 : X1 - 1, pb);
+    body(m, pa);
+    if (m + 1 >= X1) break;
+    load_plane(m + 2 < X1 ? m + 2 : X1 - 1, pa);
+    body(m + 1, pb);
   }
   if constexpr (MODE == 4) {
     // fixed-order block sum: a butterfly per wavefront, then the wavefronts in order
