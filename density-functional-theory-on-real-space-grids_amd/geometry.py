@@ -204,23 +204,64 @@ def sphere_directions(n_theta: int) -> tuple[np.ndarray, np.ndarray]:
     return directions, weights
 
 
-def sphere_nodes(centre, radius_sphere, spacing, radial_order, node_spacing):
+@functools.lru_cache(maxsize=None)
+def _direction_rows(n_theta: int) -> np.ndarray:
+    """:func:`sphere_directions` as three contiguous rows ``(3, n)`` (read-only)."""
+    rows = np.ascontiguousarray(sphere_directions(n_theta)[0].T)
+    rows.setflags(write=False)
+    return rows
+
+
+def sphere_node_rows(centre, radius_sphere, spacing, radial_order, node_spacing):
     """Composite GL radius x GL(cos theta) x uniform phi nodes and bare weights ``r^2 dr dOmega``
-    (reference ``quadrature.py:129-180``). Host-side table construction."""
+    (reference ``quadrature.py:129-180``), the coordinates as three rows ``(3, N)``. All shells are
+    built at once with the same IEEE operations per shell as a loop over panels and GL nodes; rows
+    keep numpy's loops long (an ``(N, 3)`` broadcast runs a 3-element inner loop per node)."""
     n_panels = max(2, int(math.ceil(radius_sphere / spacing)))
     edges = np.linspace(0.0, radius_sphere, n_panels + 1)
     gl_nodes, gl_weights = gauss_legendre(radial_order)
-    points, weights = [], []
-    for lo, hi in zip(edges[:-1], edges[1:]):
-        mid, half = 0.5 * (hi + lo), 0.5 * (hi - lo)
-        for node, weight in zip(gl_nodes, gl_weights):
-            r = mid + half * node
-            wr = half * weight * r * r
-            n_theta = max(4, int(math.ceil(math.sqrt(2.0 * math.pi) * r / (node_spacing * spacing))))
-            directions, angular = sphere_directions(n_theta)
-            points.append(r * directions)
-            weights.append(wr * angular)
-    return np.concatenate(points) + np.asarray(centre), np.concatenate(weights)
+    mid = (0.5 * (edges[1:] + edges[:-1]))[:, None]
+    half = (0.5 * (edges[1:] - edges[:-1]))[:, None]
+    r = (mid + half * gl_nodes[None, :]).ravel()
+    wr = (half * gl_weights[None, :]).ravel() * r * r
+    n_theta = np.maximum(4, np.ceil(math.sqrt(2.0 * math.pi) * r / (node_spacing * spacing)).astype(np.int64)).tolist()
+    counts = np.fromiter((2 * n * n for n in n_theta), dtype=np.int64, count=len(n_theta))
+    rows = np.concatenate([_direction_rows(n) for n in n_theta], axis=1) * np.repeat(r, counts)
+    rows += np.asarray(centre, dtype=np.float64)[:, None]
+    weights = np.repeat(wr, counts) * np.concatenate([sphere_directions(n)[1] for n in n_theta])
+    return rows, weights
+
+
+def sphere_nodes(centre, radius_sphere, spacing, radial_order, node_spacing):
+    """:func:`sphere_node_rows` with the coordinates as ``(N, 3)`` (a transposed view)."""
+    rows, weights = sphere_node_rows(centre, radius_sphere, spacing, radial_order, node_spacing)
+    return rows.T, weights
+
+
+def node_table(centres, radii, spacing, radial_order, node_spacings, origin) -> dict:
+    """The quadrature's nodes, numbered in the order of their interpolation base cell (stable), so
+    the nodes a grid point gathers from in the transpose are contiguous in memory (only the order of
+    the node sums changes): coordinate rows ``(3, m)``, owners, bare weights, base-cell rows ``(3,
+    m)`` (``floor((p - origin) / h)``), the permutation from the sphere-by-sphere enumeration, the
+    cell box (``lo``, ``ext``) and the node count per cell of the box."""
+    rows_list, w_list, owner_list = [], [], []
+    for atom, centre in enumerate(centres):
+        p, w = sphere_node_rows(centre, radii[atom], spacing, radial_order, node_spacings[atom])
+        rows_list.append(p)
+        w_list.append(w)
+        owner_list.append(np.full(p.shape[1], atom, dtype=np.int32))
+    rows = np.concatenate(rows_list, axis=1)
+    owners = np.concatenate(owner_list)
+    bare = np.concatenate(w_list)
+    base = np.floor((rows - np.asarray(origin, dtype=np.float64)[:, None]) / spacing).astype(np.int32)
+    lo = base.min(axis=1)
+    ext = base.max(axis=1) - lo + 1
+    key = ((base[0] - lo[0]).astype(np.int64) * int(ext[1]) + (base[1] - lo[1])) * int(ext[2]) + (base[2] - lo[2])
+    # 16-bit keys (cell boxes up to 65,536 cells) take numpy's radix sort; the same stable order
+    perm = np.argsort(key.astype(np.uint16) if int(ext.prod()) <= 65536 else key, kind="stable")
+    return {"point_rows": np.take(rows, perm, axis=1), "owners": owners[perm], "bare_weights": bare[perm],
+            "base_rows": np.take(base, perm, axis=1), "perm": perm, "lo": lo, "ext": ext,
+            "counts": np.bincount(key, minlength=int(ext.prod()))}
 
 
 class CuspQuadrature:
@@ -258,7 +299,7 @@ class CuspQuadrature:
         upper = origin + np.array([(n - 1) * h for n in grid.shape])
         margin = (self.degree // 2 + 1.5) * h
         self.taper, self.overlapping, spheres = [], [], []
-        pts_list, w_list, owner_list = [], [], []
+        radii, node_spacings = [], []
         for atom in range(n_atoms):
             centre = positions[atom]
             edge_distance = float(np.minimum(centre - origin, upper - centre).min())
@@ -278,33 +319,18 @@ class CuspQuadrature:
                 others = np.concatenate([positions[:atom], positions[atom + 1:]])
                 overlapping = float(np.sqrt(((others - centre) ** 2).sum(-1)).min()) < 2.0 * radius
             self.overlapping.append(overlapping)
-            p, w = sphere_nodes(centre, radius, h, self.radial_order,
-                                self.node_spacing_overlapping if overlapping else self.node_spacing)
-            pts_list.append(p)
-            w_list.append(w)
-            owner_list.append(np.full(p.shape[0], atom, dtype=np.int32))
+            radii.append(radius)
+            node_spacings.append(self.node_spacing_overlapping if overlapping else self.node_spacing)
             spheres.append((*centre, radius, tc, width))
         self.spheres = spheres
         self._spheres_c = host_doubles([v for s in spheres for v in s])
-        points = np.concatenate(pts_list)
-        owners = np.concatenate(owner_list)
-        bare_weights = np.concatenate(w_list)
-        # Number the nodes in the order of their interpolation base cell (stable), so the nodes a
-        # grid point gathers from in the transpose are contiguous in memory. Only the order of the
-        # node sums changes.
-        base_key = np.floor((points - origin) / h).astype(np.int32)
-        lo_key = base_key.min(axis=0)
-        ext_key = base_key.max(axis=0) - lo_key + 1
-        rel = base_key - lo_key
-        key = (rel[:, 0].astype(np.int64) * int(ext_key[1]) + rel[:, 1]) * int(ext_key[2]) + rel[:, 2]
-        # the same stable order; 16-bit keys (cell boxes up to 65,536 cells) take numpy's radix sort
-        perm = np.argsort(key.astype(np.uint16) if int(ext_key.prod()) <= 65536 else key, kind="stable")
-        points, owners, bare_weights = points[perm], owners[perm], bare_weights[perm]
-        self.n_nodes = m = points.shape[0]
+        table = node_table([positions[a] for a in range(n_atoms)], radii, h, self.radial_order, node_spacings, origin)
+        perm, lo, ext = table["perm"], table["lo"], table["ext"]
+        self.n_nodes = m = perm.shape[0]
         as_dev = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)  # noqa: E731
-        self.points = as_dev(points)
-        owner = as_dev(owners.astype(np.int32), torch.int32)
-        base_w = as_dev(bare_weights)
+        self.points = as_dev(table["point_rows"]).t().contiguous()  # (m, 3) on the device
+        owner = as_dev(table["owners"], torch.int32)
+        base_w = as_dev(table["bare_weights"])
         self.weights = torch.empty(m, dtype=torch.float64, device=dev)
         self.f2 = torch.empty(m, dtype=torch.float64, device=dev)
         self.grad_u = torch.empty((3, m), dtype=torch.float64, device=dev)
@@ -323,19 +349,18 @@ class CuspQuadrature:
         _lib.call("cdft_node_stencils_f64", self.points.data_ptr(), m, grid.c_shape, h, grid.c_origin,
                   self.base.data_ptr(), self.lw.data_ptr(), bad.data_ptr(), s)
         # The kernel's base cells are floor((p - origin) / h) in the same IEEE operations as the host's
-        # ordering key above, so the binning is built from the host copy without a device read.
+        # ordering key, so the binning is built from the host copy without a device read.
         dims = np.asarray(grid.shape)
-        if (lo_key - 3 < 0).any() or (lo_key + ext_key - 1 + 4 >= dims).any():
+        if (lo - 3 < 0).any() or (lo + ext - 1 + 4 >= dims).any():
             raise ValueError("an interpolation stencil reaches outside the grid; the sphere radius bound is wrong")
-        self._base_key, self._perm = base_key, perm
-        lo, ext = lo_key, ext_key  # the same cells, so the same cell box
-        counts = np.bincount(key, minlength=int(ext.prod()))  # nodes per cell (order-free)
+        self._base_rows = table["base_rows"]
+        counts = table["counts"]  # nodes per cell of the cell box
         start = np.zeros(counts.size + 1, dtype=np.int32)
         np.cumsum(counts, out=start[1:])
         self.cell_start = as_dev(start, torch.int32)
         self.node_order = perm  # GPU node i is node perm[i] of the sphere-by-sphere enumeration
         self._cell_box = _ints([lo[0], lo[1], lo[2], ext[0], ext[1], ext[2]])
-        self._build_lift_chunks(base_key[perm, 2], start, lo, ext, as_dev)
+        self._build_lift_chunks(self._base_rows[2], start, lo, ext, as_dev)
         # Far part of the rule.
         self.far = grid.empty()
         _lib.call("cdft_far_factor_f64", grid.c_shape, h, grid.c_origin, n_atoms, ctypes.addressof(self._spheres_c),
@@ -361,7 +386,7 @@ class CuspQuadrature:
     @property
     def _base_host(self) -> np.ndarray:
         """The nodes' base cells (``(m, 3)`` int32, node order), as the stencil kernel computes them."""
-        return self._base_key[self._perm]
+        return self._base_rows.T
 
     def _build_lift_chunks(self, base_z, cell_start, lo, ext, as_dev) -> None:
         """Split every z-column of the cell box into chunks of at most ``lift_chunk`` consecutive

@@ -102,3 +102,48 @@ def test_lumped_axis_tables_match_the_oracle_matrices(pkg):
     xc = origin + h * np.arange(n)
     p = lambda x: 1.0 + 0.3 * x - 0.05 * x**2 + 0.01 * x**3  # noqa: E731
     assert np.allclose(mat.T @ p(xc), p(fine), rtol=0, atol=1e-12)
+
+
+def _loop_node_table(gm, centres, radius, h, node_spacing, origin):
+    """The node table built shell by shell with (N, 3) arrays, as a plain loop."""
+    points, weights, owners = [], [], []
+    for atom, centre in enumerate(centres):
+        n_panels = max(2, int(np.ceil(radius / h)))
+        edges = np.linspace(0.0, radius, n_panels + 1)
+        gl_nodes, gl_weights = gm.gauss_legendre(6)
+        for lo, hi in zip(edges[:-1], edges[1:]):
+            mid, half = 0.5 * (hi + lo), 0.5 * (hi - lo)
+            for node, weight in zip(gl_nodes, gl_weights):
+                r = mid + half * node
+                wr = half * weight * r * r
+                n_theta = max(4, int(np.ceil(np.sqrt(2.0 * np.pi) * r / (node_spacing * h))))
+                directions, angular = gm.sphere_directions(n_theta)
+                points.append(r * directions + np.asarray(centre))
+                weights.append(wr * angular)
+                owners.append(np.full(directions.shape[0], atom, dtype=np.int32))
+    points, weights, owners = np.concatenate(points), np.concatenate(weights), np.concatenate(owners)
+    base = np.floor((points - origin) / h).astype(np.int32)
+    lo = base.min(axis=0)
+    ext = base.max(axis=0) - lo + 1
+    rel = base - lo
+    key = (rel[:, 0].astype(np.int64) * int(ext[1]) + rel[:, 1]) * int(ext[2]) + rel[:, 2]
+    perm = np.argsort(key, kind="stable")
+    return points[perm], owners[perm], weights[perm], base[perm], perm, lo, ext, np.bincount(key, minlength=int(ext.prod()))
+
+
+def test_node_table_is_the_shell_loop(pkg):
+    """The vectorised node table (coordinate rows, one radix sort) equals, bitwise, the table built
+    shell by shell and sorted by a 64-bit stable sort, for one and two nuclei."""
+    from cdft_amd import geometry as gm
+    origin = np.array([-8.0, -8.0, -8.0])
+    for centres, radius, h, ns in (([(0.0, 0.0, 0.0)], 2.875, 0.25, 1.0),
+                                   ([(0.0, 0.0, -0.7), (0.0, 0.0, 0.7)], 2.0, 0.2, 0.7),
+                                   ([(0.3, -0.2, 0.1)], 1.1, 0.3, 1.0)):
+        ref = _loop_node_table(gm, centres, radius, h, ns, origin)
+        t = gm.node_table(centres, [radius] * len(centres), h, 6, [ns] * len(centres), origin)
+        got = (t["point_rows"].T, t["owners"], t["bare_weights"], t["base_rows"].T, t["perm"], t["lo"], t["ext"],
+               t["counts"])
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b)
+        points, weights = gm.sphere_nodes(centres[0], radius, h, 6, ns)
+        assert points.shape == (weights.shape[0], 3)
