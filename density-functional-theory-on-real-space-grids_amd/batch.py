@@ -268,9 +268,8 @@ class BatchedCheFSI:
             if self.use_graphs:
                 graph = graphs.get(use32)
                 if graph is None:
-                    graph = torch.cuda.CUDAGraph()
-                    with torch.cuda.graph(graph):
-                        self._iteration_body(insts, body_filter, plan_rr, plan_res, degree, use32)
+                    graph = ops.capture_graph(
+                        lambda: self._iteration_body(insts, body_filter, plan_rr, plan_res, degree, use32))
                     graphs[use32] = graph
                     self.graph_record["captures"] += 1
                 graph.replay()

@@ -284,9 +284,7 @@ class ChebyshevFilteredSubspace:
             graph = self._graphs.get(key)
             if graph is None:
                 try:
-                    graph = torch.cuda.CUDAGraph()
-                    with torch.cuda.graph(graph):
-                        body()
+                    graph = ops.capture_graph(body)
                     self._graphs[key] = graph
                     self.graph_record["captures"] += 1
                 except Exception as exc:  # noqa: BLE001 - recorded; the iteration then runs eagerly

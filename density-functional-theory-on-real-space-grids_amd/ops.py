@@ -28,6 +28,31 @@ def _ptr(t: torch.Tensor | None) -> int | None:
     return None if t is None else t.data_ptr()
 
 
+_capture_streams: dict = {}
+
+
+def capture_graph(body) -> torch.cuda.CUDAGraph:
+    """``body``'s launches captured into a HIP graph (replayed later on the current stream).
+
+    Captures on a side stream of the current device without draining the device first (torch's
+    ``torch.cuda.graph`` synchronises the device and empties the allocator cache before every
+    capture, which leaves the GPU idle while the host records), in thread-local mode, so work that
+    host threads enqueue meanwhile (pinned draws) cannot invalidate it. The graph's allocations come
+    from its own private pool, as with ``torch.cuda.graph``."""
+    device = torch.cuda.current_device()
+    stream = _capture_streams.get(device)
+    if stream is None:
+        stream = _capture_streams[device] = torch.cuda.Stream(device)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(stream):
+        graph.capture_begin(capture_error_mode="thread_local")
+        try:
+            body()
+        finally:
+            graph.capture_end()
+    return graph
+
+
 def require(t: torch.Tensor, shape: tuple, dtype: torch.dtype, device: torch.device, name: str) -> None:
     if t.device != device:
         raise ValueError(f"{name} is on {t.device}, expected {device}")

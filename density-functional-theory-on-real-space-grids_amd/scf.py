@@ -136,9 +136,7 @@ class KohnShamStep:
             return
         if entry == "eager":
             try:
-                graph = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(graph):
-                    body()
+                graph = ops.capture_graph(body)
                 self.graphs[key] = graph
                 self.graph_record["captures"] += 1
             except Exception as exc:  # noqa: BLE001 - recorded; the body then runs eagerly
