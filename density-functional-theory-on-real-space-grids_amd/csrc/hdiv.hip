@@ -16,6 +16,8 @@
 // loads of plane m+1 are issued before the LDS work of plane m, so their latency overlaps it.
 // Tile extents are chosen per grid (ceil(n / ceil(n / 16))), so n = 65 runs 13-wide tiles rather
 // than four full tiles and a one-column tail.
+#include <type_traits>
+
 #include "common.h"
 
 namespace cdft {
@@ -231,21 +233,24 @@ __device__ __forceinline__ void stencil_block(
     L.pr = MODE >= 2 ? bload<T>(r_prev, col, base) : T(0);
   };
 
-  auto body = [&](int m, const Plane& cur) __attribute__((always_inline)) {
+  auto body = [&](auto rot, int m, const Plane& cur) __attribute__((always_inline)) {
+    // queue entry i of this plane (i = 0 the oldest) lives in slot (SR + i) % 2P
+    constexpr int SR = decltype(rot)::value;
+#define QI(i) (((SR) + (i)) % (2 * P))
 
     // newest x-face F(m+P-1) and the x part of the divergence
     {
       T f = 0;
 #pragma unroll
-      for (int i = 0; i < 2 * P; ++i) f += c[i] * qphi[i];
-      qf[2 * P - 1] = f * cur.wxf;
+      for (int i = 0; i < 2 * P; ++i) f += c[i] * qphi[QI(i)];
+      qf[QI(2 * P - 1)] = f * cur.wxf;
     }
     T ax = 0;
 #pragma unroll
-    for (int i = 0; i < 2 * P; ++i) ax += c[i] * qf[2 * P - 1 - i];
+    for (int i = 0; i < 2 * P; ++i) ax += c[i] * qf[QI(2 * P - 1 - i)];
 
     // stage plane m
-    if (act) s_phi[(ty + R) * LZ + tz + R] = qphi[0];
+    if (act) s_phi[(ty + R) * LZ + tz + R] = qphi[QI(0)];
 #pragma unroll
     for (int s = 0; s < MAXH; ++s)
       if (hdst[s] >= 0) s_phi[hdst[s]] = cur.hal[s];
@@ -300,7 +305,7 @@ __device__ __forceinline__ void stencil_block(
 #pragma unroll
       for (int i = 0; i < 2 * P; ++i) az += c[i] * s_zf[ty * ZP + tz + R - i];
       const T lap = (ax + ay) + az;
-      const T pv = qphi[0];
+      const T pv = qphi[QI(0)];
       T val = MODE == 3 ? lap * T(0.5) + cur.pt * cur.pr : lap * T(0.5) * cur.iw + pv * cur.pt;
       if (MODE == 1) val = (val - cc * pv) * ca;
       if (MODE == 2) val = (val - cc * pv) * ca - cb * cur.pr;
@@ -313,25 +318,57 @@ __device__ __forceinline__ void stencil_block(
     }
 
     // advance the x queues and the plane pipeline
-#pragma unroll
-    for (int j = 0; j < 2 * P - 1; ++j) {
-      qf[j] = qf[j + 1];
-      qphi[j] = qphi[j + 1];
-    }
-    qphi[2 * P - 1] = cur.phn;
+    qphi[QI(0)] = cur.phn;  // becomes logical index 2P-1 of the next plane
+#undef QI
   };
-  // Two plane buffers used in turn, the loop unrolled over them: the next plane's loads land in the
-  // other buffer and no register with a load in flight is ever copied (a copy waits for the load).
-  // The prefetch past the segment's last plane reloads that plane, so every iteration issues the
-  // same loads.
+  // Two plane buffers used in turn: the next plane's loads land in the other buffer and no register
+  // with a load in flight is ever copied (a copy waits for the load). The prefetch past the segment's
+  // last plane reloads that plane, so every iteration issues the same loads. The loop is unrolled
+  // over the x queues' period 2P, so the queues rotate by renaming (the body's compile-time `rot`)
+  // instead of 4P-2 register moves per plane.
+  static_assert(2 * P <= 8, "the unrolled plane loop covers queue periods up to 8");
   Plane pa, pb;
   load_plane(X0, pa);
-  for (int m = X0; m < X1; m += 2) {
-    load_plane(m + 1 < X1 ? m + 1 : X1 - 1, pb);
-    body(m, pa);
-    if (m + 1 >= X1) break;
-    load_plane(m + 2 < X1 ? m + 2 : X1 - 1, pa);
-    body(m + 1, pb);
+  for (int m = X0; m < X1; m += 2 * P) {
+    if constexpr (0 < 2 * P) {
+      load_plane(m + 1 < X1 ? m + 1 : X1 - 1, pb);
+      body(std::integral_constant<int, 0>(), m + 0, pa);
+    }
+    if constexpr (1 < 2 * P) {
+      if (m + 1 >= X1) break;
+      load_plane(m + 2 < X1 ? m + 2 : X1 - 1, pa);
+      body(std::integral_constant<int, 1>(), m + 1, pb);
+    }
+    if constexpr (2 < 2 * P) {
+      if (m + 2 >= X1) break;
+      load_plane(m + 3 < X1 ? m + 3 : X1 - 1, pb);
+      body(std::integral_constant<int, 2>(), m + 2, pa);
+    }
+    if constexpr (3 < 2 * P) {
+      if (m + 3 >= X1) break;
+      load_plane(m + 4 < X1 ? m + 4 : X1 - 1, pa);
+      body(std::integral_constant<int, 3>(), m + 3, pb);
+    }
+    if constexpr (4 < 2 * P) {
+      if (m + 4 >= X1) break;
+      load_plane(m + 5 < X1 ? m + 5 : X1 - 1, pb);
+      body(std::integral_constant<int, 4>(), m + 4, pa);
+    }
+    if constexpr (5 < 2 * P) {
+      if (m + 5 >= X1) break;
+      load_plane(m + 6 < X1 ? m + 6 : X1 - 1, pa);
+      body(std::integral_constant<int, 5>(), m + 5, pb);
+    }
+    if constexpr (6 < 2 * P) {
+      if (m + 6 >= X1) break;
+      load_plane(m + 7 < X1 ? m + 7 : X1 - 1, pb);
+      body(std::integral_constant<int, 6>(), m + 6, pa);
+    }
+    if constexpr (7 < 2 * P) {
+      if (m + 7 >= X1) break;
+      load_plane(m + 8 < X1 ? m + 8 : X1 - 1, pa);
+      body(std::integral_constant<int, 7>(), m + 7, pb);
+    }
   }
   if constexpr (MODE == 4) {
     // fixed-order block sum: a butterfly per wavefront, then the wavefronts in order
