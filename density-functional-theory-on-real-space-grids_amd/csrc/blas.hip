@@ -23,19 +23,31 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block-wide sum of `count` values per thread (count <= KMAX*KMAX); result valid in thread 0.
-template <int COUNT>
-__device__ __forceinline__ void block_sums(double (&acc)[COUNT], double* shared /* [4][COUNT] */) {
+// Block-wide sums of COUNT values per thread (a block of RB = 256 threads), each handed to
+// store(entry, sum) by lane 0 of the wavefront that forms it. The values go through LDS by entry
+// (row e holds every thread's value e), then wavefront w reduces entries w, w + 4, ...: lane l adds
+// threads l, l + 64, l + 128, l + 192 in that order, then a fixed butterfly. A butterfly per entry
+// and wavefront followed by a cross-wavefront pass, the obvious alternative, spends 25 butterflies
+// per wavefront on a 5 x 5 Gram matrix instead of 7 and took about 5 us more per launch. Entries go
+// through the tile GROUP at a time, which bounds its LDS.
+template <int COUNT, typename Store>
+__device__ __forceinline__ void block_partials(const double (&acc)[COUNT], Store store) {
+  static_assert(RB == 256, "four wavefronts per block");
+  constexpr int GROUP = COUNT < 32 ? COUNT : 32;
+  __shared__ double tile[GROUP][RB];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int i = 0; i < COUNT; ++i) {
-    double v = wave_sum(acc[i]);
-    if (lane == 0) shared[wave * COUNT + i] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
+  for (int g0 = 0; g0 < COUNT; g0 += GROUP) {
+    if (g0 > 0) __syncthreads();  // the previous group's reads are done
 #pragma unroll
-    for (int i = 0; i < COUNT; ++i) acc[i] = ((shared[i] + shared[COUNT + i]) + shared[2 * COUNT + i]) + shared[3 * COUNT + i];
+    for (int e = 0; e < GROUP; ++e)
+      if (g0 + e < COUNT) tile[e][threadIdx.x] = acc[g0 + e];
+    __syncthreads();
+    for (int e = wave; e < GROUP && g0 + e < COUNT; e += RB / 64) {
+      double v = ((tile[e][lane] + tile[e][lane + 64]) + tile[e][lane + 128]) + tile[e][lane + 192];
+      v = wave_sum(v);
+      if (lane == 0) store(g0 + e, v);
+    }
   }
 }
 
@@ -45,7 +57,6 @@ __global__ __launch_bounds__(RB) void wgram_partial_kernel(const double* __restr
                                                            const double* __restrict__ b, int kb,
                                                            const double* __restrict__ q, long n,
                                                            double* __restrict__ partials) {
-  __shared__ double shared[4 * KA * KB];
   double acc[KA * KB];
 #pragma unroll
   for (int i = 0; i < KA * KB; ++i) acc[i] = 0.0;
@@ -87,14 +98,11 @@ __global__ __launch_bounds__(RB) void wgram_partial_kernel(const double* __restr
       }
     }
   }
-  block_sums<KA * KB>(acc, shared);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < KA; ++i)
-#pragma unroll
-      for (int j = 0; j < KB; ++j)
-        if (i < ka && j < kb) partials[(long)blockIdx.x * KMAX * KMAX + i * KMAX + j] = acc[i * KB + j];
-  }
+  double* row = partials + (long)blockIdx.x * KMAX * KMAX;
+  block_partials<KA * KB>(acc, [&](int e, double v) {
+    const int i = e / KB, j = e - (e / KB) * KB;
+    if (i < ka && j < kb) row[i * KMAX + j] = v;
+  });
 }
 
 // One wavefront per output entry: lane l sums partials l, l+64, ... in order, then a fixed
@@ -163,7 +171,6 @@ __global__ __launch_bounds__(RB) void rotate_gram_kernel(const double* __restric
                                                          double* __restrict__ y, const double* __restrict__ q, long n,
                                                          double* __restrict__ partials) {
   __shared__ double sr[KROT * KROT];
-  __shared__ double shared[4 * K * K];
   for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
     const int i = e / K, j = e % K;  // output i, input j
     sr[i * KROT + j] = r[j * K + i];
@@ -206,13 +213,8 @@ __global__ __launch_bounds__(RB) void rotate_gram_kernel(const double* __restric
       }
     }
   }
-  block_sums<K * K>(acc, shared);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < K; ++i)
-#pragma unroll
-      for (int j = 0; j < K; ++j) partials[(long)blockIdx.x * KMAX * KMAX + i * KMAX + j] = acc[i * K + j];
-  }
+  double* row = partials + (long)blockIdx.x * KMAX * KMAX;
+  block_partials<K * K>(acc, [&](int e, double v) { row[(e / K) * KMAX + e % K] = v; });
 }
 
 // ---- linear combination ----
@@ -248,7 +250,6 @@ __global__ __launch_bounds__(RB) void resnorm_partial_kernel(const double* __res
                                                              const double* __restrict__ evals,
                                                              const double* __restrict__ q, int k,
                                                              long n, double* __restrict__ partials) {
-  __shared__ double shared[4 * KMAX];
   double acc[KMAX];
   double e[KMAX];
 #pragma unroll
@@ -267,12 +268,10 @@ __global__ __launch_bounds__(RB) void resnorm_partial_kernel(const double* __res
       }
     }
   }
-  block_sums<KMAX>(acc, shared);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < KMAX; ++i)
-      if (i < k) partials[(long)blockIdx.x * KMAX * KMAX + i * KMAX] = acc[i];
-  }
+  double* row = partials + (long)blockIdx.x * KMAX * KMAX;
+  block_partials<KMAX>(acc, [&](int i, double v) {
+    if (i < k) row[i * KMAX] = v;
+  });
 }
 
 // ---- small dense kernels (one thread: k is the subspace size, <= KROT) ----
