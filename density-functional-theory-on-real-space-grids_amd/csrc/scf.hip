@@ -72,6 +72,7 @@ __global__ void ewise_kernel(int op, const double* __restrict__ a, const double*
     case 8: r = s; break;                           // fill
     case 9: r = a[p] + b[p]; break;                 // a + b
     case 10: r = a[p] * ((b[p] * c[p] + b[len + p] * c[len + p]) + b[2 * len + p] * c[2 * len + p]); break;  // a (b . c)
+    case 11: r = (a[p] * b[p]) * c[p]; out[len + p] = (a[p] * b[len + p]) * c[p]; break;  // (a b_r) c, two rows
     default: r = 0.0;
   }
   out[p] = r;
@@ -273,9 +274,10 @@ extern "C" int cdft_lda_vwn_f64(const double* n, long len, double* e, double* v,
 
 extern "C" int cdft_ewise_f64(int op, const double* a, const double* b, const double* c, double s,
                               double* out, long len, void* stream) {
-  if (!out || len < 1 || op < 0 || op > 10) return fail(CDFT_E_ARG, "ewise: bad arguments (op %d)", op);
-  const bool need_a = op != 8, need_b = (op == 0 || op == 1 || op == 3 || op == 5 || op == 6 || op == 9 || op == 10);
-  const bool need_c = (op == 1 || op == 5 || op == 10);
+  if (!out || len < 1 || op < 0 || op > 11) return fail(CDFT_E_ARG, "ewise: bad arguments (op %d)", op);
+  const bool need_a = op != 8,
+             need_b = (op == 0 || op == 1 || op == 3 || op == 5 || op == 6 || op == 9 || op == 10 || op == 11);
+  const bool need_c = (op == 1 || op == 5 || op == 10 || op == 11);
   if ((need_a && !a) || (need_b && !b) || (need_c && !c)) return fail(CDFT_E_ARG, "ewise: missing operand (op %d)", op);
   hipLaunchKernelGGL(ewise_kernel, dim3(nblk(len)), dim3(256), 0, (hipStream_t)stream, op, a, b, c, s, out, len);
   return check_launch("ewise");

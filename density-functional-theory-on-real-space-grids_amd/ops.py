@@ -362,18 +362,24 @@ def orth_error(m: torch.Tensor, out: torch.Tensor) -> None:
 # Pointwise kernels and scalar reductions of the Kohn-Sham step (float64)
 # ------------------------------------------------------------------------------------------------
 
-EW_MUL, EW_MUL3, EW_SDIV, EW_DIV, EW_SCALE, EW_SUMDIV, EW_SUB, EW_ROUNDTRIP, EW_FILL, EW_ADD, EW_DOT3 = range(11)
+EW_MUL, EW_MUL3, EW_SDIV, EW_DIV, EW_SCALE, EW_SUMDIV, EW_SUB, EW_ROUNDTRIP, EW_FILL, EW_ADD, EW_DOT3, EW_MUL3_ROWS = range(12)
 
 
 def ewise(op: int, out: torch.Tensor, a: torch.Tensor | None = None, b: torch.Tensor | None = None,
           c: torch.Tensor | None = None, s: float = 0.0) -> torch.Tensor:
     """Pointwise expression (op codes above); ``EW_DOT3`` is ``a * (b . c)`` with ``b, c`` of shape
-    ``(3, len)``."""
+    ``(3, len)``; ``EW_MUL3_ROWS`` is ``(a b_r) c`` for both rows of ``b`` and ``out`` (shape ``(2, len)``)."""
+    rows = 2 if op == EW_MUL3_ROWS else 1
+    if out.numel() % rows:
+        raise ValueError("ewise: the output must hold two rows")
+    length = out.numel() // rows
     for i, t in enumerate((a, b, c)):
-        size = out.numel() * (3 if (op == EW_DOT3 and i > 0) else 1)
+        size = length * (3 if (op == EW_DOT3 and i > 0) else (rows if i == 1 else 1))
         if t is not None and (t.numel() != size or t.dtype != torch.float64 or not t.is_contiguous()):
             raise ValueError("ewise: operands must be contiguous float64 arrays of the output's size")
-    _lib.call("cdft_ewise_f64", op, _ptr(a), _ptr(b), _ptr(c), float(s), out.data_ptr(), out.numel(), stream_of(out))
+    if not out.is_contiguous():
+        raise ValueError("ewise: the output must be contiguous")
+    _lib.call("cdft_ewise_f64", op, _ptr(a), _ptr(b), _ptr(c), float(s), out.data_ptr(), length, stream_of(out))
     return out
 
 

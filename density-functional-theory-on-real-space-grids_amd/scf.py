@@ -94,18 +94,19 @@ class KohnShamStep:
     def _alloc(self):
         g, m = self.grid, self.q.n_nodes
         f = dict(dtype=torch.float64, device=g.device)
-        self.rho = torch.empty(g.n_points, **f)
-        self.n_smooth = torch.empty(g.n_points, **f)
-        self.v_smooth = torch.empty(g.n_points, **f)
-        self.v_h_grid = torch.empty(g.n_points, **f)
+        # the smooth Hartree potential and rho are the two rows the nodes interpolate in one launch
         self.interp_in = torch.empty((2, g.n_points), **f)
+        self.v_smooth, self.rho = self.interp_in[0], self.interp_in[1]
+        self.n_smooth = torch.empty(g.n_points, **f)
+        # Hartree and XC potentials as two rows (grid and nodes): one launch forms both weighted rows
+        self.v_grid2 = torch.empty((2, g.n_points), **f)
+        self.v_h_grid, self.v_grid = self.v_grid2[0], self.v_grid2[1]
         self.interp = torch.empty((2, m), **f)
-        self.v_h_node = torch.empty(m, **f)
+        self.v_node2 = torch.empty((2, m), **f)
+        self.v_h_node, self.v_node = self.v_node2[0], self.v_node2[1]
         self.n_node = torch.empty(m, **f)
         self.e_node = torch.empty(m, **f)
-        self.v_node = torch.empty(m, **f)
         self.e_grid = torch.empty(g.n_points, **f)
-        self.v_grid = torch.empty(g.n_points, **f)
         self.node_x = torch.empty((2, m), **f)
         self.grid_add = torch.empty((2, g.n_points), **f)
         self.tmp_node = torch.empty(m, **f)
@@ -168,8 +169,6 @@ class KohnShamStep:
         ops.atom_combine(n_grid, self.coefficients, self.cusp_grid, -1.0, self.n_smooth)
         self.poisson.solve(self.n_smooth, self.v_smooth)
         ops.atom_combine(self.v_smooth, self.coefficients, self.v1s_grid, +1.0, self.v_h_grid)
-        self.interp_in[0].copy_(self.v_smooth)
-        self.interp_in[1].copy_(self.rho)
         q.interpolate(self.interp_in, out=self.interp)
         ops.atom_combine(self.interp[0], self.coefficients, self.v1s_node, +1.0, self.v_h_node)
         ops.ewise(ops.EW_MUL, self.n_node, q.f2, self.interp[1])
@@ -177,10 +176,8 @@ class KohnShamStep:
         ops.lda_vwn(self.n_node, self.e_node, self.v_node)
         # node parts W P (f2 v), grid parts far_w (f2 v): Hartree in row 0, XC in row 1
         # (f2 v) w in one pass each: the same products as w (f2 v), so the same bits
-        ops.ewise(ops.EW_MUL3, self.node_x[0], q.f2, self.v_h_node, q.weights)
-        ops.ewise(ops.EW_MUL3, self.node_x[1], q.f2, self.v_node, q.weights)
-        ops.ewise(ops.EW_MUL3, self.grid_add[0], self.f2, self.v_h_grid, q.far_weights)
-        ops.ewise(ops.EW_MUL3, self.grid_add[1], self.f2, self.v_grid, q.far_weights)
+        ops.ewise(ops.EW_MUL3_ROWS, self.node_x, q.f2, self.v_node2, q.weights)
+        ops.ewise(ops.EW_MUL3_ROWS, self.grid_add, self.f2, self.v_grid2, q.far_weights)
         lumped = q.lift(self.node_x, add=self.grid_add, out=self.pot_lumped)
         hartree_weights, xc_weights = lumped[0], lumped[1]
         scalars = self.pot_scalars

@@ -288,7 +288,8 @@ int cdft_hydrogenic_fields_f64(const double* pts, long m, const int* shape, doub
 int cdft_lda_vwn_f64(const double* n, long len, double* e, double* v, void* stream);
 
 /* out = op(a, b, c, s) elementwise over len values: 0 a b; 1 (a b) c; 2 s / a; 3 a / b; 4 a s;
- * 5 (a + b) / c; 6 a - b; 7 (a / s) s; 8 s; 9 a + b; 10 a (b . c) with b, c of shape (3, len).
+ * 5 (a + b) / c; 6 a - b; 7 (a / s) s; 8 s; 9 a + b; 10 a (b . c) with b, c of shape (3, len);
+ * 11 (a b_r) c for the two rows r of b and out (shape (2, len)).
  * The pointwise steps of KohnShamStep.potentials and energy_terms
  * (src/cdft/scf/step.py:224-313,398-439). */
 int cdft_ewise_f64(int op, const double* a, const double* b, const double* c, double s, double* out,
