@@ -69,6 +69,7 @@ SIGNATURES: dict[str, list] = {
     "cdft_small_solve_f64": [P, P, P, I, P, P],
     "cdft_pulay_coefficients_f64": [P, I, P, I, D, P, P],
     "cdft_reduce_f64": [I, P, P, P, L, D, I, P, L, P, I, P],
+    "cdft_reduce_dots_f64": [I, P, P, P, L, P, P, P, P, L, P, P],
     "cdft_poisson_create": [I, I, I, I, I, I, P],
     "cdft_poisson_destroy": [P],
     "cdft_poisson_kernel_f64": [P, D, D, P, P],

@@ -255,6 +255,69 @@ __global__ __launch_bounds__(64) void reduce_final_kernel(int mode, const double
 
 static unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
+// Up to four dot products over one length in one pass (reduce mode 0 each: sum a b w, b and w
+// optional). For each product the per-thread order, the block combination and the final pass are
+// reduce_partial_kernel's and reduce_final_kernel's, so every result is bitwise the separate call's.
+struct Dots {
+  const double* a[4];
+  const double* b[4];
+  const double* w[4];
+};
+
+struct DotOut {
+  double scale[4];
+  int accumulate[4];
+  int slot[4];
+};
+
+template <int ND>
+__global__ __launch_bounds__(RBS) void dots_partial_kernel(Dots d, long len, double* __restrict__ partials) {
+  __shared__ double sh[ND][RBS / 64];
+  double acc[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) acc[i] = 0.0;
+  const long stride = (long)gridDim.x * RBS;
+  for (long p = (long)blockIdx.x * RBS + threadIdx.x; p < len; p += stride) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      double t = d.a[i][p];
+      if (d.b[i]) t = t * d.b[i][p];
+      if (d.w[i]) t = t * d.w[i][p];
+      acc[i] += t;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    for (int off = 32; off > 0; off >>= 1) acc[i] = acc[i] + __shfl_down(acc[i], off, 64);
+    if ((threadIdx.x & 63) == 0) sh[i][threadIdx.x >> 6] = acc[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      double r = sh[i][0];
+      for (int j = 1; j < RBS / 64; ++j) r = r + sh[i][j];
+      partials[(long)i * gridDim.x + blockIdx.x] = r;
+    }
+  }
+}
+
+// One wavefront per product.
+__global__ __launch_bounds__(256) void dots_final_kernel(int nd, const double* __restrict__ partials, int nblocks,
+                                                        DotOut o, double* __restrict__ out) {
+  const int i = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  if (i >= nd) return;
+  const double* pp = partials + (long)i * nblocks;
+  double r = 0.0;
+  for (int j = lane; j < nblocks; j += 64) r = r + pp[j];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) r = r + __shfl_xor(r, off, 64);
+  if (lane == 0) {
+    r = r * o.scale[i];
+    out[o.slot[i]] = o.accumulate[i] ? out[o.slot[i]] + r : r;
+  }
+}
+
 static int red_blocks(long n) {
   long b = (n + 1023) / 1024;
   if (b < 1) b = 1;
@@ -324,6 +387,40 @@ extern "C" int cdft_pulay_coefficients_f64(const double* gram, int ldg, const in
 }
 
 extern "C" long cdft_reduce_workspace_doubles(long len) { return red_blocks(len); }
+
+// out[slots[i]] = scale[i] * sum a_i b_i w_i (+= if accumulate[i]) for i < nd <= 4, in one pass over len
+// values; a, b, w, scale, accumulate, slots are host arrays (b[i], w[i] may be NULL). Partials: nd x
+// cdft_reduce_workspace_doubles(len) doubles. Bitwise nd cdft_reduce_f64 calls of mode 0.
+extern "C" int cdft_reduce_dots_f64(int nd, const double* const* a, const double* const* b, const double* const* w,
+                                    long len, const double* scale, const int* accumulate, const int* slots,
+                                    double* partials, long partials_len, double* out, void* stream) {
+  if (nd < 1 || nd > 4 || !a || !scale || !accumulate || !slots || !partials || !out || len < 1)
+    return fail(CDFT_E_ARG, "reduce_dots: bad arguments (nd=%d)", nd);
+  Dots d{};
+  DotOut o{};
+  for (int i = 0; i < nd; ++i) {
+    if (!a[i]) return fail(CDFT_E_ARG, "reduce_dots: product %d has no first operand", i);
+    d.a[i] = a[i];
+    d.b[i] = b ? b[i] : nullptr;
+    d.w[i] = w ? w[i] : nullptr;
+    o.scale[i] = scale[i];
+    o.accumulate[i] = accumulate[i];
+    o.slot[i] = slots[i];
+  }
+  const int blocks = red_blocks(len);
+  if (partials_len < (long)nd * blocks) return fail(CDFT_E_CAPACITY, "reduce_dots: partials workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  switch (nd) {
+    case 1: hipLaunchKernelGGL(dots_partial_kernel<1>, dim3(blocks), dim3(RBS), 0, s, d, len, partials); break;
+    case 2: hipLaunchKernelGGL(dots_partial_kernel<2>, dim3(blocks), dim3(RBS), 0, s, d, len, partials); break;
+    case 3: hipLaunchKernelGGL(dots_partial_kernel<3>, dim3(blocks), dim3(RBS), 0, s, d, len, partials); break;
+    default: hipLaunchKernelGGL(dots_partial_kernel<4>, dim3(blocks), dim3(RBS), 0, s, d, len, partials); break;
+  }
+  int rc = check_launch("reduce_dots_partial");
+  if (rc) return rc;
+  hipLaunchKernelGGL(dots_final_kernel, dim3(1), dim3(256), 0, s, nd, partials, blocks, o, out);
+  return check_launch("reduce_dots_final");
+}
 
 // out[slot] = scale * reduce(mode, a, b, w) (+= if accumulate). mode 0: sum a b w; 1: sum w |a|;
 // 2: max |a - b|.

@@ -437,6 +437,27 @@ def small_solve(a: torch.Tensor, scale: torch.Tensor, b: torch.Tensor, x: torch.
 RED_DOT, RED_L1, RED_MAXDIFF = 0, 1, 2
 
 
+def reduce_dots(products: list, out: torch.Tensor) -> torch.Tensor:
+    """Up to four ``reduce(RED_DOT, ...)`` calls over arrays of one length in one pass: each product is
+    ``(a, b, w, slot, scale, accumulate)`` (``b``, ``w`` may be None); bitwise the separate calls."""
+    nd = len(products)
+    if not 1 <= nd <= 4:
+        raise ValueError("reduce_dots: one to four products")
+    n = products[0][0].numel()
+    for a, b, w, *_ in products:
+        for t in (a, b, w):
+            if t is not None and (t.numel() != n or t.dtype != torch.float64 or not t.is_contiguous()):
+                raise ValueError("reduce_dots: operands must be contiguous float64 arrays of one length")
+    ws = partials_for(max(n, 1), products[0][0].device)  # >= 4096 doubles >= nd x blocks(n)
+    arr = lambda kind, vals: (kind * nd)(*vals)  # noqa: E731
+    _lib.call("cdft_reduce_dots_f64", nd, arr(ctypes.c_void_p, [p[0].data_ptr() for p in products]),
+              arr(ctypes.c_void_p, [_ptr(p[1]) for p in products]), arr(ctypes.c_void_p, [_ptr(p[2]) for p in products]),
+              n, arr(ctypes.c_double, [float(p[4]) for p in products]), arr(ctypes.c_int, [int(bool(p[5])) for p in products]),
+              arr(ctypes.c_int, [int(p[3]) for p in products]), ws.data_ptr(), ws.numel(), out.data_ptr(),
+              stream_of(products[0][0]))
+    return out
+
+
 def reduce(mode: int, a: torch.Tensor, out: torch.Tensor, slot: int = 0, b: torch.Tensor | None = None,
            w: torch.Tensor | None = None, scale: float = 1.0, accumulate: bool = False) -> torch.Tensor:
     """``out[slot] = scale * reduction`` (``+=`` with ``accumulate``): 0 = sum a b w, 1 = sum w |a|,

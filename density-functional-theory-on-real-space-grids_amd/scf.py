@@ -181,10 +181,10 @@ class KohnShamStep:
         lumped = q.lift(self.node_x, add=self.grid_add, out=self.pot_lumped)
         hartree_weights, xc_weights = lumped[0], lumped[1]
         scalars = self.pot_scalars
-        ops.reduce(ops.RED_DOT, hartree_weights, scalars, 0, b=self.rho, scale=0.5)
         ops.reduce(ops.RED_DOT, q.weights, scalars, 1, b=self.e_node)
-        ops.reduce(ops.RED_DOT, q.far_weights, scalars, 1, b=self.e_grid, accumulate=True)
-        ops.reduce(ops.RED_DOT, xc_weights, scalars, 2, b=self.rho)
+        ops.reduce_dots([(hartree_weights, self.rho, None, 0, 0.5, False),
+                         (q.far_weights, self.e_grid, None, 1, 1.0, True),
+                         (xc_weights, self.rho, None, 2, 1.0, False)], scalars)
         ops.ewise(ops.EW_SUMDIV, self.pot_effective, hartree_weights, xc_weights, q.mass_weights)
 
     def hamiltonian(self, pot: dict) -> CuspFactoredHamiltonian:
@@ -244,15 +244,17 @@ class KohnShamStep:
         m = q.n_nodes
         cross_node = ops.ewise(ops.EW_DOT3, torch.empty(m, dtype=torch.float64, device=g.device), q.f2, nodes[1:], q.grad_u)
         cross_grid = ops.ewise(ops.EW_DOT3, g.empty(), self.factor.weight, grad, self.factor.grad_u)
-        ops.reduce(ops.RED_DOT, q.weights, out, k * k, b=cross_node, scale=0.5)
-        ops.reduce(ops.RED_DOT, q.far_weights, out, k * k, b=cross_grid, scale=0.5, accumulate=True)
         pot_node = ops.ewise(ops.EW_MUL3, torch.empty(m, dtype=torch.float64, device=g.device), q.f2, nodes[0],
                              q.transformed_potential)
         pot_grid = ops.ewise(ops.EW_MUL3, g.empty(), self.factor.weight, rho_out, self.factor.transformed_potential)
-        ops.reduce(ops.RED_DOT, q.weights, out, k * k + 1, b=pot_node)
-        ops.reduce(ops.RED_DOT, q.far_weights, out, k * k + 1, b=pot_grid, accumulate=True)
+        # node parts first, then the grid parts accumulated onto them (the order of the separate sums)
+        ops.reduce_dots([(q.weights, cross_node, None, k * k, 0.5, False),
+                         (q.weights, pot_node, None, k * k + 1, 1.0, False)], out)
+        grid_parts = [(q.far_weights, cross_grid, None, k * k, 0.5, True),
+                      (q.far_weights, pot_grid, None, k * k + 1, 1.0, True)]
         if effective_in is not None:
-            ops.reduce(ops.RED_DOT, effective_in, out, k * k + 2, b=q.mass_weights, w=rho_out)
+            grid_parts.append((effective_in, q.mass_weights, rho_out, k * k + 2, 1.0, False))
+        ops.reduce_dots(grid_parts, out)
 
 
 def energies_from_host(k, occ, evals, host_scalars, e_in, e_out, ion_ion):

@@ -328,6 +328,14 @@ long cdft_reduce_workspace_doubles(long len);
 int cdft_reduce_f64(int mode, const double* a, const double* b, const double* w, long len,
                     double scale, int accumulate, double* partials, long partials_len, double* out,
                     int slot, void* stream);
+/* Up to four mode-0 reductions over the same len values in one pass: out[slots[i]] = scale[i] *
+ * sum a_i b_i w_i (+= with accumulate[i]); a, b, w, scale, accumulate, slots are host arrays of nd
+ * entries (b, w entries may be NULL); partials hold nd x cdft_reduce_workspace_doubles(len) doubles.
+ * Bitwise the nd separate cdft_reduce_f64 calls. The same energy and potential integrals
+ * (src/cdft/scf/step.py:224-313,398-439). */
+int cdft_reduce_dots_f64(int nd, const double* const* a, const double* const* b, const double* const* w,
+                         long len, const double* scale, const int* accumulate, const int* slots,
+                         double* partials, long partials_len, double* out, void* stream);
 
 /* ---- K5: free-space Hartree potential by the Coulomb-cutoff FFT (rocFFT through hipFFT) ---- */
 

@@ -128,3 +128,28 @@ def test_rotate_gram_is_rotate_then_gram(pkg, device, k, n):
     y2 = pkg.ops.rotate(r, x, transpose=True)
     assert torch.equal(y, y2)
     assert torch.equal(gram, pkg.ops.wgram(y2, None, q))
+
+
+@pytest.mark.parametrize("n", [1, 1000, 42244, 274625])
+def test_reduce_dots_are_the_separate_reductions(pkg, device, n):
+    """Several dot products in one pass give bitwise the separate reductions (scales, accumulation onto
+    an earlier value and a third weight factor included), and match numpy."""
+    ops = pkg.ops
+    rng = np.random.default_rng(n)
+    t = lambda: torch.as_tensor(rng.standard_normal(n)).to(device)  # noqa: E731
+    a, b, c, d, e, w = t(), t(), t(), t(), t(), t()
+    ref = torch.as_tensor(rng.standard_normal(6)).to(device)
+    got = ref.clone()
+    ops.reduce(ops.RED_DOT, a, ref, 0, b=b, scale=0.5)
+    ops.reduce(ops.RED_DOT, c, ref, 2, b=d, accumulate=True)
+    ops.reduce(ops.RED_DOT, e, ref, 3, b=a, w=w)
+    ops.reduce(ops.RED_DOT, d, ref, 5)
+    ops.reduce_dots([(a, b, None, 0, 0.5, False), (c, d, None, 2, 1.0, True), (e, a, w, 3, 1.0, False),
+                     (d, None, None, 5, 1.0, False)], got)
+    assert torch.equal(got, ref)
+    host = lambda x: x.cpu().numpy()  # noqa: E731
+    assert abs(float(got[3]) - float((host(e) * host(a) * host(w)).sum())) <= 1e-12 * max(1.0, np.abs(host(e) * host(a) * host(w)).sum())
+    for nd in (1, 2):
+        got2 = ref.clone()
+        ops.reduce_dots([(a, b, None, 0, 0.5, False), (c, d, None, 2, 1.0, True)][:nd], got2)
+        assert float(got2[0]) == float(ref[0])
