@@ -276,7 +276,7 @@ __global__ __launch_bounds__(RB) void resnorm_partial_kernel(const double* __res
 
 // ---- small dense kernels (one thread: k is the subspace size, <= KROT) ----
 
-__global__ void chol_inv_kernel(const double* __restrict__ s, int k, double* __restrict__ t,
+__global__ __launch_bounds__(64) void chol_inv_kernel(const double* __restrict__ s, int k, double* __restrict__ t,
                                 int* __restrict__ flag) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double l[KROT * KROT];
@@ -313,7 +313,7 @@ __global__ void chol_inv_kernel(const double* __restrict__ s, int k, double* __r
 
 // Cyclic Jacobi on 0.5 (M + M^T). Writes eigenvalues ascending and C with eigenvectors in columns
 // (C[i*k + j] = component i of eigenvector j), each column's largest entry made positive.
-__global__ void eigh_kernel(const double* __restrict__ m_in, int k, double* __restrict__ evals,
+__global__ __launch_bounds__(64) void eigh_kernel(const double* __restrict__ m_in, int k, double* __restrict__ evals,
                             double* __restrict__ c_out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double a[KROT * KROT], v[KROT * KROT];
@@ -379,9 +379,12 @@ __global__ void eigh_kernel(const double* __restrict__ m_in, int k, double* __re
 }
 
 // The same two kernels for a compile-time K <= 8: every index is a constant, so the matrices live
-// in registers (the generic versions above keep them in per-lane scratch).
+// in registers (the generic versions above keep them in per-lane scratch). These one-thread kernels
+// are launched with 64 threads and say so (__launch_bounds__(64)): without it the compiler budgets
+// registers for 1024-thread blocks, caps them at 128 and spills the K >= 6 matrices to scratch (the
+// K = 8 Jacobi took 370 us instead of 90).
 template <int K>
-__global__ void chol_inv_small_kernel(const double* __restrict__ s, double* __restrict__ t, int* __restrict__ flag) {
+__global__ __launch_bounds__(64) void chol_inv_small_kernel(const double* __restrict__ s, double* __restrict__ t, int* __restrict__ flag) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double l[K][K], ti[K][K];
   bool ok = true;
@@ -422,7 +425,7 @@ __global__ void chol_inv_small_kernel(const double* __restrict__ s, double* __re
 }
 
 template <int K>
-__global__ void eigh_small_kernel(const double* __restrict__ m_in, double* __restrict__ evals, double* __restrict__ c_out) {
+__global__ __launch_bounds__(64) void eigh_small_kernel(const double* __restrict__ m_in, double* __restrict__ evals, double* __restrict__ c_out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double a[K][K], v[K][K];
 #pragma unroll
@@ -510,7 +513,7 @@ __global__ void eigh_small_kernel(const double* __restrict__ m_in, double* __res
 // the device: lower = evals[index] clamped into (smin + 1e-8, smax - 1e-8), upper = smax, smin =
 // bounds[0], smax = bounds[1]; table row s = (c, a_s, b_s). The operations are those of the host
 // routine in the same order, uncontracted, so the table is bitwise the host's.
-__global__ void cheb_table_kernel(const double* __restrict__ evals, int index, const double* __restrict__ bounds,
+__global__ __launch_bounds__(64) void cheb_table_kernel(const double* __restrict__ evals, int index, const double* __restrict__ bounds,
                                   int degree, double* __restrict__ table) {
 #pragma clang fp contract(off)
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -536,7 +539,7 @@ __global__ void cheb_table_kernel(const double* __restrict__ evals, int index, c
 }
 
 // max |M - I| for a k x k matrix (orthonormality error), written to out[0].
-__global__ void max_abs_minus_identity_kernel(const double* __restrict__ m, int k, double* __restrict__ out) {
+__global__ __launch_bounds__(64) void max_abs_minus_identity_kernel(const double* __restrict__ m, int k, double* __restrict__ out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double worst = 0.0;
   for (int i = 0; i < k; ++i)

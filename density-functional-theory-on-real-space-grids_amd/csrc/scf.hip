@@ -107,7 +107,7 @@ __global__ void atom_combine_kernel(const double* __restrict__ base, const doubl
 }
 
 // Dense solve A x = nucleus_f2 * rho_nuclei for the Hartree split coefficients (na <= 8).
-__global__ void small_solve_kernel(const double* __restrict__ a_in, const double* __restrict__ scale,
+__global__ __launch_bounds__(64) void small_solve_kernel(const double* __restrict__ a_in, const double* __restrict__ scale,
                                    const double* __restrict__ b_in, int na, double* __restrict__ x) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double a[64], b[8];
@@ -149,7 +149,7 @@ __global__ void small_solve_kernel(const double* __restrict__ a_in, const double
 struct Slots {
   int s[16];
 };
-__global__ void pulay_kernel(const double* __restrict__ gram, int ldg, Slots slots, int m, double alpha,
+__global__ __launch_bounds__(64) void pulay_kernel(const double* __restrict__ gram, int ldg, Slots slots, int m, double alpha,
                              double* __restrict__ coef) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int n = m + 1;
