@@ -443,11 +443,12 @@ __global__ __launch_bounds__(NTMAX) void hdiv_batched_kernel(const BatchGrid<T>*
   const T* prev = nullptr;
   T* dst = g.out;
   if (MODE != 0) {
-    T* bufs[3] = {g.out, g.work1, g.work2};
+    // the rotation out / work1 / work2 by selects (an array indexed at run time would live in scratch)
+    auto buf = [&](int i) { return i == 0 ? g.out : (i == 1 ? g.work1 : g.work2); };
     const int offset = ((-(degree - 1)) % 3 + 3) % 3;
-    dst = bufs[(step + offset) % 3];
-    y = step == 0 ? g.x : bufs[(step - 1 + offset) % 3];
-    prev = step <= 1 ? g.x : bufs[(step - 2 + offset) % 3];
+    dst = buf((step + offset) % 3);
+    y = step == 0 ? g.x : buf((step - 1 + offset) % 3);
+    prev = step <= 1 ? g.x : buf((step - 2 + offset) % 3);
   }
   stencil_block<T, P, MODE, TZ>(y, prev, dst, g.w0, g.w1, g.w2, g.invw, g.pot, g.n0, g.n1, g.n2, taps, g.table, step,
                                 g.md, tile, v, lds);
