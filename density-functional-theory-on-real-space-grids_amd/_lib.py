@@ -42,6 +42,7 @@ SIGNATURES: dict[str, list] = {
     "cdft_wgram_f64": [P, I, P, I, P, L, D, I, P, L, P, P],
     "cdft_rotate_f64": [P, I, I, I, P, P, L, P],
     "cdft_lincomb_f64": [P, I, P, P, P, P, P, L, P],
+    "cdft_lincomb_dots_f64": [P, I, P, P, P, P, P, L, P, I, P, I, P, L, P, P],
     "cdft_resnorm_f64": [P, P, P, P, I, L, P, L, P, P],
     "cdft_chol_inv_f64": [P, I, P, P, P],
     "cdft_eigh_f64": [P, I, P, P, P],

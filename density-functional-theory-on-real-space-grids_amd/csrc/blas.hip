@@ -243,6 +243,41 @@ __global__ __launch_bounds__(256) void lincomb_kernel(double* __restrict__ out, 
   }
 }
 
+// A linear combination fused with the weighted dot products of its result that follow it (the
+// Lanczos updates): out = sum_t c_t x_t (lincomb_kernel's coefficients and term order), then the block
+// partials of sum_p q_p out_p b_jp for ND <= 2 operands b_j (b_j == out gives the squared norm), as
+// entries (0, j) of the partial tile. The thread-to-point mapping, the products ((out q) b) and the
+// block epilogue are wgram_partial_kernel<1, 1>'s, so the result and the sums are bitwise lincomb
+// followed by separate wgram calls.
+template <int ND>
+__global__ __launch_bounds__(RB) void lincomb_dots_kernel(double* __restrict__ out, int nterms, Terms t,
+                                                          const double* __restrict__ dev, long n,
+                                                          const double* __restrict__ q, const double* b0,
+                                                          const double* b1, double* __restrict__ partials) {
+  double acc[ND];
+#pragma unroll
+  for (int j = 0; j < ND; ++j) acc[j] = 0.0;
+  const double* bs[2] = {b0, b1};
+  const long stride = (long)gridDim.x * RB;
+  constexpr int U = 4;  // wgram_partial_kernel<1, 1>'s points per pass
+  for (long p0 = (long)blockIdx.x * RB + threadIdx.x; p0 < n; p0 += U * stride) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long p = p0 + u * stride;
+      if (p < n) {
+        double v = term_coef(t, 0, dev) * t.x[0][p];
+        for (int i = 1; i < nterms; ++i) v += term_coef(t, i, dev) * t.x[i][p];
+        out[p] = v;
+        const double av = v * (q ? q[p] : 1.0);
+#pragma unroll
+        for (int j = 0; j < ND; ++j) acc[j] += av * (bs[j] == out ? v : bs[j][p]);
+      }
+    }
+  }
+  double* row = partials + (long)blockIdx.x * KMAX * KMAX;
+  block_partials<ND>(acc, [&](int j, double v) { row[j] = v; });
+}
+
 // ---- residual norms ----
 
 __global__ __launch_bounds__(RB) void resnorm_partial_kernel(const double* __restrict__ av,
@@ -620,6 +655,44 @@ extern "C" int cdft_rotate_gram_f64(const double* r, int k, const double* x, dou
   if (rc) return rc;
   hipLaunchKernelGGL(wgram_final_kernel, dim3(k * k), dim3(64), 0, s, partials, blocks, k, k, 1.0, 0, gram, k, 0, 0);
   return check_launch("rotate_gram_final");
+}
+
+// out = sum_t c_t x_t (as cdft_lincomb_f64, nterms <= 16, out may not alias the b operands unless it is
+// one of them) and dots[j] = post(sum_p q_p out_p b_j[p]) for ndots <= 2 (b_j == out: the squared
+// norm; post 1 = sqrt), bitwise cdft_lincomb_f64 followed by ndots cdft_wgram_f64(out, 1, b_j, 1, ...)
+// calls; dots are written to dots[0], dots[1].
+extern "C" int cdft_lincomb_dots_f64(double* out, int nterms, const double* const* x, const double* c, const int* mode,
+                                     const int* idx, const double* dev, long n, const double* q, int ndots,
+                                     const double* const* b, int post, double* partials, long partials_len,
+                                     double* dots, void* stream) {
+  if (!out || !x || !c || !mode || !idx || !b || !dots || !partials || nterms < 1 || nterms > 16 || n < 1 ||
+      ndots < 1 || ndots > 2 || (post != 0 && post != 1))
+    return fail(CDFT_E_ARG, "lincomb_dots: bad arguments (nterms=%d, ndots=%d)", nterms, ndots);
+  Terms t{};
+  for (int i = 0; i < nterms; ++i) {
+    if (!x[i]) return fail(CDFT_E_ARG, "lincomb_dots: term %d has no operand", i);
+    if ((mode[i] == 1 || mode[i] == 2) && !dev) return fail(CDFT_E_ARG, "lincomb_dots: device coefficients missing");
+    t.x[i] = x[i];
+    t.c[i] = c[i];
+    t.mode[i] = mode[i];
+    t.idx[i] = idx[i];
+  }
+  for (int j = 0; j < ndots; ++j)
+    if (!b[j]) return fail(CDFT_E_ARG, "lincomb_dots: dot operand %d missing", j);
+  const int blocks = reduction_blocks(n);
+  if (partials_len < (long)blocks * KMAX * KMAX) return fail(CDFT_E_CAPACITY, "lincomb_dots: partials too small");
+  hipStream_t s = (hipStream_t)stream;
+  if (ndots == 1)
+    hipLaunchKernelGGL(lincomb_dots_kernel<1>, dim3(blocks), dim3(RB), 0, s, out, nterms, t, dev, n, q, b[0], b[0],
+                       partials);
+  else
+    hipLaunchKernelGGL(lincomb_dots_kernel<2>, dim3(blocks), dim3(RB), 0, s, out, nterms, t, dev, n, q, b[0], b[1],
+                       partials);
+  int rc = check_launch("lincomb_dots");
+  if (rc) return rc;
+  hipLaunchKernelGGL(wgram_final_kernel, dim3(ndots), dim3(64), 0, s, partials, blocks, 1, ndots, 1.0, post, dots,
+                     ndots, 0, 0);
+  return check_launch("lincomb_dots_final");
 }
 
 extern "C" long cdft_reduction_workspace_doubles(long n) {

@@ -298,11 +298,11 @@ def lanczos_enqueue(apply, m, start: torch.Tensor, steps: int, scal: torch.Tenso
         terms = [(w, 1.0), (v, -1.0, 1, a_slot(i))]
         if i > 0:
             terms.append((v_prev, -1.0, 1, b_slot(i - 1)))
-        ops.lincomb(w, terms, scal)
-        m.inner_into(w, v, scal[s1:s1 + 1])
-        m.inner_into(w, v_prev, scal[s2:s2 + 1])
-        ops.lincomb(w, [(w, 1.0), (v, -1.0, 1, s1), (v_prev, -1.0, 1, s2)], scal)
-        m.norm_into(w, scal[b_slot(i):b_slot(i) + 1])
+        # each update fused with the products that follow it (bitwise the separate lincomb and
+        # inner/norm calls): <w, v> and <w, v_prev> into the adjacent slots s1, s2, then ||w||
+        ops.lincomb_dots(w, terms, scal, m.q, [v, v_prev], scal[s1:s2 + 1])
+        ops.lincomb_dots(w, [(w, 1.0), (v, -1.0, 1, s1), (v_prev, -1.0, 1, s2)], scal, m.q, [w],
+                         scal[b_slot(i):b_slot(i) + 1], sqrt=True)
         v_new = ops.lincomb(torch.empty_like(v), [(w, 1.0, 2, b_slot(i))], scal)
         v_prev, v = v, v_new
     return scal

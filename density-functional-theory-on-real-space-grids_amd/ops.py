@@ -333,6 +333,42 @@ def lincomb(out: torch.Tensor, terms: list, dev: torch.Tensor | None = None) -> 
     return out
 
 
+def lincomb_dots(out: torch.Tensor, terms: list, dev: torch.Tensor, q: torch.Tensor | None, others: list,
+                 dots: torch.Tensor, sqrt: bool = False) -> torch.Tensor:
+    """:func:`lincomb` into ``out`` followed by ``dots[j] = sum q out others[j]`` (``others[j]`` may be
+    ``out`` itself: the squared norm, ``sqrt`` takes the root) for one or two operands, in one pass;
+    bitwise the separate ``lincomb`` and ``wgram`` calls. ``dots`` holds ``len(others)`` adjacent
+    doubles."""
+    n = out.numel()
+    m = len(terms)
+    ptrs = (ctypes.c_void_p * m)()
+    coefs = (ctypes.c_double * m)()
+    modes = (ctypes.c_int * m)()
+    idxs = (ctypes.c_int * m)()
+    for i, term in enumerate(terms):
+        x, c = term[0], term[1]
+        if x.numel() != n or x.dtype != torch.float64 or not x.is_contiguous():
+            raise ValueError("lincomb_dots: every term must be a contiguous float64 array of the output's size")
+        ptrs[i] = x.data_ptr()
+        coefs[i] = float(c)
+        modes[i] = term[2] if len(term) > 2 else 0
+        idxs[i] = term[3] if len(term) > 3 else 0
+    nd = len(others)
+    if not 1 <= nd <= 2 or dots.numel() < nd:
+        raise ValueError("lincomb_dots: one or two dot operands, with room for their results")
+    for b in others:
+        if b.numel() != n or b.dtype != torch.float64 or not b.is_contiguous():
+            raise ValueError("lincomb_dots: dot operands must be contiguous float64 arrays of the output's size")
+    if q is not None:
+        require(q, (n,), torch.float64, out.device, "q")
+    bs = (ctypes.c_void_p * nd)(*[b.data_ptr() for b in others])
+    ws = partials_for(n, out.device)
+    _lib.call("cdft_lincomb_dots_f64", out.data_ptr(), m, ctypes.addressof(ptrs), ctypes.addressof(coefs),
+              ctypes.addressof(modes), ctypes.addressof(idxs), _ptr(dev), n, _ptr(q), nd, ctypes.addressof(bs),
+              1 if sqrt else 0, ws.data_ptr(), ws.numel(), dots.data_ptr(), stream_of(out))
+    return out
+
+
 def resnorm(av: torch.Tensor, v: torch.Tensor, evals: torch.Tensor, q: torch.Tensor | None,
             out: torch.Tensor | None = None) -> torch.Tensor:
     """``sqrt(sum q (Av - e v)^2)`` per vector."""

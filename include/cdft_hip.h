@@ -145,6 +145,16 @@ int cdft_rotate_f64(const double* r, int kin, int kout, int transpose, const dou
 int cdft_lincomb_f64(double* out, int nterms, const double* const* x, const double* c,
                      const int* mode, const int* idx, const double* dev, long n, void* stream);
 
+/* cdft_lincomb_f64 into out, then dots[j] = post(sum_p q_p out_p b_j[p]) for ndots <= 2 operands b (host
+ * array of device pointers; b[j] == out gives the squared norm; post 1 = sqrt; q may be NULL), in one
+ * pass with one final launch: bitwise the lincomb followed by the ndots cdft_wgram_f64 calls. The
+ * Lanczos update and its reorthogonalisation / normalisation products (src/cdft/operators/hamiltonian.py:
+ * 55-65). */
+int cdft_lincomb_dots_f64(double* out, int nterms, const double* const* x, const double* c, const int* mode,
+                          const int* idx, const double* dev, long n, const double* q, int ndots,
+                          const double* const* b, int post, double* partials, long partials_len,
+                          double* dots, void* stream);
+
 /* norms[i] = sqrt(max(sum_p q (av_i - evals_i v_i)^2, 0)), k <= 8. Replaces residual_norms
  * (src/cdft/eigen/rayleigh_ritz.py:105-115). */
 int cdft_resnorm_f64(const double* av, const double* v, const double* evals, const double* q, int k,

@@ -153,3 +153,72 @@ def test_reduce_dots_are_the_separate_reductions(pkg, device, n):
         got2 = ref.clone()
         ops.reduce_dots([(a, b, None, 0, 0.5, False), (c, d, None, 2, 1.0, True)][:nd], got2)
         assert float(got2[0]) == float(ref[0])
+
+
+@pytest.mark.parametrize("n,weighted", [(1, True), (3001, True), (274625, True), (4097, False)])
+def test_lincomb_dots_is_lincomb_then_gram(pkg, device, n, weighted):
+    """A linear combination fused with the weighted products of its result gives bitwise the separate
+    combination and Gram calls (two products, and the norm, in place as the Lanczos step uses them)."""
+    ops = pkg.ops
+    rng = np.random.default_rng(n + weighted)
+    t = lambda: torch.as_tensor(rng.standard_normal(n)).to(device)  # noqa: E731
+    w, v, vp = t(), t(), t()
+    q = torch.as_tensor(rng.uniform(0.1, 2.0, n)).to(device) if weighted else None
+    dev = torch.as_tensor(rng.uniform(0.5, 2.0, 4)).to(device)
+    terms_of = lambda out: [(out, 1.0), (v, -1.0, 1, 0), (vp, -0.5, 2, 1)]  # noqa: E731
+    ref_w = w.clone()
+    ops.lincomb(ref_w, terms_of(ref_w), dev)
+    ref_dots = torch.cat([ops.wgram(ref_w.view(1, -1), v.view(1, -1), q).view(1),
+                          ops.wgram(ref_w.view(1, -1), vp.view(1, -1), q).view(1)])
+    got_w = w.clone()
+    dots = torch.empty(2, dtype=torch.float64, device=device)
+    ops.lincomb_dots(got_w, terms_of(got_w), dev, q, [v, vp], dots)
+    assert torch.equal(got_w, ref_w) and torch.equal(dots, ref_dots)
+    # the norm of the result, in place, with the root taken
+    ref2 = ref_w.clone()
+    ops.lincomb(ref2, [(ref2, 1.0), (v, -1.0, 1, 2)], dev)
+    ref_norm = ops.wgram(ref2.view(1, -1), None, q, sqrt=True).view(1)
+    got2 = ref_w.clone()
+    nrm = torch.empty(1, dtype=torch.float64, device=device)
+    ops.lincomb_dots(got2, [(got2, 1.0), (v, -1.0, 1, 2)], dev, q, [got2], nrm, sqrt=True)
+    assert torch.equal(got2, ref2) and torch.equal(nrm, ref_norm)
+
+
+def test_fused_lanczos_is_the_unfused_recurrence(pkg, device):
+    """The Lanczos recurrence with its updates fused into the following products gives bitwise the
+    scalars of the recurrence written with separate combination and product calls."""
+    from cdft_amd import hamiltonian as hm
+
+    ops = pkg.ops
+    n, steps = 20000, 9
+    rng = np.random.default_rng(21)
+    diag = torch.as_tensor(rng.uniform(-2.0, 5.0, n)).to(device)
+    q = torch.as_tensor(rng.uniform(0.2, 1.5, n)).to(device)
+    start = torch.as_tensor(rng.standard_normal(n)).to(device)
+    m = hm.DeviceMeasure(None, q)
+
+    def apply(x, y):
+        ops.ewise(ops.EW_MUL, y.view(-1), x.view(-1), diag)
+
+    got = hm.lanczos_enqueue(apply, m, start, steps).clone()
+    scal = torch.zeros(2 * steps + 3, dtype=torch.float64, device=device)
+    a_slot, b_slot = (lambda i: i), (lambda i: steps + i)
+    s1, s2, s0 = 2 * steps, 2 * steps + 1, 2 * steps + 2
+    m.norm_into(start, scal[s0:s0 + 1])
+    v = ops.lincomb(torch.empty(n, dtype=torch.float64, device=device), [(start, 1.0, 2, s0)], scal)
+    v_prev = torch.zeros_like(v)
+    w = torch.empty_like(v)
+    for i in range(steps):
+        apply(v.view(1, -1), w.view(1, -1))
+        m.inner_into(v, w, scal[a_slot(i):a_slot(i) + 1])
+        terms = [(w, 1.0), (v, -1.0, 1, a_slot(i))]
+        if i > 0:
+            terms.append((v_prev, -1.0, 1, b_slot(i - 1)))
+        ops.lincomb(w, terms, scal)
+        m.inner_into(w, v, scal[s1:s1 + 1])
+        m.inner_into(w, v_prev, scal[s2:s2 + 1])
+        ops.lincomb(w, [(w, 1.0), (v, -1.0, 1, s1), (v_prev, -1.0, 1, s2)], scal)
+        m.norm_into(w, scal[b_slot(i):b_slot(i) + 1])
+        v_new = ops.lincomb(torch.empty_like(v), [(w, 1.0, 2, b_slot(i))], scal)
+        v_prev, v = v, v_new
+    assert torch.equal(got, scal)
