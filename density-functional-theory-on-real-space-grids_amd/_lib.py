@@ -23,14 +23,16 @@ L = ctypes.c_long
 #: addresses passed as integers; every function returns an int status.
 SIGNATURES: dict[str, list] = {
     "cdft_abi_version": [],
-    "cdft_hdiv_apply_f64": [P, P, P, P, P, P, P, I, I, I, I, P, I, P],
-    "cdft_hdiv_apply_f32": [P, P, P, P, P, P, P, I, I, I, I, P, I, P],
-    "cdft_cheb_step_f64": [P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
-    "cdft_cheb_step_f32": [P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
-    "cdft_cheb_filter_f64": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
-    "cdft_cheb_filter_f32": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
-    "cdft_hdiv_apply_adjoint_f64": [P, P, P, P, P, P, P, P, I, I, I, I, P, I, P],
-    "cdft_hdiv_apply_adjoint_f32": [P, P, P, P, P, P, P, P, I, I, I, I, P, I, P],
+    "cdft_hdiv_pack_f64": [P, P, P, P, I, I, I, I, P, P],
+    "cdft_hdiv_pack_f32": [P, P, P, P, I, I, I, I, P, P],
+    "cdft_hdiv_apply_f64": [P, P, P, P, P, P, P, P, I, I, I, I, P, I, P],
+    "cdft_hdiv_apply_f32": [P, P, P, P, P, P, P, P, I, I, I, I, P, I, P],
+    "cdft_cheb_step_f64": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
+    "cdft_cheb_step_f32": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
+    "cdft_cheb_filter_f64": [P, P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
+    "cdft_cheb_filter_f32": [P, P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, I, P],
+    "cdft_hdiv_apply_adjoint_f64": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P],
+    "cdft_hdiv_apply_adjoint_f32": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P],
     "cdft_batch_plan_f64": [I, P, I, I, P, P, P, L, P],
     "cdft_cheb_filter_batched_f64": [P, I, P, I, I, P],
     "cdft_hdiv_apply_batched_f64": [P, I, P, I, P],
@@ -49,7 +51,7 @@ SIGNATURES: dict[str, list] = {
     "cdft_orth_error_f64": [P, I, P, P],
     "cdft_cheb_table_f64": [P, I, P, I, P, P],
     "cdft_rotate_gram_f64": [P, I, P, P, P, L, P, L, P, P],
-    "cdft_hdiv_resnorm_f64": [P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, L, P, P],
+    "cdft_hdiv_resnorm_f64": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, L, P, P],
     "cdft_cusp_grid_fields_f64": [I, P, P, P, D, P, D, P, P, D, P, P, P, P, P],
     "cdft_face_weights_f64": [I, P, P, P, D, P, I, P, P, P, P, P],
     "cdft_node_fields_f64": [P, P, P, L, I, P, P, P, D, P, P, P, P, P, P],
@@ -82,6 +84,7 @@ VALUED = {
     "cdft_reduction_workspace_doubles": (ctypes.c_long, [L]),
     "cdft_reduce_workspace_doubles": (ctypes.c_long, [L]),
     "cdft_batch_desc_bytes": (ctypes.c_long, [I]),
+    "cdft_hdiv_pack_len": (ctypes.c_long, [I, I, I, I]),
 }
 
 

@@ -49,7 +49,7 @@ class BatchPlan:
         device = grids[0][1].device
         shapes = (ctypes.c_int * (3 * n))()
         coefs = (ctypes.c_double * (order * n))()
-        ptrs = (ctypes.c_void_p * (10 * n))()
+        ptrs = (ctypes.c_void_p * (11 * n))()
         for i, (geom, x, out, w1, w2, table) in enumerate(grids):
             if geom.order != order or geom.dtype != dtype:
                 raise ValueError("a batch needs one stencil order and one geometry dtype")
@@ -59,9 +59,9 @@ class BatchPlan:
             shapes[3 * i: 3 * i + 3] = list(geom.shape)
             for j in range(order):
                 coefs[order * i + j] = geom.taps[j]
-            vals = [x, out, w1, w2, geom.w0, geom.w1, geom.w2, geom.invw, geom.potential, table]
+            vals = [x, out, w1, w2, geom.w0, geom.w1, geom.w2, geom.invw, geom.potential, geom.packed, table]
             for j, t in enumerate(vals):
-                ptrs[10 * i + j] = t.data_ptr()
+                ptrs[11 * i + j] = t.data_ptr()
         nbytes = int(_lib.load().cdft_batch_desc_bytes(n))
         host = (ctypes.c_uint8 * nbytes)()
         launch = (ctypes.c_int * 3)()

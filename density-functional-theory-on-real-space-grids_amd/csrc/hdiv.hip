@@ -47,6 +47,34 @@ __device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, int voff
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
+// One geometry record (wx, wy, wz, invw) of four T at byte offset voff + soff; zeros when out of range.
+// The 16-byte load is bit-cast as a whole to a vector of the element type: extracting single 32-bit
+// lanes of it and bit-casting each (the obvious form) is compiled here into one 4-byte load read for
+// every lane.
+template <typename T>
+__device__ __forceinline__ void bload_rec(__amdgpu_buffer_rsrc_t r, int voff, int soff, T& a, T& b, T& c, T& d);
+template <>
+__device__ __forceinline__ void bload_rec<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff, double& a, double& b,
+                                                  double& c, double& d) {
+  typedef __attribute__((ext_vector_type(2))) double d2;
+  const d2 lo = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  const d2 hi = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16, soff, 0));
+  a = lo[0];
+  b = lo[1];
+  c = hi[0];
+  d = hi[1];
+}
+template <>
+__device__ __forceinline__ void bload_rec<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff, float& a, float& b,
+                                                 float& c, float& d) {
+  typedef __attribute__((ext_vector_type(4))) float f4;
+  const f4 v = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  a = v[0];
+  b = v[1];
+  c = v[2];
+  d = v[3];
+}
+
 template <typename T>
 __device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, int voff, int soff);
 template <>
@@ -94,7 +122,7 @@ template <typename T, int P, int MODE, int TZ>
 __device__ __forceinline__ void stencil_block(
     const T* __restrict__ phi, const T* __restrict__ prev, T* __restrict__ out,
     const T* __restrict__ w0, const T* __restrict__ w1, const T* __restrict__ w2,
-    const T* __restrict__ invw, const T* __restrict__ pot, int n0, int n1, int n2,
+    const T* __restrict__ invw, const T* __restrict__ pot, const T* __restrict__ gp, int n0, int n1, int n2,
     const Taps<P>& taps, const double* __restrict__ table, int step, const March& md, unsigned tile, int v,
     StencilLds<T, P, TZ>& lds) {
   constexpr int R = 2 * P - 1;  // stencil reach: 2P-1 points each side
@@ -138,7 +166,8 @@ __device__ __forceinline__ void stencil_block(
   const auto r_w0 = make_rsrc(w0, (n0 + 2 * P) * plane * S);
   const auto r_w1 = make_rsrc(w1, n0 * pl1 * S);
   const auto r_w2 = make_rsrc(w2, n0 * pl2 * S);
-  const auto r_iw = make_rsrc(invw, n0 * plane * S);
+  // per-point geometry records (wx, wy, wz, invw) over the face-extended (y, z) rows
+  const auto r_gp = make_rsrc(gp, n0 * n1f * n2f * 4 * S);
   const auto r_pt = make_rsrc(pot, n0 * plane * S);
   const int col = own ? (y * n2 + z) * S : NONE;
 
@@ -181,9 +210,10 @@ __device__ __forceinline__ void stencil_block(
   const bool zin = act && z < n2, yin = act && y < n1;
   const int ay0 = Y0 + ty, ay1 = Y0 + ty + TY;
   const int az0 = Z0 + tz, az1 = Z0 + tz + TZ;
-  const int wy0_off = (zin && ay0 < n1f) ? (ay0 * n2 + z) * S : NONE;
+  // the record of column (y, z) (records exist for y < n1 + 2P, z < n2 + 2P; the wy / wz parts are
+  // zero where the face is not this thread's, as the separate loads' out-of-range offsets were)
+  const int rec = (act && y < n1f && z < n2f) ? (y * n2f + z) * 4 * S : NONE;
   const int wy1_off = (zin && ty < R && ay1 < n1f) ? (ay1 * n2 + z) * S : NONE;
-  const int wz0_off = (yin && az0 < n2f) ? (y * n2f + az0) * S : NONE;
   const int wz1_off = (yin && tz < R && az1 < n2f) ? (y * n2f + az1) * S : NONE;
 
   // ---- x warm-up: phi(X0-2P+1 .. X0+2P-1) and the faces F(X0-P .. X0+P-2) ----
@@ -217,7 +247,8 @@ __device__ __forceinline__ void stencil_block(
   };
   auto load_plane = [&](int m, Plane& L) {
     const int base = m * plane * S;
-    L.wxf = bload<T>(r_w0, col, (m + 2 * P - 1) * plane * S);
+    bload_rec<T>(r_gp, rec, m * n1f * n2f * 4 * S, L.wxf, L.wy0, L.wz0, L.iw);
+    if (MODE == 3) L.iw = T(0);
     {
       // loaded unconditionally (a clamped plane, then a select): a load on only one side of a
       // branch makes the compiler's wait counting assume the worst and wait for every load
@@ -227,11 +258,8 @@ __device__ __forceinline__ void stencil_block(
     }
 #pragma unroll
     for (int s = 0; s < MAXH; ++s) L.hal[s] = bload<T>(r_phi, hsrc[s], base);
-    L.wy0 = bload<T>(r_w1, wy0_off, m * pl1 * S);
     L.wy1 = bload<T>(r_w1, wy1_off, m * pl1 * S);
-    L.wz0 = bload<T>(r_w2, wz0_off, m * pl2 * S);
     L.wz1 = bload<T>(r_w2, wz1_off, m * pl2 * S);
-    L.iw = MODE == 3 ? T(0) : bload<T>(r_iw, col, base);
     L.pt = bload<T>(r_pt, col, base);
     L.pr = MODE >= 2 ? bload<T>(r_prev, col, base) : T(0);
   };
@@ -393,13 +421,13 @@ template <typename T, int P, int MODE, int TZ>
 __global__ __launch_bounds__(NTMAX) void hdiv_kernel(
     const T* __restrict__ phi, const T* __restrict__ prev, T* __restrict__ out,
     const T* __restrict__ w0, const T* __restrict__ w1, const T* __restrict__ w2,
-    const T* __restrict__ invw, const T* __restrict__ pot, int n0, int n1, int n2, int k,
+    const T* __restrict__ invw, const T* __restrict__ pot, const T* __restrict__ gp, int n0, int n1, int n2, int k,
     Taps<P> taps, const double* __restrict__ table, int step, March md) {
   __shared__ StencilLds<T, P, TZ> lds;
   const unsigned lin = xcd_tile(blockIdx.x, gridDim.x);
   const unsigned tile = lin / (unsigned)k;
   const int v = (int)(lin - tile * (unsigned)k);
-  stencil_block<T, P, MODE, TZ>(phi, prev, MODE == 4 ? out + lin : out, w0, w1, w2, invw, pot, n0, n1, n2, taps, table,
+  stencil_block<T, P, MODE, TZ>(phi, prev, MODE == 4 ? out + lin : out, w0, w1, w2, invw, pot, gp, n0, n1, n2, taps, table,
                                 step, md, tile, v, lds);
 }
 
@@ -414,6 +442,7 @@ struct BatchGrid {
   T* work1;
   T* work2;
   const T *w0, *w1, *w2, *invw, *pot;
+  const T* gp;          // the grid's geometry records (cdft_hdiv_pack_*)
   const double* table;  // (degree, 3) Chebyshev rows of this grid
   double taps[8];
   int n0, n1, n2, k;
@@ -450,8 +479,44 @@ __global__ __launch_bounds__(NTMAX) void hdiv_batched_kernel(const BatchGrid<T>*
     y = step == 0 ? g.x : buf((step - 1 + offset) % 3);
     prev = step <= 1 ? g.x : buf((step - 2 + offset) % 3);
   }
-  stencil_block<T, P, MODE, TZ>(y, prev, dst, g.w0, g.w1, g.w2, g.invw, g.pot, g.n0, g.n1, g.n2, taps, g.table, step,
+  stencil_block<T, P, MODE, TZ>(y, prev, dst, g.w0, g.w1, g.w2, g.invw, g.pot, g.gp, g.n0, g.n1, g.n2, taps, g.table, step,
                                 g.md, tile, v, lds);
+}
+
+// The per-point geometry records the stencil kernels read each plane: for plane m and column (y, z) of
+// the face-extended rows (y < n1 + 2P, z < n2 + 2P), (w0 at extended plane m + 2P - 1, w1 at (m, y, z),
+// w2 at (m, y, z), invw), each part zero where the separate array has no such entry for this column.
+template <typename T>
+__global__ __launch_bounds__(256) void pack_kernel(const T* __restrict__ w0, const T* __restrict__ w1,
+                                                   const T* __restrict__ w2, const T* __restrict__ invw, int n0,
+                                                   int n1, int n2, int two_p, T* __restrict__ gp) {
+  const int n1f = n1 + two_p, n2f = n2 + two_p;
+  const long total = (long)n0 * n1f * n2f;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const long per = (long)n1f * n2f;
+  const int m = (int)(idx / per);
+  const int r = (int)(idx - (long)m * per);
+  const int y = r / n2f, z = r - (r / n2f) * n2f;
+  const long plane = (long)n1 * n2;
+  const bool own = y < n1 && z < n2;
+  T* o = gp + 4 * idx;
+  o[0] = own ? w0[(long)(m + two_p - 1) * plane + (long)y * n2 + z] : T(0);
+  o[1] = z < n2 ? w1[(long)m * n1f * n2 + (long)y * n2 + z] : T(0);
+  o[2] = y < n1 ? w2[(long)m * n1 * n2f + (long)y * n2f + z] : T(0);
+  o[3] = own ? invw[(long)m * plane + (long)y * n2 + z] : T(0);
+}
+
+template <typename T>
+static int pack_impl(const T* w0, const T* w1, const T* w2, const T* invw, int n0, int n1, int n2, int order, T* gp,
+                     hipStream_t s) {
+  if (!w0 || !w1 || !w2 || !invw || !gp || n0 < 1 || n1 < 1 || n2 < 1)
+    return fail(CDFT_E_ARG, "hdiv_pack: bad arguments");
+  if (order != 2 && order != 4 && order != 6 && order != 8) return fail(CDFT_E_ARG, "hdiv_pack: bad order %d", order);
+  const long total = (long)n0 * (n1 + order) * (n2 + order);
+  hipLaunchKernelGGL(pack_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, w0, w1, w2, invw, n0, n1,
+                     n2, order, gp);
+  return check_launch("hdiv_pack");
 }
 
 static int ceil_div(int a, int b) { return (a + b - 1) / b; }
@@ -490,44 +555,44 @@ static March march_dims(int n0, int n1, int n2, int k, int reach) {
 
 template <typename T, int P, int MODE>
 static int launch_p(const T* phi, const T* prev, T* out, const T* w0, const T* w1, const T* w2,
-                    const T* invw, const T* pot, int k, int n0, int n1, int n2, const double* coef,
+                    const T* invw, const T* pot, const T* gp, int k, int n0, int n1, int n2, const double* coef,
                     const double* table, int step, hipStream_t stream) {
   Taps<P> taps;
   for (int i = 0; i < 2 * P; ++i) taps.c[i] = coef[i];
   const March d = march_dims(n0, n1, n2, k, 2 * P - 1);
   if (d.threads > NTMAX) return fail(CDFT_E_ARG, "hdiv: tile needs %d threads", d.threads);
   // Offsets inside one vector and inside the extended face-weight arrays are 32-bit.
-  if ((long)(n0 + 2 * P) * (n1 + 2 * P) * (n2 + 2 * P) >= 0x7fffffffL)
+  if ((long)(n0 + 2 * P) * (n1 + 2 * P) * (n2 + 2 * P) * 4 * (long)sizeof(T) >= 0x7fffffffL)
     return fail(CDFT_E_ARG, "hdiv: grid %d x %d x %d too large for one launch", n0, n1, n2);
   const unsigned long blocks = (unsigned long)d.tiles * (unsigned long)k;
   if (blocks > 0x7fffffffUL) return fail(CDFT_E_ARG, "hdiv: too many blocks");
   const dim3 grid((unsigned)blocks), block(d.threads);
   if (d.tz == 13)
     hipLaunchKernelGGL((hdiv_kernel<T, P, MODE, 13>), grid, block, 0, stream, phi, prev, out, w0, w1, w2,
-                       invw, pot, n0, n1, n2, k, taps, table, step, d);
+                       invw, pot, gp, n0, n1, n2, k, taps, table, step, d);
   else
     hipLaunchKernelGGL((hdiv_kernel<T, P, MODE, 16>), grid, block, 0, stream, phi, prev, out, w0, w1, w2,
-                       invw, pot, n0, n1, n2, k, taps, table, step, d);
+                       invw, pot, gp, n0, n1, n2, k, taps, table, step, d);
   return check_launch("hdiv_kernel");
 }
 
 template <typename T, int MODE>
 static int launch(const T* phi, const T* prev, T* out, const T* w0, const T* w1, const T* w2,
-                  const T* invw, const T* pot, int k, int n0, int n1, int n2, const double* coef,
+                  const T* invw, const T* pot, const T* gp, int k, int n0, int n1, int n2, const double* coef,
                   int order, const double* table, int step, hipStream_t stream) {
   switch (order) {
-    case 2: return launch_p<T, 1, MODE>(phi, prev, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, table, step, stream);
-    case 4: return launch_p<T, 2, MODE>(phi, prev, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, table, step, stream);
-    case 6: return launch_p<T, 3, MODE>(phi, prev, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, table, step, stream);
-    case 8: return launch_p<T, 4, MODE>(phi, prev, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, table, step, stream);
+    case 2: return launch_p<T, 1, MODE>(phi, prev, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, table, step, stream);
+    case 4: return launch_p<T, 2, MODE>(phi, prev, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, table, step, stream);
+    case 6: return launch_p<T, 3, MODE>(phi, prev, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, table, step, stream);
+    case 8: return launch_p<T, 4, MODE>(phi, prev, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, table, step, stream);
     default: return fail(CDFT_E_ARG, "unsupported stencil order %d (2, 4, 6 or 8)", order);
   }
 }
 
 static int validate(const void* phi, const void* out, const void* w0, const void* w1,
-                    const void* w2, const void* invw, const void* pot, int k, int n0, int n1,
+                    const void* w2, const void* invw, const void* pot, const void* gp, int k, int n0, int n1,
                     int n2, const double* coef, int order) {
-  if (!phi || !out || !w0 || !w1 || !w2 || !invw || !pot || !coef)
+  if (!phi || !out || !w0 || !w1 || !w2 || !invw || !pot || !gp || !coef)
     return fail(CDFT_E_ARG, "hdiv: null pointer argument");
   if (k < 1 || n0 < 1 || n1 < 1 || n2 < 1)
     return fail(CDFT_E_ARG, "hdiv: sizes must be positive (k=%d, n=%d,%d,%d)", k, n0, n1, n2);
@@ -540,10 +605,10 @@ static int validate(const void* phi, const void* out, const void* w0, const void
 
 template <typename T>
 static int filter_impl(const T* x, T* out, T* work1, T* work2, const T* w0, const T* w1,
-                       const T* w2, const T* invw, const T* pot, int k, int n0, int n1, int n2,
+                       const T* w2, const T* invw, const T* pot, const T* gp, int k, int n0, int n1, int n2,
                        const double* coef, int order, const double* table, int degree,
                        hipStream_t stream) {
-  int rc = validate(x, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order);
+  int rc = validate(x, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order);
   if (rc) return rc;
   if (!work1 || !work2 || !table) return fail(CDFT_E_ARG, "cheb_filter: null workspace or table");
   if (degree < 1) return fail(CDFT_E_ARG, "cheb_filter: degree must be >= 1");
@@ -557,9 +622,9 @@ static int filter_impl(const T* x, T* out, T* work1, T* work2, const T* w0, cons
   for (int s = 0; s < degree; ++s) {
     T* dst = bufs[(s + offset) % 3];
     if (s == 0) {
-      rc = launch<T, 1>(y, nullptr, dst, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order, table, s, stream);
+      rc = launch<T, 1>(y, nullptr, dst, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order, table, s, stream);
     } else {
-      rc = launch<T, 2>(y, prev, dst, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order, table, s, stream);
+      rc = launch<T, 2>(y, prev, dst, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order, table, s, stream);
     }
     if (rc) return rc;
     prev = y;
@@ -573,68 +638,68 @@ static int filter_impl(const T* x, T* out, T* work1, T* work2, const T* w0, cons
 using namespace cdft;
 
 extern "C" int cdft_hdiv_apply_f64(const double* phi, const double* w0, const double* w1,
-                                   const double* w2, const double* invw, const double* pot,
+                                   const double* w2, const double* invw, const double* pot, const double* gp,
                                    double* out, int k, int n0, int n1, int n2, const double* coef,
                                    int order, void* stream) {
-  int rc = validate(phi, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order);
+  int rc = validate(phi, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order);
   if (rc) return rc;
-  return launch<double, 0>(phi, nullptr, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order,
+  return launch<double, 0>(phi, nullptr, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order,
                            nullptr, 0, (hipStream_t)stream);
 }
 
 extern "C" int cdft_hdiv_apply_f32(const float* phi, const float* w0, const float* w1,
-                                   const float* w2, const float* invw, const float* pot, float* out,
+                                   const float* w2, const float* invw, const float* pot, const float* gp, float* out,
                                    int k, int n0, int n1, int n2, const double* coef, int order,
                                    void* stream) {
-  int rc = validate(phi, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order);
+  int rc = validate(phi, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order);
   if (rc) return rc;
-  return launch<float, 0>(phi, nullptr, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order,
+  return launch<float, 0>(phi, nullptr, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order,
                           nullptr, 0, (hipStream_t)stream);
 }
 
 extern "C" int cdft_cheb_step_f64(const double* y, const double* prev, const double* w0,
                                   const double* w1, const double* w2, const double* invw,
-                                  const double* pot, double* out, int k, int n0, int n1, int n2,
+                                  const double* pot, const double* gp, double* out, int k, int n0, int n1, int n2,
                                   const double* coef, int order, const double* table, int step,
                                   void* stream) {
-  int rc = validate(y, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order);
+  int rc = validate(y, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order);
   if (rc) return rc;
   if (!table || step < 0) return fail(CDFT_E_ARG, "cheb_step: null table or negative step");
   if (step > 0 && !prev) return fail(CDFT_E_ARG, "cheb_step: step > 0 needs the previous block");
   if (step == 0)
-    return launch<double, 1>(y, nullptr, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order, table, 0, (hipStream_t)stream);
-  return launch<double, 2>(y, prev, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order, table, step, (hipStream_t)stream);
+    return launch<double, 1>(y, nullptr, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order, table, 0, (hipStream_t)stream);
+  return launch<double, 2>(y, prev, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order, table, step, (hipStream_t)stream);
 }
 
 extern "C" int cdft_cheb_step_f32(const float* y, const float* prev, const float* w0,
                                   const float* w1, const float* w2, const float* invw,
-                                  const float* pot, float* out, int k, int n0, int n1, int n2,
+                                  const float* pot, const float* gp, float* out, int k, int n0, int n1, int n2,
                                   const double* coef, int order, const double* table, int step,
                                   void* stream) {
-  int rc = validate(y, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order);
+  int rc = validate(y, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order);
   if (rc) return rc;
   if (!table || step < 0) return fail(CDFT_E_ARG, "cheb_step: null table or negative step");
   if (step > 0 && !prev) return fail(CDFT_E_ARG, "cheb_step: step > 0 needs the previous block");
   if (step == 0)
-    return launch<float, 1>(y, nullptr, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order, table, 0, (hipStream_t)stream);
-  return launch<float, 2>(y, prev, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order, table, step, (hipStream_t)stream);
+    return launch<float, 1>(y, nullptr, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order, table, 0, (hipStream_t)stream);
+  return launch<float, 2>(y, prev, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order, table, step, (hipStream_t)stream);
 }
 
 extern "C" int cdft_cheb_filter_f64(const double* x, double* out, double* work1, double* work2,
                                     const double* w0, const double* w1, const double* w2,
-                                    const double* invw, const double* pot, int k, int n0, int n1,
+                                    const double* invw, const double* pot, const double* gp, int k, int n0, int n1,
                                     int n2, const double* coef, int order, const double* table,
                                     int degree, void* stream) {
-  return filter_impl<double>(x, out, work1, work2, w0, w1, w2, invw, pot, k, n0, n1, n2, coef,
+  return filter_impl<double>(x, out, work1, work2, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef,
                              order, table, degree, (hipStream_t)stream);
 }
 
 extern "C" int cdft_cheb_filter_f32(const float* x, float* out, float* work1, float* work2,
                                     const float* w0, const float* w1, const float* w2,
-                                    const float* invw, const float* pot, int k, int n0, int n1,
+                                    const float* invw, const float* pot, const float* gp, int k, int n0, int n1,
                                     int n2, const double* coef, int order, const double* table,
                                     int degree, void* stream) {
-  return filter_impl<float>(x, out, work1, work2, w0, w1, w2, invw, pot, k, n0, n1, n2, coef,
+  return filter_impl<float>(x, out, work1, work2, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef,
                             order, table, degree, (hipStream_t)stream);
 }
 
@@ -652,9 +717,9 @@ __global__ void scale_rows_kernel_f32(const float* __restrict__ x, const float* 
 }  // namespace cdft
 
 template <typename T>
-static int adjoint_impl(const T* x, const T* w0, const T* w1, const T* w2, const T* invw, const T* pot, T* out,
+static int adjoint_impl(const T* x, const T* w0, const T* w1, const T* w2, const T* invw, const T* pot, const T* gp, T* out,
                         T* work, int k, int n0, int n1, int n2, const double* coef, int order, hipStream_t s) {
-  int rc = validate(x, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order);
+  int rc = validate(x, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order);
   if (rc) return rc;
   if (!work || work == x || work == out) return fail(CDFT_E_ARG, "hdiv_apply_adjoint: work must be a distinct (k, N) block");
   const long n = (long)n0 * n1 * n2, total = n * k;
@@ -667,19 +732,19 @@ static int adjoint_impl(const T* x, const T* w0, const T* w1, const T* w2, const
                        total, (float*)work);
   rc = check_launch("scale_rows_kernel");
   if (rc) return rc;
-  return launch<T, 3>(work, x, out, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order, nullptr, 0, s);
+  return launch<T, 3>(work, x, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order, nullptr, 0, s);
 }
 
 extern "C" int cdft_hdiv_apply_adjoint_f64(const double* x, const double* w0, const double* w1, const double* w2,
-                                           const double* invw, const double* pot, double* out, double* work, int k,
+                                           const double* invw, const double* pot, const double* gp, double* out, double* work, int k,
                                            int n0, int n1, int n2, const double* coef, int order, void* stream) {
-  return adjoint_impl<double>(x, w0, w1, w2, invw, pot, out, work, k, n0, n1, n2, coef, order, (hipStream_t)stream);
+  return adjoint_impl<double>(x, w0, w1, w2, invw, pot, gp, out, work, k, n0, n1, n2, coef, order, (hipStream_t)stream);
 }
 
 extern "C" int cdft_hdiv_apply_adjoint_f32(const float* x, const float* w0, const float* w1, const float* w2,
-                                           const float* invw, const float* pot, float* out, float* work, int k,
+                                           const float* invw, const float* pot, const float* gp, float* out, float* work, int k,
                                            int n0, int n1, int n2, const double* coef, int order, void* stream) {
-  return adjoint_impl<float>(x, w0, w1, w2, invw, pot, out, work, k, n0, n1, n2, coef, order, (hipStream_t)stream);
+  return adjoint_impl<float>(x, w0, w1, w2, invw, pot, gp, out, work, k, n0, n1, n2, coef, order, (hipStream_t)stream);
 }
 
 namespace cdft {
@@ -697,10 +762,10 @@ __global__ __launch_bounds__(64) void resnorm_tiles_kernel(const double* __restr
 }  // namespace cdft
 
 extern "C" int cdft_hdiv_resnorm_f64(const double* phi, const double* w0, const double* w1, const double* w2,
-                                     const double* invw, const double* pot, const double* evals, const double* q, int k,
+                                     const double* invw, const double* pot, const double* gp, const double* evals, const double* q, int k,
                                      int n0, int n1, int n2, const double* coef, int order, double* partials,
                                      long partials_len, double* norms, void* stream) {
-  int rc = validate(phi, partials, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order);
+  int rc = validate(phi, partials, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order);
   if (rc) return rc;
   if (!evals || !q || !norms) return fail(CDFT_E_ARG, "hdiv_resnorm: null evals, q or norms");
   const March d = march_dims(n0, n1, n2, k, order - 1);
@@ -708,10 +773,24 @@ extern "C" int cdft_hdiv_resnorm_f64(const double* phi, const double* w0, const 
   if (partials_len < blocks)
     return fail(CDFT_E_CAPACITY, "hdiv_resnorm: partials hold %ld doubles, the launch needs %ld", partials_len, blocks);
   hipStream_t s = (hipStream_t)stream;
-  rc = launch<double, 4>(phi, q, partials, w0, w1, w2, invw, pot, k, n0, n1, n2, coef, order, evals, 0, s);
+  rc = launch<double, 4>(phi, q, partials, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order, evals, 0, s);
   if (rc) return rc;
   hipLaunchKernelGGL(resnorm_tiles_kernel, dim3(k), dim3(64), 0, s, partials, (int)d.tiles, k, norms);
   return check_launch("resnorm_tiles");
+}
+
+extern "C" long cdft_hdiv_pack_len(int n0, int n1, int n2, int order) {
+  return (n0 < 1 || n1 < 1 || n2 < 1 || order < 1) ? 0L : 4L * n0 * (n1 + order) * (n2 + order);
+}
+
+extern "C" int cdft_hdiv_pack_f64(const double* w0, const double* w1, const double* w2, const double* invw, int n0,
+                                  int n1, int n2, int order, double* gp, void* stream) {
+  return pack_impl<double>(w0, w1, w2, invw, n0, n1, n2, order, gp, (hipStream_t)stream);
+}
+
+extern "C" int cdft_hdiv_pack_f32(const float* w0, const float* w1, const float* w2, const float* invw, int n0, int n1,
+                                  int n2, int order, float* gp, void* stream) {
+  return pack_impl<float>(w0, w1, w2, invw, n0, n1, n2, order, gp, (hipStream_t)stream);
 }
 
 extern "C" int cdft_hdiv_set_segment(int planes) {
@@ -795,15 +874,15 @@ static int batch_plan(int n_grids, const int* shapes, int k, int order, const do
   for (int i = 0; i < n_grids; ++i) {
     const int n0 = shapes[3 * i], n1 = shapes[3 * i + 1], n2 = shapes[3 * i + 2];
     if (n0 < 1 || n1 < 1 || n2 < 1) return fail(CDFT_E_ARG, "batch_plan: grid %d has a bad shape", i);
-    if ((long)(n0 + order) * (n1 + order) * (n2 + order) * (long)sizeof(T) >= 0x7fffffffL)
+    if ((long)(n0 + order) * (n1 + order) * (n2 + order) * 4 * (long)sizeof(T) >= 0x7fffffffL)
       return fail(CDFT_E_ARG, "batch_plan: grid %d too large for 32-bit offsets", i);
     const March d = march_dims(n0, n1, n2, k, order - 1);
     if (d.threads > NTMAX) return fail(CDFT_E_ARG, "batch_plan: grid %d needs %d threads", i, d.threads);
     if (tz == 0) tz = d.tz;
     if (d.tz != tz) return fail(CDFT_E_ARG, "batch_plan: grid %d has tile width %d, the batch %d", i, d.tz, tz);
     threads = d.threads > threads ? d.threads : threads;
-    const void* const* p = ptrs + 10 * i;
-    for (int j = 0; j < 10; ++j)
+    const void* const* p = ptrs + 11 * i;
+    for (int j = 0; j < 11; ++j)
       if (!p[j]) return fail(CDFT_E_ARG, "batch_plan: grid %d pointer %d is null", i, j);
     g[i].x = (const T*)p[0];
     g[i].out = (T*)p[1];
@@ -814,7 +893,8 @@ static int batch_plan(int n_grids, const int* shapes, int k, int order, const do
     g[i].w2 = (const T*)p[6];
     g[i].invw = (const T*)p[7];
     g[i].pot = (const T*)p[8];
-    g[i].table = (const double*)p[9];
+    g[i].gp = (const T*)p[9];
+    g[i].table = (const double*)p[10];
     for (int j = 0; j < 8; ++j) g[i].taps[j] = j < order ? coefs[order * i + j] : 0.0;
     g[i].n0 = n0;
     g[i].n1 = n1;

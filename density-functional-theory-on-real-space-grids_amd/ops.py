@@ -86,6 +86,11 @@ class StencilGeometry:
     invw: torch.Tensor
     potential: torch.Tensor
     taps: ctypes.Array
+    packed: torch.Tensor | None = None  # the geometry records of the stencil kernels (built on creation)
+
+    def __post_init__(self) -> None:
+        if self.packed is None:
+            self.packed = pack_records(self)
 
     @property
     def dtype(self) -> torch.dtype:
@@ -108,9 +113,12 @@ class StencilGeometry:
         require(self.w2, (n0, n1, n2 + o), self.dtype, self.device, "w2")
         require(self.invw, (n0 * n1 * n2,), self.dtype, self.device, "invw")
         require(self.potential, (n0 * n1 * n2,), self.dtype, self.device, "potential")
+        require(self.packed, (int(_lib.load().cdft_hdiv_pack_len(n0, n1, n2, o)),), self.dtype, self.device, "packed")
 
     def with_potential(self, potential: torch.Tensor) -> "StencilGeometry":
-        return StencilGeometry(self.shape, self.order, self.w0, self.w1, self.w2, self.invw, potential, self.taps)
+        """The same geometry with another diagonal potential (the records do not involve it)."""
+        return StencilGeometry(self.shape, self.order, self.w0, self.w1, self.w2, self.invw, potential, self.taps,
+                               self.packed)
 
     def to(self, dtype: torch.dtype) -> "StencilGeometry":
         if dtype == self.dtype:
@@ -118,6 +126,20 @@ class StencilGeometry:
         cast = lambda t: t.to(dtype).contiguous()  # noqa: E731
         return StencilGeometry(self.shape, self.order, cast(self.w0), cast(self.w1), cast(self.w2),
                                cast(self.invw), cast(self.potential), self.taps)
+
+
+def pack_records(geom: "StencilGeometry") -> torch.Tensor:
+    """The per-point geometry records (w0 at the plane's newest x-face, w1, w2, invw) over the
+    face-extended (y, z) rows that the stencil kernels read each plane (``cdft_hdiv_pack_*``)."""
+    n0, n1, n2 = geom.shape
+    length = int(_lib.load().cdft_hdiv_pack_len(n0, n1, n2, geom.order))
+    out = torch.empty(length, dtype=geom.w0.dtype, device=geom.w0.device)
+    for name, t in (("w0", geom.w0), ("w1", geom.w1), ("w2", geom.w2), ("invw", geom.invw)):
+        if t.dtype != out.dtype or not t.is_contiguous():
+            raise ValueError(f"StencilGeometry: {name} must be contiguous and of the geometry's dtype")
+    _lib.call(f"cdft_hdiv_pack_{_suffix(out.dtype)}", geom.w0.data_ptr(), geom.w1.data_ptr(), geom.w2.data_ptr(),
+              geom.invw.data_ptr(), n0, n1, n2, geom.order, out.data_ptr(), stream_of(out))
+    return out
 
 
 def _suffix(dtype: torch.dtype) -> str:
@@ -139,7 +161,7 @@ def hdiv_apply(phi: torch.Tensor, geom: StencilGeometry, out: torch.Tensor | Non
     require(out, (k, geom.n_points), geom.dtype, geom.device, "out")
     n0, n1, n2 = geom.shape
     _lib.call(f"cdft_hdiv_apply_{_suffix(geom.dtype)}", block.data_ptr(), geom.w0.data_ptr(), geom.w1.data_ptr(),
-              geom.w2.data_ptr(), geom.invw.data_ptr(), geom.potential.data_ptr(), out.data_ptr(), k, n0, n1, n2,
+              geom.w2.data_ptr(), geom.invw.data_ptr(), geom.potential.data_ptr(), geom.packed.data_ptr(), out.data_ptr(), k, n0, n1, n2,
               ctypes.addressof(geom.taps), geom.order, stream_of(block))
     return out.reshape(-1) if squeeze else out
 
@@ -157,7 +179,8 @@ def hdiv_apply_adjoint(x: torch.Tensor, geom: StencilGeometry, out: torch.Tensor
     require(work, (k, geom.n_points), geom.dtype, geom.device, "work")
     n0, n1, n2 = geom.shape
     _lib.call(f"cdft_hdiv_apply_adjoint_{_suffix(geom.dtype)}", block.data_ptr(), geom.w0.data_ptr(), geom.w1.data_ptr(),
-              geom.w2.data_ptr(), geom.invw.data_ptr(), geom.potential.data_ptr(), out.data_ptr(), work.data_ptr(), k,
+              geom.w2.data_ptr(), geom.invw.data_ptr(), geom.potential.data_ptr(), geom.packed.data_ptr(), out.data_ptr(),
+              work.data_ptr(), k,
               n0, n1, n2, ctypes.addressof(geom.taps), geom.order, stream_of(block))
     return out.reshape(-1) if squeeze else out
 
@@ -199,7 +222,7 @@ def hdiv_resnorm(x: torch.Tensor, geom: StencilGeometry, evals: torch.Tensor, q:
         _stencil_partials[key] = ws
     n0, n1, n2 = geom.shape
     _lib.call("cdft_hdiv_resnorm_f64", x.data_ptr(), geom.w0.data_ptr(), geom.w1.data_ptr(), geom.w2.data_ptr(),
-              geom.invw.data_ptr(), geom.potential.data_ptr(), evals.data_ptr(), q.data_ptr(), k, n0, n1, n2,
+              geom.invw.data_ptr(), geom.potential.data_ptr(), geom.packed.data_ptr(), evals.data_ptr(), q.data_ptr(), k, n0, n1, n2,
               ctypes.addressof(geom.taps), geom.order, ws.data_ptr(), ws.numel(), out.data_ptr(), stream_of(x))
     return out
 
@@ -211,7 +234,7 @@ def cheb_step(y: torch.Tensor, prev: torch.Tensor | None, geom: StencilGeometry,
     n0, n1, n2 = geom.shape
     require(table, (table.shape[0], 3), torch.float64, geom.device, "table")
     _lib.call(f"cdft_cheb_step_{_suffix(geom.dtype)}", y.data_ptr(), _ptr(prev), geom.w0.data_ptr(), geom.w1.data_ptr(),
-              geom.w2.data_ptr(), geom.invw.data_ptr(), geom.potential.data_ptr(), out.data_ptr(), k, n0, n1, n2,
+              geom.w2.data_ptr(), geom.invw.data_ptr(), geom.potential.data_ptr(), geom.packed.data_ptr(), out.data_ptr(), k, n0, n1, n2,
               ctypes.addressof(geom.taps), geom.order, table.data_ptr(), int(step), stream_of(y))
     return out
 
@@ -228,7 +251,7 @@ def cheb_filter(x: torch.Tensor, geom: StencilGeometry, table: torch.Tensor, deg
     n0, n1, n2 = geom.shape
     _lib.call(f"cdft_cheb_filter_{_suffix(geom.dtype)}", x.data_ptr(), out.data_ptr(), work[0].data_ptr(),
               work[1].data_ptr(), geom.w0.data_ptr(), geom.w1.data_ptr(), geom.w2.data_ptr(), geom.invw.data_ptr(),
-              geom.potential.data_ptr(), k, n0, n1, n2, ctypes.addressof(geom.taps), geom.order, table.data_ptr(),
+              geom.potential.data_ptr(), geom.packed.data_ptr(), k, n0, n1, n2, ctypes.addressof(geom.taps), geom.order, table.data_ptr(),
               int(degree), stream_of(x))
     return out
 

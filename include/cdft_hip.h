@@ -30,34 +30,49 @@ const char* cdft_last_error(void);
 /* ABI version of this header (1). */
 int cdft_abi_version(void);
 
+/* The geometry records every stencil entry below also takes (`gp`): for plane m and column (y, z) of
+ * the face-extended rows (y < n1 + order, z < n2 + order), four values (w0 at extended plane
+ * m + order - 1, w1 at (m, y, z), w2 at (m, y, z), invw at (m, y, z); zero where the array has no
+ * such entry), built from w0, w1, w2, invw by cdft_hdiv_pack_* into cdft_hdiv_pack_len(..) elements.
+ * A plane's four per-point geometry values are then two 16-byte loads instead of four. Built once per
+ * geometry (they do not involve pot). Part of the stencil interface below; no reference counterpart. */
+long cdft_hdiv_pack_len(int n0, int n1, int n2, int order);
+int cdft_hdiv_pack_f64(const double* w0, const double* w1, const double* w2, const double* invw, int n0,
+                       int n1, int n2, int order, double* gp, void* stream);
+int cdft_hdiv_pack_f32(const float* w0, const float* w1, const float* w2, const float* invw, int n0,
+                       int n1, int n2, int order, float* gp, void* stream);
+
 /* K1: out = 0.5 * invw * sum_d D_d^T [w_d D_d phi] + pot * phi for a block of k vectors.
  * Replaces CuspFactoredHamiltonian.apply / apply_with_potential / kinetic_apply
  * (src/cdft/operators/hamiltonian.py:251-318) and weighted_divergence
  * (src/cdft/operators/divergence.py:153-197, device tier fused.py:353-380). order in {2,4,6,8}. */
 int cdft_hdiv_apply_f64(const double* phi, const double* w0, const double* w1, const double* w2,
-                        const double* invw, const double* pot, double* out, int k, int n0, int n1,
-                        int n2, const double* coef, int order, void* stream);
+                        const double* invw, const double* pot, const double* gp, double* out, int k,
+                        int n0, int n1, int n2, const double* coef, int order, void* stream);
 int cdft_hdiv_apply_f32(const float* phi, const float* w0, const float* w1, const float* w2,
-                        const float* invw, const float* pot, float* out, int k, int n0, int n1,
-                        int n2, const double* coef, int order, void* stream);
+                        const float* invw, const float* pot, const float* gp, float* out, int k,
+                        int n0, int n1, int n2, const double* coef, int order, void* stream);
 
 /* The Euclidean transpose of K1 for the backward pass: out = 0.5 sum_d D_d^T[w_d D_d (invw x)]
  * + pot x (invw applied on the input). work: a distinct (k, N) scratch block. No reference
  * counterpart (the reference's inversion is a stub, src/cdft/inversion/__init__.py:1; interface
  * InversionProtocol contract.py:1263-1283). */
 int cdft_hdiv_apply_adjoint_f64(const double* x, const double* w0, const double* w1, const double* w2,
-                                const double* invw, const double* pot, double* out, double* work, int k,
-                                int n0, int n1, int n2, const double* coef, int order, void* stream);
+                                const double* invw, const double* pot, const double* gp, double* out,
+                                double* work, int k, int n0, int n1, int n2, const double* coef, int order,
+                                void* stream);
 int cdft_hdiv_apply_adjoint_f32(const float* x, const float* w0, const float* w1, const float* w2,
-                                const float* invw, const float* pot, float* out, float* work, int k,
-                                int n0, int n1, int n2, const double* coef, int order, void* stream);
+                                const float* invw, const float* pot, const float* gp, float* out,
+                                float* work, int k, int n0, int n1, int n2, const double* coef, int order,
+                                void* stream);
 
 /* K1 fused with the residual norms of Rayleigh-Ritz: norms[v] = ||A phi_v - evals[v] phi_v||_q
  * (sqrt of sum_p q_p r_p^2), with nothing stored but per-block partials (partials: at least the launch's
  * block count, cdft_hdiv_launch_shape(..)[5] doubles) combined in a fixed order. Replaces the apply and
  * the norm of residual_norms (src/cdft/eigen/rayleigh_ritz.py:105-115). */
 int cdft_hdiv_resnorm_f64(const double* phi, const double* w0, const double* w1, const double* w2,
-                          const double* invw, const double* pot, const double* evals, const double* q, int k,
+                          const double* invw, const double* pot, const double* gp, const double* evals,
+                          const double* q, int k,
                           int n0, int n1, int n2, const double* coef, int order, double* partials, long partials_len,
                           double* norms, void* stream);
 
@@ -66,11 +81,13 @@ int cdft_hdiv_resnorm_f64(const double* phi, const double* w0, const double* w1,
  * Replaces chebyshev_update (src/cdft/operators/fused.py:391-417) and the body of
  * chebyshev_recurrence (src/cdft/eigen/chefsi.py:139-160). */
 int cdft_cheb_step_f64(const double* y, const double* prev, const double* w0, const double* w1,
-                       const double* w2, const double* invw, const double* pot, double* out, int k,
+                       const double* w2, const double* invw, const double* pot, const double* gp,
+                       double* out, int k,
                        int n0, int n1, int n2, const double* coef, int order, const double* table,
                        int step, void* stream);
 int cdft_cheb_step_f32(const float* y, const float* prev, const float* w0, const float* w1,
-                       const float* w2, const float* invw, const float* pot, float* out, int k,
+                       const float* w2, const float* invw, const float* pot, const float* gp,
+                       float* out, int k,
                        int n0, int n1, int n2, const double* coef, int order, const double* table,
                        int step, void* stream);
 
@@ -79,11 +96,11 @@ int cdft_cheb_step_f32(const float* y, const float* prev, const float* w0, const
  * the captured GraphedChebyshevFilter replay (chefsi.py:185-392). */
 int cdft_cheb_filter_f64(const double* x, double* out, double* work1, double* work2,
                          const double* w0, const double* w1, const double* w2, const double* invw,
-                         const double* pot, int k, int n0, int n1, int n2, const double* coef,
+                         const double* pot, const double* gp, int k, int n0, int n1, int n2, const double* coef,
                          int order, const double* table, int degree, void* stream);
 int cdft_cheb_filter_f32(const float* x, float* out, float* work1, float* work2, const float* w0,
                          const float* w1, const float* w2, const float* invw, const float* pot,
-                         int k, int n0, int n1, int n2, const double* coef, int order,
+                         const float* gp, int k, int n0, int n1, int n2, const double* coef, int order,
                          const double* table, int degree, void* stream);
 
 /* ---- batched grids: many independent operators in one launch ----
@@ -94,8 +111,8 @@ int cdft_cheb_filter_f32(const float* x, float* out, float* work1, float* work2,
 long cdft_batch_desc_bytes(int n_grids);
 /* Host-only planning: fills `desc` (host buffer of cdft_batch_desc_bytes bytes, to be copied to the
  * device unchanged) for n_grids grids of one stencil order and block width k. shapes (host,
- * n_grids x 3), coefs (host, n_grids x order staggered weights / h), ptrs (host, n_grids x 10 device
- * pointers: x, out, work1, work2, w0, w1, w2, invw, pot, Chebyshev table (degree x 3)). launch (host,
+ * n_grids x 3), coefs (host, n_grids x order staggered weights / h), ptrs (host, n_grids x 11 device
+ * pointers: x, out, work1, work2, w0, w1, w2, invw, pot, geometry records, Chebyshev table (degree x 3)). launch (host,
  * 3 ints) receives {blocks, threads, tile width}; every grid must get the same tile width. */
 int cdft_batch_plan_f64(int n_grids, const int* shapes, int k, int order, const double* coefs,
                         const void* const* ptrs, void* desc, long desc_bytes, int* launch);

@@ -56,4 +56,4 @@ def test_abi_version_and_error_string(pkg):
 def test_null_arguments_are_rejected_before_any_launch(pkg):
     """Validation runs on the host: a null pointer returns CDFT_E_ARG without touching a device."""
     with pytest.raises(pkg._lib.HipLibraryError, match="null pointer"):
-        pkg._lib.call("cdft_hdiv_apply_f64", 0, 0, 0, 0, 0, 0, 0, 1, 4, 4, 4, 0, 8, 0)
+        pkg._lib.call("cdft_hdiv_apply_f64", 0, 0, 0, 0, 0, 0, 0, 0, 1, 4, 4, 4, 0, 8, 0)

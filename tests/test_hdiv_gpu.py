@@ -107,7 +107,7 @@ def test_invalid_arguments_raise(pkg, device):
     with pytest.raises(ValueError):
         pkg.ops.hdiv_apply(phi[:, :-1], geom)
     with pytest.raises(pkg._lib.HipLibraryError):
-        pkg._lib.call("cdft_hdiv_apply_f64", phi.data_ptr(), 0, 0, 0, 0, 0, phi.data_ptr(), 2, 9, 9, 9,
+        pkg._lib.call("cdft_hdiv_apply_f64", phi.data_ptr(), 0, 0, 0, 0, 0, 0, phi.data_ptr(), 2, 9, 9, 9,
                       0, 8, 0)
 
 
@@ -138,3 +138,25 @@ def test_fused_residual_norms_match_apply_then_norm(pkg, device, order, shape, k
     assert np.abs(fused.cpu().numpy() - numpy_norm).max() <= 1e-13 * numpy_norm.max()
     again = pkg.ops.hdiv_resnorm(x, geom, evals, q, torch.empty(k, dtype=torch.float64, device=device))
     assert torch.equal(again, pkg.ops.hdiv_resnorm(x, geom, evals, q, torch.empty(k, dtype=torch.float64, device=device)))
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_geometry_records_hold_the_separate_arrays(pkg, device, dtype):
+    """The per-plane geometry records are the face weights and invw at each column of the face-extended
+    rows (the newest x-face, the y- and z-faces, invw), zero where an array has no such entry."""
+    shape, order = (7, 5, 9), 8
+    rng = np.random.default_rng(4)
+    n = int(np.prod(shape))
+    faces = [rng.uniform(0.5, 1.5, size=tuple(s + (order if d == a else 0) for d, s in enumerate(shape))) for a in range(3)]
+    invw = rng.uniform(0.5, 2.0, n)
+    geom = pkg.geometry_from_arrays(shape, order, 0.25, faces, invw, rng.uniform(-1, 1, n), device, dtype=dtype)
+    n0, n1, n2 = shape
+    rec = geom.packed.view(n0, n1 + order, n2 + order, 4).double().cpu().numpy()
+    cast = lambda a: np.asarray(torch.as_tensor(a).to(dtype).double())  # noqa: E731
+    w0, w1, w2, iw = cast(faces[0]), cast(faces[1]), cast(faces[2]), cast(invw).reshape(shape)
+    expect = np.zeros_like(rec)
+    expect[:, :n1, :n2, 0] = w0[order - 1: order - 1 + n0]
+    expect[:, :, :n2, 1] = w1
+    expect[:, :n1, :, 2] = w2
+    expect[:, :n1, :n2, 3] = iw
+    assert np.array_equal(rec, expect)
