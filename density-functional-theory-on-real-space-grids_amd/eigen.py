@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import ops
-from .hamiltonian import lanczos_enqueue, lanczos_finish, seeded_randn
+from .hamiltonian import DeviceMeasure, lanczos_enqueue, lanczos_finish, seeded_randn
 
 BOUND_REUSE_RITZ_FRACTION = 0.9
 
@@ -228,7 +228,8 @@ class ChebyshevFilteredSubspace:
         n = grid.n_points
         self._pot = torch.empty(n, dtype=torch.float64, device=grid.device)
         self._pot32 = torch.empty(n, dtype=torch.float32, device=grid.device) if filter_dtype != torch.float64 else None
-        self._geom_key = None
+        self._sources = None
+        self._layout = None
         self._geom64 = self._geom32 = None
         self._measure = None
 
@@ -242,22 +243,55 @@ class ChebyshevFilteredSubspace:
         return self._work[k]
 
     def _bind(self, hamiltonian) -> None:
-        """Point the solver's static geometry at this operator: copy its diagonal potential into the
-        static buffer the captured graphs read (the face weights, ``invw`` and ``q`` are
-        geometry-only and shared by every operator of a solve)."""
+        """Copy this operator into the static geometry the captured graphs read: its diagonal
+        potential on every call; its face weights, records, ``invw`` and ``q`` when they are other
+        arrays than the last bound ones (a new solve). The graphs are recaptured only when the
+        shape, order, taps or device change: the taps are kernel arguments, captured by value.
+        The last bound arrays stay referenced, so their addresses cannot be handed to new arrays."""
         geom = hamiltonian.geometry
-        key = (geom.w0.data_ptr(), geom.w1.data_ptr(), geom.w2.data_ptr(), geom.invw.data_ptr(),
-               hamiltonian.measure.q.data_ptr())
-        if key != self._geom_key:
-            self._geom_key = key
-            self._graphs.clear()
-            self._geom64 = geom.with_potential(self._pot)
-            if self.mixed:
-                self._geom32 = hamiltonian.geometry_in(self.filter_dtype).with_potential(self._pot32)
-            self._measure = hamiltonian.measure
+        q = hamiltonian.measure.q
+        sources = (geom.w0, geom.w1, geom.w2, geom.invw, geom.packed, q)
+        if self._sources is None or any(a is not b for a, b in zip(sources, self._sources)):
+            layout = (geom.shape, geom.order, tuple(geom.taps), geom.device, geom.dtype)
+            if layout != self._layout:
+                self._layout = layout
+                self._graphs.clear()
+                clone = lambda t: t.clone()  # noqa: E731
+                self._geom64 = ops.StencilGeometry(geom.shape, geom.order, clone(geom.w0), clone(geom.w1),
+                                                   clone(geom.w2), clone(geom.invw), self._pot, geom.taps,
+                                                   clone(geom.packed))
+                self._measure = DeviceMeasure(self.grid, q.clone())
+                if self.mixed:
+                    g, cast = self._geom64, lambda t: t.to(self.filter_dtype)  # noqa: E731
+                    # the records hold the arrays' values and zeros only, so their cast is the records
+                    # of the cast arrays
+                    self._geom32 = ops.StencilGeometry(g.shape, g.order, cast(g.w0), cast(g.w1), cast(g.w2),
+                                                       cast(g.invw), self._pot32, g.taps, cast(g.packed))
+            else:
+                targets = (self._geom64.w0, self._geom64.w1, self._geom64.w2, self._geom64.invw,
+                           self._geom64.packed, self._measure.q)
+                for dst, src in zip(targets, sources):
+                    dst.copy_(src)
+                if self.mixed:
+                    g, g32 = self._geom64, self._geom32
+                    for dst, src in ((g32.w0, g.w0), (g32.w1, g.w1), (g32.w2, g.w2), (g32.invw, g.invw),
+                                     (g32.packed, g.packed)):
+                        dst.copy_(src)
+            self._sources = sources
         self._pot.copy_(hamiltonian.potential)
         if self.mixed:
             self._pot32.copy_(self._pot)
+
+    def begin_solve(self, grid, config: EigenConfig, filter_hook=None) -> None:
+        """Start a new solve with this solver (its buffers, static geometry and captured graphs are
+        kept): clear the per-solve records and take the solve's grid, configuration and hook."""
+        self.grid = grid
+        self.config = config
+        self.filter_hook = filter_hook
+        self.graph_record = {"captures": 0, "replays": 0, "disabled": self.graph_record["disabled"]}
+        self.history = []
+        self.filter_precisions = []
+        self.speculation = {"used": 0, "redone": 0, "wasted": 0}
 
     # ---- one iteration ----
 

@@ -230,7 +230,13 @@ class CuspFactoredHamiltonian:
         order = grid.fd_order
         faces = factor.face_weights(order)
         taps = host_doubles([c * (1.0 / grid.spacing) for c in staggered_coefficients(order)])
-        self.geometry = StencilGeometry(grid.shape, order, faces[0], faces[1], faces[2], invw, potential, taps)
+        # the records depend on the faces and invw only: the operators of one quadrature share them
+        records = quadrature.__dict__.setdefault("_stencil_records", {})
+        held = records.get(order)
+        packed = held[2] if held is not None and held[0] is faces and held[1] is invw else None
+        self.geometry = StencilGeometry(grid.shape, order, faces[0], faces[1], faces[2], invw, potential, taps,
+                                        packed)
+        records[order] = (faces, invw, self.geometry.packed)
         self.measure = DeviceMeasure(grid, q)
         self.apply_count = 0
         self._kinetic_geometry = None

@@ -398,6 +398,27 @@ def initial_density(step: KohnShamStep) -> torch.Tensor:
 #: Float64 filter passes that finish a mixed-precision solve.
 REFINEMENT_PASSES = 2
 
+#: Subspace solvers kept between SCF solves (least recently used last out), by grid shape, device,
+#: eigensolver settings and filter precision: a later solve of the same kind reuses their buffers and
+#: captured graphs, and copies its own geometry into their static arrays.
+SOLVER_CACHE_SIZE = 2
+_solvers: dict = {}
+
+
+def subspace_solver(grid, config: EigenConfig, filter_dtype, filter_hook) -> ChebyshevFilteredSubspace:
+    """The SCF's subspace solver: a cached one of the same kind, started afresh, or a new one."""
+    key = (grid.shape, str(grid.device), repr(config), filter_dtype)
+    solver = _solvers.pop(key, None)
+    if solver is None:
+        solver = ChebyshevFilteredSubspace(grid, config, ritz_stop=False, filter_dtype=filter_dtype,
+                                           filter_hook=filter_hook)
+    else:
+        solver.begin_solve(grid, config, filter_hook)
+    _solvers[key] = solver
+    while len(_solvers) > SOLVER_CACHE_SIZE:
+        _solvers.pop(next(iter(_solvers)))
+    return solver
+
 
 def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None, filter_dtype=torch.float64,
                           filter_hook=None):
@@ -424,8 +445,7 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
     step.draws = store
     g = grid
     inner_weights = ops.ewise(ops.EW_DIV, g.empty(), q.mass_weights, ops.ewise(ops.EW_MUL, g.empty(), step.f2, step.f2))
-    solver = ChebyshevFilteredSubspace(grid, numerics.eigen, ritz_stop=False, filter_dtype=filter_dtype,
-                                       filter_hook=filter_hook)
+    solver = subspace_solver(grid, numerics.eigen, filter_dtype, filter_hook)
     n_in = initial_density(step)
     mixer = PulayMixer(numerics.alpha, numerics.history, numerics.pulay_period, inner_weights)
     ladder = [

@@ -283,3 +283,30 @@ def test_speculative_filters_change_nothing(pkg, device, monkeypatch):
         assert runs[a]["applications"] == runs[b]["applications"]
     assert runs["on"]["speculative_filters"]["used"] > 0
     assert runs["off"]["speculative_filters"]["used"] == 0
+
+
+def test_reused_solver_matches_a_fresh_one(pkg, device, monkeypatch):
+    """An SCF solve that reuses the cached subspace solver of an earlier solve with other atoms on the
+    same grid shape (its buffers and captured graphs, with the new geometry copied into its static
+    arrays) gives bitwise the energies, eigenvalues and application counts of a solve with a fresh
+    solver; float64 and float32 filters."""
+    from cdft_amd import scf
+
+    first = ophys.Scenario((1.0, 1.0), ((-0.7, 0.0, 0.0), (0.7, 0.0, 0.0)), 2.0, "lda")
+    second = ophys.Scenario((1.0, 1.0), ((-0.75, 0.05, 0.0), (0.75, 0.05, 0.0)), 2.0, "lda")
+    num = ophys.Numerics(box_lengths=(9.8, 8.4, 8.4), spacing=0.35)
+    for dtype in (torch.float64, torch.float32):
+        monkeypatch.setattr(scf, "_solvers", {})
+        monkeypatch.setattr(scf, "SOLVER_CACHE_SIZE", 2)
+        scf.solve_self_consistent(first, num, device, filter_dtype=dtype)
+        reused = scf.solve_self_consistent(second, num, device, filter_dtype=dtype)
+        monkeypatch.setattr(scf, "_solvers", {})
+        monkeypatch.setattr(scf, "SOLVER_CACHE_SIZE", 0)
+        fresh = scf.solve_self_consistent(second, num, device, filter_dtype=dtype)
+        assert reused["step"].grid.shape == fresh["step"].grid.shape
+        assert reused["energy_history"] == fresh["energy_history"], dtype
+        assert np.array_equal(reused["eigenvalues"], fresh["eigenvalues"]), dtype
+        assert reused["applications"] == fresh["applications"], dtype
+        assert reused["filter_precisions"] == fresh["filter_precisions"], dtype
+        assert reused["graphs"]["disabled"] is None and fresh["graphs"]["disabled"] is None
+        assert reused["graphs"]["captures"] < fresh["graphs"]["captures"], (reused["graphs"], fresh["graphs"])
