@@ -346,6 +346,25 @@ __global__ __launch_bounds__(64) void chol_inv_kernel(const double* __restrict__
   flag[0] = ok ? 0 : 1;
 }
 
+// The Jacobi rotation (cs, sn) that zeroes apq in the symmetric 2 x 2 block [[app, apq], [apq, aqq]]:
+// t = tan(angle) is the root of smaller magnitude of t^2 + 2 theta t - 1 = 0 with
+// theta = (aqq - app) / (2 apq), i.e. t = sgn(theta) / (|theta| + sqrt(theta^2 + 1)). Multiplying
+// through by 2 |apq| gives t = 2 apq sgn(d) / (|d| + sqrt(d^2 + 4 apq^2)), d = aqq - app: one square
+// root and one divide where theta's form takes two of each, then cs = rsqrt(1 + t^2), sn = t cs.
+// These long-latency operations are the critical path of the one-thread kernels below. When
+// d = 0 and apq^2 underflows, the denominator is zero and t = 1 (the 45-degree rotation of d = 0).
+// The rotated block is diagonal with app - t apq and aqq + t apq (Golub and Van Loan, the
+// symmetric Schur decomposition), and rows r != p, q of columns p and q rotate by (cs, sn); the
+// kernels write each rotated entry to both of its symmetric places.
+__device__ __forceinline__ void jacobi_rotation(double app, double aqq, double apq, double& cs, double& sn,
+                                                double& t) {
+  const double d = aqq - app;
+  const double den = fabs(d) + sqrt(d * d + 4.0 * apq * apq);
+  t = den > 0.0 ? (d >= 0.0 ? 2.0 * apq : -2.0 * apq) / den : 1.0;
+  cs = rsqrt(t * t + 1.0);
+  sn = t * cs;
+}
+
 // Cyclic Jacobi on 0.5 (M + M^T). Writes eigenvalues ascending and C with eigenvectors in columns
 // (C[i*k + j] = component i of eigenvector j), each column's largest entry made positive.
 __global__ __launch_bounds__(64) void eigh_kernel(const double* __restrict__ m_in, int k, double* __restrict__ evals,
@@ -369,19 +388,16 @@ __global__ __launch_bounds__(64) void eigh_kernel(const double* __restrict__ m_i
         const double apq = a[p * k + q];
         if (apq == 0.0) continue;
         const double app = a[p * k + p], aqq = a[q * k + q];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double cs = 1.0 / sqrt(tt * tt + 1.0), sn = tt * cs;
+        double cs, sn, t;
+        jacobi_rotation(app, aqq, apq, cs, sn, t);
         for (int r = 0; r < k; ++r) {
+          if (r == p || r == q) continue;
           const double arp = a[r * k + p], arq = a[r * k + q];
-          a[r * k + p] = cs * arp - sn * arq;
-          a[r * k + q] = sn * arp + cs * arq;
+          a[r * k + p] = a[p * k + r] = cs * arp - sn * arq;
+          a[r * k + q] = a[q * k + r] = sn * arp + cs * arq;
         }
-        for (int r = 0; r < k; ++r) {
-          const double apr = a[p * k + r], aqr = a[q * k + r];
-          a[p * k + r] = cs * apr - sn * aqr;
-          a[q * k + r] = sn * apr + cs * aqr;
-        }
+        a[p * k + p] = app - t * apq;
+        a[q * k + q] = aqq + t * apq;
         a[p * k + q] = a[q * k + p] = 0.0;
         for (int r = 0; r < k; ++r) {
           const double vrp = v[r * k + p], vrq = v[r * k + q];
@@ -486,21 +502,17 @@ __global__ __launch_bounds__(64) void eigh_small_kernel(const double* __restrict
         const double apq = a[p][q];
         if (apq != 0.0) {
           const double app = a[p][p], aqq = a[q][q];
-          const double theta = (aqq - app) / (2.0 * apq);
-          const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-          const double cs = 1.0 / sqrt(tt * tt + 1.0), sn = tt * cs;
+          double cs, sn, t;
+          jacobi_rotation(app, aqq, apq, cs, sn, t);
 #pragma unroll
           for (int r = 0; r < K; ++r) {
+            if (r == p || r == q) continue;
             const double arp = a[r][p], arq = a[r][q];
-            a[r][p] = cs * arp - sn * arq;
-            a[r][q] = sn * arp + cs * arq;
+            a[r][p] = a[p][r] = cs * arp - sn * arq;
+            a[r][q] = a[q][r] = sn * arp + cs * arq;
           }
-#pragma unroll
-          for (int r = 0; r < K; ++r) {
-            const double apr = a[p][r], aqr = a[q][r];
-            a[p][r] = cs * apr - sn * aqr;
-            a[q][r] = sn * apr + cs * aqr;
-          }
+          a[p][p] = app - t * apq;
+          a[q][q] = aqq + t * apq;
           a[p][q] = a[q][p] = 0.0;
 #pragma unroll
           for (int r = 0; r < K; ++r) {
