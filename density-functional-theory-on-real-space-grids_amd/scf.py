@@ -406,7 +406,9 @@ _solvers: dict = {}
 
 
 def subspace_solver(grid, config: EigenConfig, filter_dtype, filter_hook) -> ChebyshevFilteredSubspace:
-    """The SCF's subspace solver: a cached one of the same kind, started afresh, or a new one."""
+    """The SCF's subspace solver: a cached one of the same kind, started afresh, or a new one. A cached
+    solver holds one solve's state at a time: solves that run at once (several host threads) must
+    not share the cache (set ``SOLVER_CACHE_SIZE = 0``)."""
     key = (grid.shape, str(grid.device), repr(config), filter_dtype)
     solver = _solvers.pop(key, None)
     if solver is None:
