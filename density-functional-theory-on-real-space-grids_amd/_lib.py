@@ -52,6 +52,7 @@ SIGNATURES: dict[str, list] = {
     "cdft_cheb_table_f64": [P, I, P, I, P, P],
     "cdft_rotate_gram_f64": [P, I, P, P, P, L, P, L, P, P],
     "cdft_hdiv_resnorm_f64": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, L, P, P],
+    "cdft_hdiv_resnorm_tail_f64": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, L, P, I, P, I, P, P, P, P],
     "cdft_cusp_grid_fields_f64": [I, P, P, P, D, P, D, P, P, D, P, P, P, P, P],
     "cdft_face_weights_f64": [I, P, P, P, D, P, I, P, P, P, P, P],
     "cdft_node_fields_f64": [P, P, P, L, I, P, P, P, D, P, P, P, P, P, P],

@@ -19,6 +19,7 @@
 #include <type_traits>
 
 #include "common.h"
+#include "ritz_tail.h"
 
 namespace cdft {
 
@@ -754,21 +755,46 @@ extern "C" int cdft_hdiv_apply_adjoint_f32(const float* x, const float* w0, cons
 namespace cdft {
 // ||A phi_v - e_v phi_v||_q from the stencil blocks' partials: one wavefront per vector, lane l sums the
 // partials of tiles l, l+64, ... in order, then a fixed butterfly.
-__global__ __launch_bounds__(64) void resnorm_tiles_kernel(const double* __restrict__ partials, int tiles, int k,
-                                                           double* __restrict__ norms) {
-  const int v = blockIdx.x;
+__device__ __forceinline__ void resnorm_combine(const double* __restrict__ partials, int tiles, int k, int v,
+                                                double* __restrict__ norms) {
   double s = 0.0;
   for (int t = threadIdx.x; t < tiles; t += 64) s += partials[(long)t * k + v];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if (threadIdx.x == 0) norms[v] = sqrt(s > 0.0 ? s : 0.0);
 }
+
+__global__ __launch_bounds__(64) void resnorm_tiles_kernel(const double* __restrict__ partials, int tiles, int k,
+                                                           double* __restrict__ norms) {
+  resnorm_combine(partials, tiles, k, blockIdx.x, norms);
+}
+
+// The end of a Rayleigh-Ritz pass in one launch of k + 2 blocks: blocks 0..k-1 combine the residual
+// norms, block k builds the next filter's Chebyshev table, block k + 1 writes max |G - I|. The three
+// are independent (the table reads the eigenvalues and bounds, the error reads the Gram matrix).
+__global__ __launch_bounds__(64) void resnorm_tail_kernel(const double* __restrict__ partials, int tiles, int k,
+                                                          double* __restrict__ norms, const double* __restrict__ evals,
+                                                          int index, const double* __restrict__ bounds, int degree,
+                                                          double* __restrict__ table, const double* __restrict__ gram,
+                                                          double* __restrict__ orth) {
+  const int b = blockIdx.x;
+  if (b < k) {
+    resnorm_combine(partials, tiles, k, b, norms);
+    return;
+  }
+  if (threadIdx.x != 0) return;
+  if (b == k)
+    cheb_table_rows(evals, index, bounds, degree, table);
+  else
+    max_abs_minus_identity(gram, k, orth);
+}
 }  // namespace cdft
 
-extern "C" int cdft_hdiv_resnorm_f64(const double* phi, const double* w0, const double* w1, const double* w2,
-                                     const double* invw, const double* pot, const double* gp, const double* evals, const double* q, int k,
-                                     int n0, int n1, int n2, const double* coef, int order, double* partials,
-                                     long partials_len, double* norms, void* stream) {
+// The stencil launch of the residual norms (MODE 4); returns the number of tile partials per vector.
+static int resnorm_apply(const double* phi, const double* w0, const double* w1, const double* w2,
+                         const double* invw, const double* pot, const double* gp, const double* evals, const double* q,
+                         int k, int n0, int n1, int n2, const double* coef, int order, double* partials,
+                         long partials_len, const double* norms, hipStream_t s, int* tiles) {
   int rc = validate(phi, partials, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order);
   if (rc) return rc;
   if (!evals || !q || !norms) return fail(CDFT_E_ARG, "hdiv_resnorm: null evals, q or norms");
@@ -776,11 +802,39 @@ extern "C" int cdft_hdiv_resnorm_f64(const double* phi, const double* w0, const 
   const long blocks = (long)d.tiles * k;
   if (partials_len < blocks)
     return fail(CDFT_E_CAPACITY, "hdiv_resnorm: partials hold %ld doubles, the launch needs %ld", partials_len, blocks);
+  *tiles = (int)d.tiles;
+  return launch<double, 4>(phi, q, partials, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order, evals, 0, s);
+}
+
+extern "C" int cdft_hdiv_resnorm_f64(const double* phi, const double* w0, const double* w1, const double* w2,
+                                     const double* invw, const double* pot, const double* gp, const double* evals, const double* q, int k,
+                                     int n0, int n1, int n2, const double* coef, int order, double* partials,
+                                     long partials_len, double* norms, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  rc = launch<double, 4>(phi, q, partials, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, order, evals, 0, s);
+  int tiles = 0;
+  int rc = resnorm_apply(phi, w0, w1, w2, invw, pot, gp, evals, q, k, n0, n1, n2, coef, order, partials, partials_len,
+                         norms, s, &tiles);
   if (rc) return rc;
-  hipLaunchKernelGGL(resnorm_tiles_kernel, dim3(k), dim3(64), 0, s, partials, (int)d.tiles, k, norms);
+  hipLaunchKernelGGL(resnorm_tiles_kernel, dim3(k), dim3(64), 0, s, partials, tiles, k, norms);
   return check_launch("resnorm_tiles");
+}
+
+extern "C" int cdft_hdiv_resnorm_tail_f64(const double* phi, const double* w0, const double* w1, const double* w2,
+                                          const double* invw, const double* pot, const double* gp, const double* evals,
+                                          const double* q, int k, int n0, int n1, int n2, const double* coef, int order,
+                                          double* partials, long partials_len, double* norms, int index,
+                                          const double* bounds, int degree, double* table, const double* gram,
+                                          double* orth, void* stream) {
+  if (!bounds || !table || !gram || !orth || index < 0 || index >= k || degree < 1)
+    return fail(CDFT_E_ARG, "hdiv_resnorm_tail: null bounds, table, gram or orth, or index/degree out of range");
+  hipStream_t s = (hipStream_t)stream;
+  int tiles = 0;
+  int rc = resnorm_apply(phi, w0, w1, w2, invw, pot, gp, evals, q, k, n0, n1, n2, coef, order, partials, partials_len,
+                         norms, s, &tiles);
+  if (rc) return rc;
+  hipLaunchKernelGGL(resnorm_tail_kernel, dim3(k + 2), dim3(64), 0, s, partials, tiles, k, norms, evals, index, bounds,
+                     degree, table, gram, orth);
+  return check_launch("resnorm_tail");
 }
 
 extern "C" long cdft_hdiv_pack_len(int n0, int n1, int n2, int order) {

@@ -138,10 +138,11 @@ class RitzWork:
         return self.scalars[self.k: 2 * self.k]
 
 
-def rayleigh_ritz(geometry, measure, block: torch.Tensor, work: RitzWork) -> None:
+def rayleigh_ritz(geometry, measure, block: torch.Tensor, work: RitzWork, orth: bool = True) -> None:
     """Orthonormalise ``block`` (Cholesky, ``T = L^-1``), project, diagonalise, rotate: eigenvalues
     in ``work.evals``, Ritz vectors in ``work.vectors``, ``max|G - I|`` in ``work.scalars[2k]``; a
-    non-positive-definite Gram matrix sets ``work.flag``."""
+    non-positive-definite Gram matrix sets ``work.flag``. ``orth=False`` leaves ``max|G - I|`` to the
+    caller (the residual-norm launch can compute it from ``work.G``)."""
     k = work.k
     measure.gram(block, out=work.S)
     ops.chol_inv(work.S, work.T, work.flag)
@@ -154,7 +155,8 @@ def rayleigh_ritz(geometry, measure, block: torch.Tensor, work: RitzWork) -> Non
     else:
         ops.rotate(work.C, work.basis, transpose=True, out=work.vectors)
         measure.gram(work.vectors, out=work.G)
-    ops.orth_error(work.G, work.scalars[2 * k: 2 * k + 1])
+    if orth:
+        ops.orth_error(work.G, work.scalars[2 * k: 2 * k + 1])
 
 
 def residual_norms(geometry, measure, work: RitzWork) -> None:
@@ -306,10 +308,13 @@ class ChebyshevFilteredSubspace:
             work.filtered.copy_(work.y32)
 
     def _ritz(self, work: RitzWork) -> None:
-        rayleigh_ritz(self._geom64, self._measure, work.filtered, work)
-        residual_norms(self._geom64, self._measure, work)
-        # the next filter's table, from this pass's last Ritz value and the current bounds
-        ops.cheb_table(work.evals, work.k - 1, work.bounds, self.config.chebyshev_degree, work.table)
+        rayleigh_ritz(self._geom64, self._measure, work.filtered, work, orth=False)
+        # the residual norms, and in the same combine launch the next filter's table (from this
+        # pass's last Ritz value and the current bounds) and the orthonormality error
+        k = work.k
+        ops.hdiv_resnorm(work.vectors, self._geom64, work.evals, self._measure.q, work.norms,
+                         tail=(k - 1, work.bounds, self.config.chebyshev_degree, work.table, work.G,
+                               work.scalars[2 * k: 2 * k + 1]))
 
     def _replay(self, key: tuple, body) -> None:
         """Run ``body`` through its captured graph (captured on first use), or eagerly when capture

@@ -202,8 +202,12 @@ _stencil_partials: dict[str, torch.Tensor] = {}
 
 
 def hdiv_resnorm(x: torch.Tensor, geom: StencilGeometry, evals: torch.Tensor, q: torch.Tensor,
-                 out: torch.Tensor) -> torch.Tensor:
-    """``out[v] = ||A x_v - evals[v] x_v||_q`` in one stencil launch plus a small combine (float64)."""
+                 out: torch.Tensor, tail: tuple | None = None) -> torch.Tensor:
+    """``out[v] = ||A x_v - evals[v] x_v||_q`` in one stencil launch plus a small combine (float64).
+
+    ``tail = (index, bounds, degree, table, gram, orth)`` also builds the next filter's Chebyshev
+    table (as :func:`cheb_table` from ``evals[index]`` and ``bounds``) and writes ``max|gram - I|``
+    to ``orth`` (as :func:`orth_error`) in the combine's launch, bitwise the separate calls."""
     k = x.shape[0]
     if geom.dtype != torch.float64:
         raise ValueError("hdiv_resnorm: float64 only")
@@ -221,9 +225,21 @@ def hdiv_resnorm(x: torch.Tensor, geom: StencilGeometry, evals: torch.Tensor, q:
         ws = torch.empty(max(blocks, 4096), dtype=torch.float64, device=geom.device)
         _stencil_partials[key] = ws
     n0, n1, n2 = geom.shape
-    _lib.call("cdft_hdiv_resnorm_f64", x.data_ptr(), geom.w0.data_ptr(), geom.w1.data_ptr(), geom.w2.data_ptr(),
-              geom.invw.data_ptr(), geom.potential.data_ptr(), geom.packed.data_ptr(), evals.data_ptr(), q.data_ptr(), k, n0, n1, n2,
-              ctypes.addressof(geom.taps), geom.order, ws.data_ptr(), ws.numel(), out.data_ptr(), stream_of(x))
+    args = (x.data_ptr(), geom.w0.data_ptr(), geom.w1.data_ptr(), geom.w2.data_ptr(), geom.invw.data_ptr(),
+            geom.potential.data_ptr(), geom.packed.data_ptr(), evals.data_ptr(), q.data_ptr(), k, n0, n1, n2,
+            ctypes.addressof(geom.taps), geom.order, ws.data_ptr(), ws.numel(), out.data_ptr())
+    if tail is None:
+        _lib.call("cdft_hdiv_resnorm_f64", *args, stream_of(x))
+    else:
+        index, bounds, degree, table, gram, orth = tail
+        require(bounds, (2,), torch.float64, geom.device, "bounds")
+        require(table, (degree, 3), torch.float64, geom.device, "table")
+        require(gram, (k, k), torch.float64, geom.device, "gram")
+        require(orth, (1,), torch.float64, geom.device, "orth")
+        if not 0 <= index < k:
+            raise ValueError("hdiv_resnorm: the table's eigenvalue index must be below k")
+        _lib.call("cdft_hdiv_resnorm_tail_f64", *args, int(index), bounds.data_ptr(), int(degree), table.data_ptr(),
+                  gram.data_ptr(), orth.data_ptr(), stream_of(x))
     return out
 
 

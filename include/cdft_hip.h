@@ -76,6 +76,17 @@ int cdft_hdiv_resnorm_f64(const double* phi, const double* w0, const double* w1,
                           int n0, int n1, int n2, const double* coef, int order, double* partials, long partials_len,
                           double* norms, void* stream);
 
+/* cdft_hdiv_resnorm_f64 with the end of a Rayleigh-Ritz pass folded into its combine launch: also
+ * builds the next filter's Chebyshev table from evals[index] and bounds (as cdft_cheb_table_f64) and
+ * writes max |gram - I| of the k x k gram to orth[0] (as cdft_orth_error_f64), bitwise the separate
+ * calls. No single reference function: it ends the CheFSI pass of src/cdft/eigen/chefsi.py:499-547
+ * (residual_norms, the next filter's rows of chefsi.py:116-136, the orthonormality error). */
+int cdft_hdiv_resnorm_tail_f64(const double* phi, const double* w0, const double* w1, const double* w2,
+                               const double* invw, const double* pot, const double* gp, const double* evals,
+                               const double* q, int k, int n0, int n1, int n2, const double* coef, int order,
+                               double* partials, long partials_len, double* norms, int index, const double* bounds,
+                               int degree, double* table, const double* gram, double* orth, void* stream);
+
 /* K2: one fused scaled-Chebyshev step, out = (A y - c y) a - b prev with (c, a, b) = table[step]
  * read on the device (table is a (degree, 3) float64 device array), prev ignored at step 0.
  * Replaces chebyshev_update (src/cdft/operators/fused.py:391-417) and the body of

@@ -140,6 +140,43 @@ def test_fused_residual_norms_match_apply_then_norm(pkg, device, order, shape, k
     assert torch.equal(again, pkg.ops.hdiv_resnorm(x, geom, evals, q, torch.empty(k, dtype=torch.float64, device=device)))
 
 
+@pytest.mark.parametrize("k,degree", [(5, 20), (1, 3), (8, 12)])
+def test_residual_norm_tail_is_the_separate_calls(pkg, device, k, degree):
+    """The residual-norm launch with the pass's tail (the next filter's Chebyshev table and the
+    orthonormality error in the combine launch) gives bitwise the residual norms, ``cheb_table`` and
+    ``orth_error`` of the separate calls; the table also equals the oracle's host rows."""
+    from oracle.eigen import chebyshev_coefficients
+
+    rng = np.random.default_rng(40 + k)
+    shape = (29, 25, 27)
+    n = int(np.prod(shape))
+    faces = [rng.uniform(0.5, 1.5, size=tuple(s + (8 if d == a else 0) for d, s in enumerate(shape))) for a in range(3)]
+    geom = pkg.geometry_from_arrays(shape, 8, 0.35, faces, rng.uniform(0.5, 2.0, n), rng.uniform(-1, 1, n), device)
+    f = dict(dtype=torch.float64, device=device)
+    x = torch.as_tensor(rng.uniform(-1, 1, size=(k, n))).to(device)
+    q = torch.as_tensor(rng.uniform(0.1, 1.5, size=n)).to(device)
+    evals = torch.as_tensor(np.sort(rng.uniform(-2, 2, size=k))).to(device)
+    bounds = torch.tensor([-2.5, 40.0], **f)
+    gram = torch.eye(k, **f) + torch.as_tensor(rng.uniform(-1e-9, 1e-9, size=(k, k))).to(device)
+    norms_a, norms_b = torch.empty(k, **f), torch.empty(k, **f)
+    table_a, table_b = torch.full((degree, 3), 7.0, **f), torch.full((degree, 3), -7.0, **f)
+    orth_a, orth_b = torch.full((1,), 7.0, **f), torch.full((1,), -7.0, **f)
+    pkg.ops.hdiv_resnorm(x, geom, evals, q, norms_a)
+    pkg.ops.cheb_table(evals, k - 1, bounds, degree, table_a)
+    pkg.ops.orth_error(gram, orth_a)
+    pkg.ops.hdiv_resnorm(x, geom, evals, q, norms_b, tail=(k - 1, bounds, degree, table_b, gram, orth_b))
+    torch.cuda.synchronize()
+    assert torch.equal(norms_a, norms_b)
+    assert torch.equal(table_a, table_b)
+    assert torch.equal(orth_a, orth_b)
+    assert orth_b.item() == (gram - torch.eye(k, **f)).abs().max().item()
+    ev = evals.cpu().numpy()
+    host = chebyshev_coefficients(degree, float(ev[k - 1]), 40.0, -2.5)  # lower inside the clamp
+    assert np.array_equal(table_b.cpu().numpy(), np.asarray(host, dtype=np.float64).reshape(degree, 3))
+    with pytest.raises(Exception):
+        pkg.ops.hdiv_resnorm(x, geom, evals, q, norms_b, tail=(k, bounds, degree, table_b, gram, orth_b))
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_geometry_records_hold_the_separate_arrays(pkg, device, dtype):
     """The per-plane geometry records are the face weights and invw at each column of the face-extended
