@@ -7,6 +7,7 @@ HIP stream of the tensors' device, so every kernel is ordered with the caller's 
 from __future__ import annotations
 
 import ctypes
+import threading
 from dataclasses import dataclass
 
 import torch
@@ -17,10 +18,16 @@ from . import _lib
 _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
 
 
+def _device_index(device: torch.device | None) -> int:
+    if device is None or device.index is None:
+        return torch.cuda.current_device()
+    return device.index
+
+
 def stream_of(tensor: torch.Tensor) -> int:
     """The raw ``hipStream_t`` torch currently issues work on for the tensor's device."""
     if _raw_stream is not None:
-        return _raw_stream(tensor.device.index if tensor.device.index is not None else torch.cuda.current_device())
+        return _raw_stream(_device_index(tensor.device))
     return torch.cuda.current_stream(tensor.device).cuda_stream
 
 
@@ -28,28 +35,52 @@ def _ptr(t: torch.Tensor | None) -> int | None:
     return None if t is None else t.data_ptr()
 
 
+#: The stream a graph capture of this thread was started from (``None`` outside a capture).
+_capture_origin = threading.local()
+
+
+def workspace_key(device: torch.device) -> tuple:
+    """The key of the scratch buffers (reduction partials, Poisson plans, cached solvers) of work
+    issued now on ``device``: the device and the caller's stream. Work on different streams, which may
+    run at once, never shares scratch memory; work on one stream is ordered by the stream. Inside a
+    capture (:func:`capture_graph`) the stream is the one the capture was started from, where the
+    graph is replayed."""
+    index = _device_index(device)
+    origin = getattr(_capture_origin, "stream", None)
+    if origin is None:
+        origin = _raw_stream(index) if _raw_stream is not None else torch.cuda.current_stream(index).cuda_stream
+    return (index, int(origin))
+
+
 _capture_streams: dict = {}
 
 
 def capture_graph(body) -> torch.cuda.CUDAGraph:
-    """``body``'s launches captured into a HIP graph (replayed later on the current stream).
+    """``body``'s launches captured into a HIP graph, to be replayed on the current stream.
 
-    Captures on a side stream of the current device without draining the device first (torch's
-    ``torch.cuda.graph`` synchronises the device and empties the allocator cache before every
+    Captures on a side stream (one per device and origin stream) without draining the device first
+    (torch's ``torch.cuda.graph`` synchronises the device and empties the allocator cache before every
     capture, which leaves the GPU idle while the host records), in thread-local mode, so work that
     host threads enqueue meanwhile (pinned draws) cannot invalidate it. The graph's allocations come
-    from its own private pool, as with ``torch.cuda.graph``."""
+    from its own private pool, as with ``torch.cuda.graph``. Scratch buffers the captured calls take
+    are those of the origin stream (:func:`workspace_key`)."""
     device = torch.cuda.current_device()
-    stream = _capture_streams.get(device)
+    key = workspace_key(torch.device("cuda", device))
+    stream = _capture_streams.get(key)
     if stream is None:
-        stream = _capture_streams[device] = torch.cuda.Stream(device)
+        stream = _capture_streams[key] = torch.cuda.Stream(device)
     graph = torch.cuda.CUDAGraph()
-    with torch.cuda.stream(stream):
-        graph.capture_begin(capture_error_mode="thread_local")
-        try:
-            body()
-        finally:
-            graph.capture_end()
+    outer = getattr(_capture_origin, "stream", None)
+    _capture_origin.stream = key[1]
+    try:
+        with torch.cuda.stream(stream):
+            graph.capture_begin(capture_error_mode="thread_local")
+            try:
+                body()
+            finally:
+                graph.capture_end()
+    finally:
+        _capture_origin.stream = outer
     return graph
 
 
@@ -198,7 +229,7 @@ def hdiv_launch_shape(shape, k: int, order: int = 8) -> dict:
     return dict(zip(("tile_y", "tile_z", "segment", "segments", "threads", "blocks"), list(buf)))
 
 
-_stencil_partials: dict[str, torch.Tensor] = {}
+_stencil_partials: dict[tuple, torch.Tensor] = {}
 
 
 def hdiv_resnorm(x: torch.Tensor, geom: StencilGeometry, evals: torch.Tensor, q: torch.Tensor,
@@ -217,7 +248,7 @@ def hdiv_resnorm(x: torch.Tensor, geom: StencilGeometry, evals: torch.Tensor, q:
     if evals.dtype != torch.float64 or evals.numel() < k or not evals.is_contiguous():
         raise ValueError("hdiv_resnorm: evals must hold k contiguous float64 values")
     blocks = hdiv_launch_shape(geom.shape, k, geom.order)["blocks"]
-    key = str(geom.device)
+    key = workspace_key(geom.device)
     ws = _stencil_partials.get(key)
     if ws is None or ws.numel() < blocks:
         if ws is not None:
@@ -281,11 +312,11 @@ _retired: list[torch.Tensor] = []
 
 
 def partials_for(n: int, device: torch.device) -> torch.Tensor:
-    """The block-partials workspace of the deterministic reductions for length ``n`` (one per
-    device, grown on demand through torch). A grown workspace's predecessor is kept alive:
-    captured graphs may still address it."""
+    """The block-partials workspace of the deterministic reductions for length ``n`` (one per device
+    and stream, :func:`workspace_key`; grown on demand through torch). A grown workspace's
+    predecessor is kept alive: captured graphs may still address it."""
     need = int(_lib.load().cdft_reduction_workspace_doubles(int(n)))
-    key = ("partials", str(device))
+    key = ("partials", *workspace_key(device))
     ws = _workspaces.get(key)
     if ws is None or ws.numel() < need:
         if ws is not None:

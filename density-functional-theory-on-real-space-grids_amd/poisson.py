@@ -16,7 +16,7 @@ import math
 import torch
 
 from . import _lib
-from .ops import stream_of
+from .ops import stream_of, workspace_key
 
 
 def fft_friendly(n: int) -> int:
@@ -60,9 +60,12 @@ class _Plan:
             self.handle = None
 
 
-#: Plans are built once per (shape, padded shape, spacing, device) and shared by every solver of that
-#: key for the life of the process, as FFT libraries cache plans. Solvers sharing a plan share its
-#: scratch buffers, so their solves must be ordered (one stream), as every solve of one SCF is.
+#: Plans are built once per (shape, padded shape, spacing, device, stream) and shared by the solvers
+#: of that key, as FFT libraries cache plans. Solvers sharing a plan share its scratch buffers; their
+#: solves are ordered by the one stream they run on (:func:`ops.workspace_key`). At most
+#: ``PLAN_CACHE_SIZE`` plans are cached, least recently used first out: a scan whose box changes with
+#: the geometry does not keep one per step. An evicted plan is destroyed once no solver holds it.
+PLAN_CACHE_SIZE = 4
 _plans: dict[tuple, _Plan] = {}
 
 
@@ -73,10 +76,13 @@ class CoulombCutoffPoisson:
         self.grid = grid
         self.padded_shape = tuple(fft_friendly(int(math.ceil(pad_factor * n))) for n in grid.shape)
         self.cutoff_radius = 0.5 * min(n * grid.spacing for n in self.padded_shape)
-        key = (tuple(grid.shape), self.padded_shape, float(grid.spacing), str(grid.device))
-        plan = _plans.get(key)
+        key = (tuple(grid.shape), self.padded_shape, float(grid.spacing), *workspace_key(grid.device))
+        plan = _plans.pop(key, None)
         if plan is None:
-            plan = _plans[key] = _Plan(grid.shape, self.padded_shape, grid.spacing, self.cutoff_radius, grid.device)
+            plan = _Plan(grid.shape, self.padded_shape, grid.spacing, self.cutoff_radius, grid.device)
+        _plans[key] = plan
+        while len(_plans) > PLAN_CACHE_SIZE:
+            _plans.pop(next(iter(_plans)))
         self._plan = plan
         self._handle = plan.handle
         self.padded_in = plan.padded_in

@@ -398,18 +398,18 @@ def initial_density(step: KohnShamStep) -> torch.Tensor:
 #: Float64 filter passes that finish a mixed-precision solve.
 REFINEMENT_PASSES = 2
 
-#: Subspace solvers kept between SCF solves (least recently used last out), by grid shape, device,
-#: eigensolver settings and filter precision: a later solve of the same kind reuses their buffers and
-#: captured graphs, and copies its own geometry into their static arrays.
+#: Subspace solvers kept between SCF solves (least recently used first out), by grid shape, device,
+#: stream, eigensolver settings and filter precision: a later solve of the same kind reuses their
+#: buffers and captured graphs, and copies its own geometry into their static arrays.
 SOLVER_CACHE_SIZE = 2
 _solvers: dict = {}
 
 
 def subspace_solver(grid, config: EigenConfig, filter_dtype, filter_hook) -> ChebyshevFilteredSubspace:
-    """The SCF's subspace solver: a cached one of the same kind, started afresh, or a new one. A cached
-    solver holds one solve's state at a time: solves that run at once (several host threads) must
-    not share the cache (set ``SOLVER_CACHE_SIZE = 0``)."""
-    key = (grid.shape, str(grid.device), repr(config), filter_dtype)
+    """The SCF's subspace solver: a cached one of the same kind, started afresh, or a new one. The
+    cache is keyed by the caller's stream (:func:`ops.workspace_key`), so solves that run at once on
+    different streams (one host thread each) never share a solver or its graphs' scratch buffers."""
+    key = (grid.shape, ops.workspace_key(grid.device), repr(config), filter_dtype)
     solver = _solvers.pop(key, None)
     if solver is None:
         solver = ChebyshevFilteredSubspace(grid, config, ritz_stop=False, filter_dtype=filter_dtype,
