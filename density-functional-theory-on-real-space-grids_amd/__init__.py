@@ -37,8 +37,8 @@ def geometry_from_arrays(shape, order, spacing, face_weights, inverse_weight, po
 
 def chebyshev_filter_device(block, geom, degree, lower, upper, spectrum_min):
     """Scaled Chebyshev filter with the coefficient table built on the host and uploaded."""
-    from .chefsi import chebyshev_coefficients
+    from .chefsi import scaled_chebyshev_table
 
-    rows = chebyshev_coefficients(degree, lower, upper, spectrum_min)
+    rows = scaled_chebyshev_table(degree, lower, upper, spectrum_min)
     table = torch.tensor(rows, dtype=torch.float64).to(geom.device)
     return ops.cheb_filter(block.to(geom.dtype).contiguous(), geom, table, degree)

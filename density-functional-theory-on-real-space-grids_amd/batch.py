@@ -29,7 +29,7 @@ import torch
 
 from . import _lib, ops
 from .eigen import BOUND_REUSE_RITZ_FRACTION, EigenConfig, RitzWork, _fraction  # noqa: F401
-from .chefsi import chebyshev_coefficients
+from .chefsi import scaled_chebyshev_table
 from .hamiltonian import lanczos_finish, lanczos_launch
 from .ops import stream_of
 
@@ -258,7 +258,7 @@ class BatchedCheFSI:
                 if it.done:
                     continue
                 lower = min(max(it.next_lower, it.smin + 1e-8), it.smax - 1e-8)
-                tables_host[i] = torch.as_tensor(chebyshev_coefficients(degree, lower, it.smax, it.smin))
+                tables_host[i] = torch.as_tensor(scaled_chebyshev_table(degree, lower, it.smax, it.smin))
             tables.copy_(tables_host, non_blocking=True)
             active_worst = max(it.worst for it in insts if not it.done)
             use32 = (self.mixed and not stalled32 and active_worst > self.switch_residual

@@ -7,10 +7,29 @@ lines operation for operation, in numpy float64, so the GPU path can be checked 
 from __future__ import annotations
 
 import math
+from concurrent.futures import ThreadPoolExecutor
 from fractions import Fraction
 from functools import lru_cache
 
 import numpy as np
+
+#: Optional host threads for the block operator (set by :func:`set_threads`): the rows of a block are
+#: independent, so each thread applies the divergence to a slice of rows; numpy releases the
+#: interpreter lock inside its array loops. Every row is computed by the same operations as without
+#: threads, so the results are bitwise the single-threaded ones.
+_POOL: ThreadPoolExecutor | None = None
+_POOL_THREADS = 1
+
+
+def set_threads(threads: int) -> None:
+    """Use ``threads`` host threads for block applies (1 = none)."""
+    global _POOL, _POOL_THREADS
+    if _POOL is not None:
+        _POOL.shutdown(wait=True)
+        _POOL = None
+    _POOL_THREADS = max(1, int(threads))
+    if _POOL_THREADS > 1:
+        _POOL = ThreadPoolExecutor(max_workers=_POOL_THREADS)
 
 # --------------------------------------------------------------------------------------------
 # Finite-difference weights (exact rationals)
@@ -363,6 +382,14 @@ class CuspHamiltonian:
         self.uses_divergence_form = True
 
     def kinetic_apply(self, phi):
+        phi = np.asarray(phi)
+        if _POOL is not None and phi.ndim == 2 and phi.shape[0] > 1:
+            parts = np.array_split(np.arange(phi.shape[0]), min(_POOL_THREADS, phi.shape[0]))
+            rows = _POOL.map(lambda idx: self._kinetic_rows(phi[idx[0]:idx[-1] + 1]), parts)
+            return np.concatenate(list(rows), axis=0)
+        return self._kinetic_rows(phi)
+
+    def _kinetic_rows(self, phi):
         box = self.grid.to_box(phi)
         div = weighted_divergence(box, self.face, self.grid.spacing, self.grid.fd_order)
         return (div.reshape(*np.shape(phi)[:-1], -1) * 0.5) * self.inverse_weight
