@@ -9,6 +9,8 @@
 //
 // Reductions never use floating-point atomics: every block writes its partial sums, and one block
 // sums the partials in block order, so a repeat is bitwise identical.
+#include <mutex>
+
 #include "common.h"
 #include "ritz_tail.h"
 
@@ -573,9 +575,11 @@ __global__ __launch_bounds__(64) void max_abs_minus_identity_kernel(const double
 
 using namespace cdft;
 
-extern "C" int cdft_wgram_f64(const double* a, int ka, const double* b, int kb, const double* q,
-                              long n, double scale, int post, double* partials, long partials_len,
-                              double* out, void* stream) {
+namespace cdft {
+
+// The weighted Gram product of cdft_wgram_f64 with its block partials in `partials`.
+static int wgram_impl(const double* a, int ka, const double* b, int kb, const double* q, long n, double scale,
+                      int post, double* partials, long partials_len, double* out, hipStream_t s) {
   if (!a || !b || !out || !partials) return fail(CDFT_E_ARG, "wgram: null pointer argument");
   if (ka < 1 || kb < 1 || n < 1) return fail(CDFT_E_ARG, "wgram: sizes must be positive");
   if (post != 0 && post != 1) return fail(CDFT_E_ARG, "wgram: post must be 0 or 1");
@@ -583,7 +587,6 @@ extern "C" int cdft_wgram_f64(const double* a, int ka, const double* b, int kb, 
   if (partials_len < (long)blocks * KMAX * KMAX)
     return fail(CDFT_E_CAPACITY, "wgram: partials workspace holds %ld doubles, needs %ld", partials_len,
                 (long)blocks * KMAX * KMAX);
-  hipStream_t s = (hipStream_t)stream;
   for (int i0 = 0; i0 < ka; i0 += KMAX) {
     for (int j0 = 0; j0 < kb; j0 += KMAX) {
       const int ti = ka - i0 < KMAX ? ka - i0 : KMAX;
@@ -613,6 +616,72 @@ extern "C" int cdft_wgram_f64(const double* a, int ka, const double* b, int kb, 
     }
   }
   return CDFT_OK;
+}
+
+// ---- the allocation-free Gram product ----
+//
+// Its block partials live in `scratch`, a static device array of the library: one copy per GPU,
+// loaded with the code object and held until the process ends (512 KiB). The host-side name of the
+// array is not a device address; its address on each device is looked up once, with that device
+// current, by hipGetSymbolAddress and kept per device. Every call on one GPU shares that GPU's array:
+// calls that overlap on one GPU (two streams, or two host threads) race on it.
+constexpr long SCRATCH_DOUBLES = (long)MAX_PARTIALS * KMAX * KMAX;
+constexpr int SCRATCH_DEVICES = 64;
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+static std::mutex g_scratch_mutex;
+static double* g_scratch_ptr[SCRATCH_DEVICES] = {};
+
+// The device address of `scratch` on the device `stream` belongs to (the current device for the null
+// stream).
+static int scratch_for_stream(hipStream_t stream, double** ptr) {
+  int device = 0;
+  hipError_t e = stream ? hipStreamGetDevice(stream, &device) : hipGetDevice(&device);
+  if (e != hipSuccess) return fail((int)e, "wgram_noalloc: cannot find the stream's device: %s", hipGetErrorString(e));
+  if (device < 0 || device >= SCRATCH_DEVICES)
+    return fail(CDFT_E_ARG, "wgram_noalloc: device %d is beyond the %d devices the library keeps scratch for", device,
+                SCRATCH_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    int current = 0;
+    e = hipGetDevice(&current);
+    if (e != hipSuccess) return fail((int)e, "wgram_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    if (current != device && (e = hipSetDevice(device)) != hipSuccess)
+      return fail((int)e, "wgram_noalloc: hipSetDevice(%d): %s", device, hipGetErrorString(e));
+    void* address = nullptr;
+    const hipError_t found = hipGetSymbolAddress(&address, HIP_SYMBOL(scratch));
+    if (current != device && (e = hipSetDevice(current)) != hipSuccess)
+      return fail((int)e, "wgram_noalloc: hipSetDevice(%d): %s", current, hipGetErrorString(e));
+    if (found != hipSuccess || !address)
+      return fail(found != hipSuccess ? (int)found : CDFT_E_LAUNCH, "wgram_noalloc: hipGetSymbolAddress(scratch): %s",
+                  hipGetErrorString(found));
+    g_scratch_ptr[device] = (double*)address;
+  }
+  *ptr = g_scratch_ptr[device];
+  return CDFT_OK;
+}
+
+}  // namespace cdft
+
+extern "C" int cdft_wgram_f64(const double* a, int ka, const double* b, int kb, const double* q,
+                              long n, double scale, int post, double* partials, long partials_len,
+                              double* out, void* stream) {
+  return wgram_impl(a, ka, b, kb, q, n, scale, post, partials, partials_len, out, (hipStream_t)stream);
+}
+
+extern "C" int cdft_wgram_noalloc_f64(const double* a, int ka, const double* b, int kb, const double* q, long n,
+                                      double scale, int post, double* out, void* stream) {
+  if (!a || !b || !out) return fail(CDFT_E_ARG, "wgram_noalloc: null pointer argument");
+  if (ka < 1 || kb < 1 || n < 1) return fail(CDFT_E_ARG, "wgram_noalloc: sizes must be positive");
+  // The reduction takes at most MAX_PARTIALS blocks (its grid-stride loop covers any n), each with one
+  // KMAX x KMAX tile of partial sums; a request beyond the static scratch is refused, not truncated.
+  const long need = (long)reduction_blocks(n) * KMAX * KMAX;
+  if (need > SCRATCH_DOUBLES)
+    return fail(CDFT_E_CAPACITY, "wgram_noalloc: %ld partial sums exceed the static scratch of %ld doubles", need,
+                SCRATCH_DOUBLES);
+  double* partials = nullptr;
+  const int rc = scratch_for_stream((hipStream_t)stream, &partials);
+  if (rc) return rc;
+  return wgram_impl(a, ka, b, kb, q, n, scale, post, partials, SCRATCH_DOUBLES, out, (hipStream_t)stream);
 }
 
 // y = R^T x and gram = sum_p q_p y_ip y_jp (k <= 8; q may be NULL): bitwise cdft_rotate_f64 (transpose)

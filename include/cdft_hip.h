@@ -154,6 +154,16 @@ int cdft_wgram_f64(const double* a, int ka, const double* b, int kb, const doubl
                    double scale, int post, double* partials, long partials_len, double* out,
                    void* stream);
 
+/* cdft_wgram_f64 without a workspace argument and without any allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The block partials live in a static device array of
+ * the library (512 KiB per GPU, held until the process ends; at most 1024 blocks x 8 x 8 partial sums,
+ * which covers every n, since the reduction's grid-stride loop caps its blocks at 1024); a request that
+ * needed more would return CDFT_E_CAPACITY. Results are bitwise those of cdft_wgram_f64. Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (two streams, two host threads) race. */
+int cdft_wgram_noalloc_f64(const double* a, int ka, const double* b, int kb, const double* q, long n,
+                           double scale, int post, double* out, void* stream);
+
 /* y = R^T x (k x k, k <= 8) fused with gram = sum_p q_p y_ip y_jp (q may be NULL), bitwise cdft_rotate_f64
  * (transpose = 1) followed by cdft_wgram_f64 of y with itself: the rotation of the Ritz vectors and
  * their orthonormality Gram matrix (src/cdft/eigen/rayleigh_ritz.py:95-102). */
