@@ -214,6 +214,79 @@ class KohnShamProblem:
         return grad_v
 
 
+class ChebyshevFilterFunction(torch.autograd.Function):
+    """``Y = p_d(A(V)) X``: the scaled Chebyshev filter of degree ``d`` (the fused K2 steps, bitwise
+    :func:`ops.cheb_filter`), differentiable in the block ``X`` and the diagonal potential ``V`` by the
+    adjoint recurrence, with checkpointing of the filter iterates.
+
+    With rows ``(c_j, a_j, b_j)`` the forward runs ``y_1 = a_0 (A - c_0) y_0`` and
+    ``y_{j+1} = a_j (A - c_j) y_j - b_j y_{j-1}`` from ``y_0 = X``. For ``t = a_j ybar_{j+1}`` the
+    backward adds ``(A^T - c_j) t`` to ``ybar_j`` (the hand-written transpose apply),
+    ``-b_j ybar_{j+1}`` to ``ybar_{j-1}`` and ``sum_k t_k y_{j,k}`` to ``Vbar``, for ``j = d-1 .. 0``; it
+    needs the iterates in reverse order. The forward keeps only every ``every``-th pair of consecutive
+    iterates ``(y_{j-1}, y_j)``; the backward recomputes each segment's ``every`` iterates from its
+    pair before walking back through it, so at most ``2 ceil(d / every) + every`` blocks are held
+    instead of ``d + 1``. Call as ``ChebyshevFilterFunction.apply(x, v, geometry, rows, every)`` with
+    ``rows`` the host rows of :func:`chefsi.scaled_chebyshev_table`; ``geometry``'s own potential is
+    replaced by ``v``."""
+
+    @staticmethod
+    def forward(ctx, x, v, geometry: StencilGeometry, rows, every: int = 4):
+        geom = geometry.with_potential(v.detach().contiguous())
+        table = torch.tensor(rows, dtype=torch.float64, device=x.device)
+        degree = len(rows)
+        every = max(1, int(every))
+        y_prev, y = None, x.detach().contiguous()
+        checkpoints = {}
+        for j in range(degree):
+            if j % every == 0:
+                checkpoints[j] = (y_prev, y)
+            out = torch.empty_like(y)
+            ops.cheb_step(y, y_prev, geom, table, j, out)
+            y_prev, y = y, out
+        ctx.geometry, ctx.table, ctx.rows, ctx.every = geom, table, [tuple(map(float, r)) for r in rows], every
+        ctx.checkpoints = checkpoints
+        ctx.held = 2 * len(checkpoints) + every  # blocks the backward holds at most (for the record)
+        return y
+
+    @staticmethod
+    def backward(ctx, grad):
+        geom, table, rows, every = ctx.geometry, ctx.table, ctx.rows, ctx.every
+        degree = len(rows)
+        k, n = grad.shape
+        ybar_next = grad.detach().contiguous().clone()  # ybar_{j+1}
+        ybar_cur = torch.zeros_like(ybar_next)          # ybar_j (accumulated)
+        vbar = torch.zeros(n, dtype=torch.float64, device=grad.device)
+        t = torch.empty_like(ybar_next)
+        tmp = torch.empty(n, dtype=torch.float64, device=grad.device)
+        for start in sorted(ctx.checkpoints, reverse=True):
+            stop = min(start + every, degree)
+            # recompute y_start .. y_{stop-1} from the checkpoint pair
+            y_prev, y = ctx.checkpoints[start]
+            iterates = {start: y, start - 1: y_prev}
+            for j in range(start, stop - 1):
+                out = torch.empty_like(y)
+                ops.cheb_step(y, y_prev, geom, table, j, out)
+                y_prev, y = y, out
+                iterates[j + 1] = y
+            for j in range(stop - 1, start - 1, -1):
+                c, a, b = rows[j]
+                ops.lincomb(t.view(-1), [(ybar_next.view(-1), a)])
+                at = ops.hdiv_apply_adjoint(t, geom)
+                ops.lincomb(ybar_cur.view(-1), [(ybar_cur.view(-1), 1.0), (at.view(-1), 1.0), (t.view(-1), -c)])
+                yj = iterates[j]
+                for r in range(k):
+                    ops.ewise(ops.EW_MUL, tmp, t[r], yj[r])
+                    ops.ewise(ops.EW_ADD, vbar, vbar, tmp)
+                # shift: ybar_{j} becomes the next "next"; ybar_{j-1} starts as -b_j ybar_{j+1}
+                if j >= 1:
+                    ops.lincomb(t.view(-1), [(ybar_next.view(-1), -b)])
+                    ybar_next, ybar_cur, t = ybar_cur, t, ybar_next
+                else:
+                    ybar_next = ybar_cur
+        return ybar_next, vbar, None, None, None
+
+
 class KohnShamDensity(torch.autograd.Function):
     """``rho(V)`` with the Sternheimer backward: ``KohnShamDensity.apply(v, problem)``."""
 

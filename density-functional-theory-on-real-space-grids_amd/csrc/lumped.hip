@@ -32,7 +32,8 @@ __global__ void far_fine_kernel(FarSpheres sp, int f0, int f1, int f2, const dou
   double far = 1.0, u = 0.0;
   for (int a = 0; a < sp.n; ++a) {
     const double dx = x - sp.cx[a], dy = y - sp.cy[a], dz = z - sp.cz[a];
-    const double s = sqrt((dx * dx + dy * dy) + dz * dz);
+    // separately rounded, as the reference's torch expression (s < radius is a threshold test)
+    const double s = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)));
     const double bump = s < sp.radius[a] ? 0.5 * erfc((s - sp.centre[a]) / sp.width[a]) : 0.0;
     far = far * (1.0 - bump);
     u = u + sp.z[a] * s;

@@ -44,8 +44,16 @@ struct Spheres {
   double cell_width;  // Becke erf width, already times sqrt(2)
 };
 
+// Distances and lattice coordinates are formed with separately rounded products and sums (no fused
+// multiply-add), as the reference's torch CPU expressions round them: a point two spacings from a
+// nucleus sits on the cell-average threshold, and a coordinate one ulp off moves it across.
 __device__ __forceinline__ double dist3(double dx, double dy, double dz) {
-  return sqrt((dx * dx + dy * dy) + dz * dz);
+  return sqrt(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)));
+}
+
+// origin + index * h, the reference's grid point coordinate (grid.py:667-670, cusp.py:190-196).
+__device__ __forceinline__ double lattice(double origin, double index, double h) {
+  return __dadd_rn(origin, __dmul_rn(index, h));
 }
 
 __global__ void cusp_grid_fields_kernel(GridDesc g, Nuclei nu, double reach, CellRule rule,
@@ -57,7 +65,7 @@ __global__ void cusp_grid_fields_kernel(GridDesc g, Nuclei nu, double reach, Cel
   const int i2 = (int)(p % g.n2);
   const int i1 = (int)((p / g.n2) % g.n1);
   const int i0 = (int)(p / ((long)g.n1 * g.n2));
-  const double x = g.ox + (double)i0 * g.h, y = g.oy + (double)i1 * g.h, z = g.oz + (double)i2 * g.h;
+  const double x = lattice(g.ox, (double)i0, g.h), y = lattice(g.oy, (double)i1, g.h), z = lattice(g.oz, (double)i2, g.h);
   double u = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
   bool near = false;
   for (int a = 0; a < nu.n; ++a) {
@@ -124,11 +132,11 @@ __global__ void face_weights_kernel(GridDesc g, Nuclei nu, int axis, int order,
   if (axis == 0) c0 = c0 - half + 0.5;
   if (axis == 1) c1 = c1 - half + 0.5;
   if (axis == 2) c2 = c2 - half + 0.5;
-  const double x = g.ox + c0 * g.h, y = g.oy + c1 * g.h, z = g.oz + c2 * g.h;
+  const double x = lattice(g.ox, c0, g.h), y = lattice(g.oy, c1, g.h), z = lattice(g.oz, c2, g.h);
   double u = 0.0;
   for (int a = 0; a < nu.n; ++a) {
     const double dx = x - nu.px[a], dy = y - nu.py[a], dz = z - nu.pz[a];
-    u = u + nu.z[a] * sqrt((dx * dx + dy * dy) + dz * dz);
+    u = u + nu.z[a] * dist3(dx, dy, dz);
   }
   out[p] = exp(-2.0 * u);
   atomicMax(umax, (unsigned long long)__double_as_longlong(u));  // u >= 0: integer order = float order
@@ -154,7 +162,7 @@ __global__ void far_factor_kernel(GridDesc g, Spheres s, double* __restrict__ fa
   const int i2 = (int)(p % g.n2);
   const int i1 = (int)((p / g.n2) % g.n1);
   const int i0 = (int)(p / ((long)g.n1 * g.n2));
-  far[p] = far_factor_at(s, g.ox + (double)i0 * g.h, g.oy + (double)i1 * g.h, g.oz + (double)i2 * g.h);
+  far[p] = far_factor_at(s, lattice(g.ox, (double)i0, g.h), lattice(g.oy, (double)i1, g.h), lattice(g.oz, (double)i2, g.h));
 }
 
 // Partition P_a = Q (beta_a + sum_b beta_b [s_b >= R_b]) and the analytic fields at each node.
@@ -491,9 +499,9 @@ __global__ void hydrogenic_fields_kernel(const double* __restrict__ pts, long m,
     const int i2 = (int)(p % g.n2);
     const int i1 = (int)((p / g.n2) % g.n1);
     const int i0 = (int)(p / ((long)g.n1 * g.n2));
-    x = g.ox + (double)i0 * g.h;
-    y = g.oy + (double)i1 * g.h;
-    z = g.oz + (double)i2 * g.h;
+    x = lattice(g.ox, (double)i0, g.h);
+    y = lattice(g.oy, (double)i1, g.h);
+    z = lattice(g.oz, (double)i2, g.h);
   }
   for (int a = 0; a < nu.n; ++a) {
     const double s = dist3(x - nu.px[a], y - nu.py[a], z - nu.pz[a]);

@@ -13,10 +13,10 @@ from functools import lru_cache
 
 import numpy as np
 
-#: Optional host threads for the block operator (set by :func:`set_threads`): the rows of a block are
-#: independent, so each thread applies the divergence to a slice of rows; numpy releases the
-#: interpreter lock inside its array loops. Every row is computed by the same operations as without
-#: threads, so the results are bitwise the single-threaded ones.
+#: Optional host threads for the divergence (set by :func:`set_threads`): each axis's term is computed
+#: in slices along another spatial axis on the threads (numpy releases the interpreter lock inside its
+#: array loops). Every element gets the same operations as without threads, so the results are
+#: bitwise the single-threaded ones.
 _POOL: ThreadPoolExecutor | None = None
 _POOL_THREADS = 1
 
@@ -291,39 +291,78 @@ def _slab(array: np.ndarray, axis: int, start: int, length: int) -> np.ndarray:
     return array[tuple(index)]
 
 
+def _axis_term(box: np.ndarray, weight: np.ndarray, ax: int, coeffs, scale: float, accuracy: int) -> np.ndarray:
+    """``D_ax^T [w D_ax box]`` along array axis ``ax`` (``weight`` extended by ``accuracy`` along it)."""
+    half = accuracy // 2
+    extent = box.shape[ax]
+    shape = list(box.shape)
+    shape[ax] = extent + 2 * accuracy
+    padded = np.zeros(shape)
+    _slab(padded, ax, accuracy, extent)[...] = box
+    length = extent + accuracy
+    fshape = list(box.shape)
+    fshape[ax] = length
+    faces = np.empty(fshape)
+    tmp = np.empty(fshape)
+    for index, c in enumerate(coeffs):
+        k = index - half
+        term = _slab(padded, ax, accuracy - half + 1 + k, length)
+        if index == 0:
+            np.multiply(term, c * scale, out=faces)
+        else:
+            np.multiply(term, c * scale, out=tmp)
+            np.add(faces, tmp, out=faces)
+    np.multiply(faces, weight, out=faces)
+    term_sum = np.empty(box.shape)
+    part = np.empty(box.shape)
+    for index, c in enumerate(coeffs):
+        k = index - half
+        term = _slab(faces, ax, half - 1 - k, extent)
+        if index == 0:
+            np.multiply(term, c * scale, out=term_sum)
+        else:
+            np.multiply(term, c * scale, out=part)
+            np.add(term_sum, part, out=term_sum)
+    return term_sum
+
+
 def weighted_divergence(box: np.ndarray, face_weights, spacing: float, accuracy: int) -> np.ndarray:
     """``sum_d D_d^T [w_d D_d phi]`` with a zero halo, box-shaped ``(..., n0, n1, n2)``.
 
     Tap and term order of ``_weighted_divergence_eager`` (reference ``operators/divergence.py:
     84-150,172-197``): per axis pad by ``accuracy``, face ``n`` (from ``-p``) reads node
-    ``n + 1 + k``; interior ``m`` reads face ``m - 1 - k``.
+    ``n + 1 + k``; interior ``m`` reads face ``m - 1 - k``. Each sum is accumulated term by term in
+    tap order (``acc = acc + term * (c / h)``, every product and sum rounded once), and the axes in
+    order 0, 1, 2. With host threads (:func:`set_threads`) each axis's term is computed in slices
+    along another spatial axis, which the axis's stencil does not couple: every element still gets
+    the same operations in the same order.
     """
     coeffs = staggered_coefficients(accuracy)
-    half = accuracy // 2
     scale = 1.0 / spacing
+    box = np.asarray(box, dtype=np.float64)
     nd = box.ndim
     out = None
     for axis in range(3):
         ax = nd - 3 + axis
-        extent = box.shape[ax]
-        pad = [(0, 0)] * nd
-        pad[ax] = (accuracy, accuracy)
-        padded = np.pad(box, pad)
-        length = extent + accuracy
-        faces = None
-        for index, c in enumerate(coeffs):
-            k = index - half
-            start = accuracy - half + 1 + k
-            term = _slab(padded, ax, start, length)
-            faces = term * (c * scale) if faces is None else faces + term * (c * scale)
-        faces = faces * face_weights[axis]
-        term_sum = None
-        for index, c in enumerate(coeffs):
-            k = index - half
-            start = half - 1 - k
-            term = _slab(faces, ax, start, extent)
-            term_sum = term * (c * scale) if term_sum is None else term_sum + term * (c * scale)
-        out = term_sum if out is None else out + term_sum
+        if _POOL is None:
+            term = _axis_term(box, face_weights[axis], ax, coeffs, scale, accuracy)
+        else:
+            split = nd - 3 + (1 if axis == 0 else 0)  # a spatial axis this stencil does not run along
+            wsplit = split - (nd - 3)
+            bounds = np.linspace(0, box.shape[split], min(_POOL_THREADS, box.shape[split]) + 1).astype(int)
+            term = np.empty(box.shape)
+
+            def part(i, ax=ax, axis=axis, split=split, wsplit=wsplit, term=term):
+                lo, hi = bounds[i], bounds[i + 1]
+                _slab(term, split, lo, hi - lo)[...] = _axis_term(
+                    _slab(box, split, lo, hi - lo), _slab(face_weights[axis], wsplit, lo, hi - lo), ax, coeffs, scale,
+                    accuracy)
+
+            list(_POOL.map(part, range(len(bounds) - 1)))
+        if out is None:
+            out = term
+        else:
+            np.add(out, term, out=out)
     return out
 
 
@@ -382,14 +421,6 @@ class CuspHamiltonian:
         self.uses_divergence_form = True
 
     def kinetic_apply(self, phi):
-        phi = np.asarray(phi)
-        if _POOL is not None and phi.ndim == 2 and phi.shape[0] > 1:
-            parts = np.array_split(np.arange(phi.shape[0]), min(_POOL_THREADS, phi.shape[0]))
-            rows = _POOL.map(lambda idx: self._kinetic_rows(phi[idx[0]:idx[-1] + 1]), parts)
-            return np.concatenate(list(rows), axis=0)
-        return self._kinetic_rows(phi)
-
-    def _kinetic_rows(self, phi):
         box = self.grid.to_box(phi)
         div = weighted_divergence(box, self.face, self.grid.spacing, self.grid.fd_order)
         return (div.reshape(*np.shape(phi)[:-1], -1) * 0.5) * self.inverse_weight

@@ -170,3 +170,99 @@ def test_sternheimer_gradient_pointwise_finite_differences(pkg, device):
         worst = max(worst, abs(fd - grad[p]))
     print(f"pointwise FD: worst |fd - grad| = {worst:.3e}, max |grad| = {scale:.3e}", flush=True)
     assert worst <= 1e-6 * scale
+
+
+def test_sternheimer_gradient_on_converged_lda_potential(pkg, device):
+    """The inversion step's own setting at reduced size: He LDA on the 16-bohr box at h = 0.5 (33^3),
+    the quadrature measure (omega / h^3) and the converged LDA effective potential as V_0, the loss
+    in the cusp-aware measure (omega f^2). Central differences along three smooth seeded directions
+    agree with the Sternheimer gradient, relative to ``sum |dL/dV_p d_p|`` (the scale of the directional
+    derivative before its cancellations): at the configuration's step 1e-6 to 1e-5 (the forward solves
+    converge to residual 1e-11, so the loss carries about 1e-13 of noise, which the difference quotient
+    divides by 2e-6) and at step 1e-4 to 1e-6. The conjugate-gradient solves converge (below their
+    1e-11 tolerance, well before the cap)."""
+    from cdft_amd import scf
+    from cdft_amd.adjoint import KohnShamDensity, KohnShamProblem, density_loss
+    from cdft_amd.contract import Numerics, helium_lda
+
+    r = scf.solve_self_consistent(helium_lda(), Numerics(box_lengths=(16.0,) * 3, spacing=0.5), device)
+    step_ = r["step"]
+    ham = step_.hamiltonian({"effective": r["effective_potential"]})
+    grid = step_.grid
+    n0, n1, n2 = grid.shape
+    axes = [np.arange(n) * grid.spacing + o for n, o in zip(grid.shape, grid.origin)]
+    pts = np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1).reshape(-1, 3)
+    weights = step_.q.mass_weights * step_.f2
+    problem = KohnShamProblem(grid, ham.geometry, ham.measure.q, [2.0])
+    rho_t, _ = problem.forward(torch.as_tensor(_bumps(pts, seed=1)).to(device))
+    v = torch.zeros(grid.n_points, dtype=torch.float64, device=device, requires_grad=True)
+    loss = density_loss(KohnShamDensity.apply(v, problem), rho_t, weights)
+    loss.backward()
+    grad = v.grad.detach()
+    cg = problem.history[-1]["sternheimer"][0]
+    assert cg["converged"] and cg["iterations"] < problem.cg.max_iterations, cg
+    for step, tol in ((1e-6, 1e-5), (1e-4, 1e-6)):
+        for seed in (11, 12, 13):
+            d = torch.as_tensor(_bumps(pts, seed=seed, amplitude=1.0)).to(device)
+            lp = float(density_loss(problem.forward(step * d)[0], rho_t, weights))
+            lm = float(density_loss(problem.forward(-step * d)[0], rho_t, weights))
+            fd = (lp - lm) / (2 * step)
+            an = float((grad * d).sum())
+            scale = float((grad * d).abs().sum())
+            print(f"step {step:g} seed {seed}: fd {fd:.12e} analytic {an:.12e} error/scale {abs(fd - an) / scale:.2e}",
+                  flush=True)
+            assert abs(fd - an) <= tol * scale, (step, seed, fd, an, scale)
+
+
+def test_checkpointed_filter_gradient(pkg, device):
+    """The differentiable Chebyshev filter (degree 20) with checkpointing of its iterates: the forward is
+    bitwise the fused filter; the gradients for the block and the potential with a checkpoint every 4
+    steps are bitwise those with every iterate stored (the recomputed iterates are the forward's), agree
+    with torch autograd through a chain of HApply calls to 1e-10 of their max-norm, and with central
+    differences (step 1e-6) along a seeded potential direction to 1e-7 relative."""
+    from cdft_amd.adjoint import ChebyshevFilterFunction, HApply
+    from cdft_amd.chefsi import scaled_chebyshev_table
+
+    grid_o, ham_o, grid, geom, q = _small_case(pkg, device, n=13)
+    n = grid.n_points
+    rows = scaled_chebyshev_table(20, 0.5, 40.0, -2.5)
+    gen = torch.Generator().manual_seed(21)
+    x0 = (torch.rand((3, n), generator=gen, dtype=torch.float64) * 2 - 1).to(device)
+    v0 = (0.1 * torch.rand(n, generator=gen, dtype=torch.float64)).to(device) + geom.potential
+    weight = (torch.rand((3, n), generator=gen, dtype=torch.float64) * 2 - 1).to(device)
+
+    def run(every):
+        x = x0.clone().requires_grad_(True)
+        v = v0.clone().requires_grad_(True)
+        y = ChebyshevFilterFunction.apply(x, v, geom, rows, every)
+        (weight * y).sum().backward()
+        return y.detach(), x.grad, v.grad
+
+    y4, gx4, gv4 = run(4)
+    y1, gx1, gv1 = run(1)
+    table = torch.tensor(rows, dtype=torch.float64, device=device)
+    assert torch.equal(y4, pkg.ops.cheb_filter(x0, geom.with_potential(v0), table, 20))
+    assert torch.equal(gx4, gx1) and torch.equal(gv4, gv1)
+
+    # autograd through the recurrence written with HApply (every iterate kept by autograd)
+    x = x0.clone().requires_grad_(True)
+    v = v0.clone().requires_grad_(True)
+    prev, y = None, x
+    for j, (c, a, b) in enumerate(rows):
+        nxt = (HApply.apply(y, v, geom) - c * y) * a
+        if j > 0:
+            nxt = nxt - b * prev
+        prev, y = y, nxt
+    (weight * y).sum().backward()
+    assert (gx4 - x.grad).abs().max() <= 1e-10 * x.grad.abs().max()
+    assert (gv4 - v.grad).abs().max() <= 1e-10 * v.grad.abs().max()
+
+    d = torch.as_tensor(_bumps(grid_o.points(), seed=5, amplitude=1.0)).to(device)
+    step = 1e-6
+
+    def loss(vv):
+        return float((weight * pkg.ops.cheb_filter(x0, geom.with_potential(vv.contiguous()), table, 20)).sum())
+
+    fd = (loss(v0 + step * d) - loss(v0 - step * d)) / (2 * step)
+    an = float((gv4 * d).sum())
+    assert abs(fd - an) <= 1e-7 * abs(an), (fd, an)
