@@ -68,6 +68,10 @@ SIGNATURES: dict[str, list] = {
     "cdft_mt19937_states": [ctypes.c_uint64, I, P, P, P],
     "cdft_hydrogenic_fields_f64": [P, L, P, D, P, I, P, P, P, P, P],
     "cdft_lda_vwn_f64": [P, L, P, P, P],
+    "cdft_pbe_f64": [P, P, L, P, P, P, P],
+    "cdft_gga_points_f64": [P, P, P, P, P, P, L, P, P, P, P, P],
+    "cdft_divergence_f64": [P, P, P, I, P, P],
+    "cdft_gga_combine_f64": [P, P, P, P, D, L, P, P],
     "cdft_ewise_f64": [I, P, P, P, D, P, L, P],
     "cdft_density_f64": [P, I, L, P, P, P, P, P],
     "cdft_atom_combine_f64": [P, P, P, I, L, D, P, P],

@@ -84,6 +84,11 @@ class UniformGrid:
     def zeros(self, *lead: int) -> torch.Tensor:
         return torch.zeros((*lead, self.n_points), dtype=torch.float64, device=self.device)
 
+    @property
+    def gradient_taps(self) -> torch.Tensor:
+        """The central first-derivative taps divided by h, on the device."""
+        return self._grad_taps
+
     def gradient(self, field: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """Order-``gradient_order`` central gradient, ``(nf, 3, N)`` for ``(nf, N)`` input."""
         f = field.reshape(-1, self.n_points)

@@ -489,6 +489,52 @@ def ewise(op: int, out: torch.Tensor, a: torch.Tensor | None = None, b: torch.Te
     return out
 
 
+def pbe(n: torch.Tensor, sigma: torch.Tensor, e: torch.Tensor | None = None, vn: torch.Tensor | None = None,
+        vs: torch.Tensor | None = None):
+    """Spin-unpolarised PBE at (n, sigma): ``(e, de/dn, de/dsigma)``."""
+    e = torch.empty_like(n) if e is None else e
+    vn = torch.empty_like(n) if vn is None else vn
+    vs = torch.empty_like(n) if vs is None else vs
+    for t, name in ((sigma, "sigma"), (e, "e"), (vn, "vn"), (vs, "vs")):
+        require(t, tuple(n.shape), torch.float64, n.device, name)
+    _lib.call("cdft_pbe_f64", n.data_ptr(), sigma.data_ptr(), n.numel(), e.data_ptr(), vn.data_ptr(), vs.data_ptr(),
+              stream_of(n))
+    return e, vn, vs
+
+
+def gga_points(rho: torch.Tensor, grad_rho: torch.Tensor, f2: torch.Tensor, grad_u: torch.Tensor,
+               w_s: torch.Tensor, w_f: torch.Tensor, e: torch.Tensor, aw: torch.Tensor, crossw: torch.Tensor,
+               fluxw: torch.Tensor) -> None:
+    """The pointwise sigma path (``cdft_gga_points_f64``) at ``len(rho)`` points."""
+    n = rho.numel()
+    for t, shape, name in ((rho, (n,), "rho"), (grad_rho, (3, n), "grad_rho"), (f2, (n,), "f2"),
+                           (grad_u, (3, n), "grad_u"), (w_s, (n,), "w_s"), (w_f, (n,), "w_f"), (e, (n,), "e"),
+                           (aw, (n,), "aw"), (crossw, (n,), "crossw"), (fluxw, (3, n), "fluxw")):
+        require(t, shape, torch.float64, rho.device, name)
+    _lib.call("cdft_gga_points_f64", rho.data_ptr(), grad_rho.data_ptr(), f2.data_ptr(), grad_u.data_ptr(),
+              w_s.data_ptr(), w_f.data_ptr(), n, e.data_ptr(), aw.data_ptr(), crossw.data_ptr(), fluxw.data_ptr(),
+              stream_of(rho))
+
+
+def divergence(field: torch.Tensor, grid, out: torch.Tensor) -> torch.Tensor:
+    """``d0 F0 + d1 F1 + d2 F2`` of a ``(3, N)`` field with the grid's gradient taps."""
+    require(field, (3, grid.n_points), torch.float64, field.device, "field")
+    require(out, (grid.n_points,), torch.float64, field.device, "out")
+    _lib.call("cdft_divergence_f64", field.data_ptr(), grid.c_shape, grid.gradient_taps.data_ptr(),
+              grid.gradient_order, out.data_ptr(), stream_of(field))
+    return out
+
+
+def gga_combine(a: torch.Tensor, near_div: torch.Tensor, far_div: torch.Tensor, cross: torch.Tensor, h3: float,
+                out: torch.Tensor) -> torch.Tensor:
+    n = out.numel()
+    for t, name in ((a, "a"), (near_div, "near_div"), (far_div, "far_div"), (cross, "cross")):
+        require(t, (n,), torch.float64, out.device, name)
+    _lib.call("cdft_gga_combine_f64", a.data_ptr(), near_div.data_ptr(), far_div.data_ptr(), cross.data_ptr(),
+              float(h3), n, out.data_ptr(), stream_of(out))
+    return out
+
+
 def lda_vwn(n: torch.Tensor, e: torch.Tensor | None = None, v: torch.Tensor | None = None):
     e = torch.empty_like(n) if e is None else e
     v = torch.empty_like(n) if v is None else v

@@ -60,6 +60,10 @@ class KohnShamStep:
     def __init__(self, scenario, numerics, grid: UniformGrid, factor: CuspFactor, quadrature: CuspQuadrature):
         self.scenario = scenario
         self.numerics = numerics
+        xc = getattr(scenario, "xc", "lda")
+        if xc not in ("lda", "pbe", "none"):
+            raise ValueError(f"unsupported functional {xc!r} (lda or pbe)")
+        self.gga = xc == "pbe"
         self.grid = grid
         self.factor = factor
         self.q = quadrature
@@ -119,6 +123,20 @@ class KohnShamStep:
         self.pot_lumped = torch.empty((2, g.n_points), **f)
         self.pot_effective = torch.empty(g.n_points, **f)
         self.pot_scalars = torch.empty(3, **f)
+        if self.gga:
+            # the sigma path: grad rho on the grid and at the nodes, the pointwise terms at both, the
+            # lifted and the grid flux (see _gga_xc)
+            self.grad_rho = torch.empty((3, g.n_points), **f)
+            self.node_grad = torch.empty((3, m), **f)
+            self.node_rows = torch.empty((3, m), **f)   # W f2 v_H | W f2 v_n | W cross
+            self.grid_rows = torch.empty((3, g.n_points), **f)  # far_w f2 v_H | far_w f2 v_n | far_w cross
+            self.node_flux = torch.empty((3, m), **f)   # W flux
+            self.grid_flux = torch.empty((3, g.n_points), **f)  # far flux
+            self.lumped3 = torch.empty((3, g.n_points), **f)
+            self.lifted_flux = torch.empty((3, g.n_points), **f)
+            self.near_div = torch.empty(g.n_points, **f)
+            self.far_div = torch.empty(g.n_points, **f)
+            self.xc_weights = torch.empty(g.n_points, **f)
         self.graphs: dict = {}
         self.graph_record = {"captures": 0, "replays": 0, "disabled": None}
         self.use_graphs = True
@@ -171,6 +189,9 @@ class KohnShamStep:
         ops.atom_combine(self.v_smooth, self.coefficients, self.v1s_grid, +1.0, self.v_h_grid)
         q.interpolate(self.interp_in, out=self.interp)
         ops.atom_combine(self.interp[0], self.coefficients, self.v1s_node, +1.0, self.v_h_node)
+        if self.gga:
+            self._gga_xc()
+            return
         ops.ewise(ops.EW_MUL, self.n_node, q.f2, self.interp[1])
         ops.lda_vwn(n_grid, self.e_grid, self.v_grid)
         ops.lda_vwn(self.n_node, self.e_node, self.v_node)
@@ -186,6 +207,35 @@ class KohnShamStep:
                          (q.far_weights, self.e_grid, None, 1, 1.0, True),
                          (xc_weights, self.rho, None, 2, 1.0, False)], scalars)
         ops.ewise(ops.EW_SUMDIV, self.pot_effective, hartree_weights, xc_weights, q.mass_weights)
+
+    def _gga_xc(self) -> None:
+        """The PBE part of the potentials (reference ``scf/step.py:253-279``): sigma from the order-8 grid
+        gradient of rho (interpolated to the nodes), the pointwise terms, then the XC lumped weights as
+        the scalar part ``lumped(f2 v_n)`` plus the gradient part ``-div_FD(L^T (W flux)) - h^3
+        div_FD(far flux)`` (``quadrature.py:640-650,718-729``) plus the cross part ``lumped(-2 flux .
+        grad u)``, and the Hartree lumped weights in the same lift."""
+        q, g, fac = self.q, self.grid, self.factor
+        g.gradient(self.rho, out=self.grad_rho.view(1, 3, -1))
+        q.interpolate(self.grad_rho, out=self.node_grad)
+        # Hartree rows (f2 v_H) W at the nodes and (f2 v_H) far_w on the grid
+        ops.ewise(ops.EW_MUL3, self.node_rows[0], q.f2, self.v_h_node, q.weights)
+        ops.ewise(ops.EW_MUL3, self.grid_rows[0], self.f2, self.v_h_grid, q.far_weights)
+        ops.gga_points(self.interp[1], self.node_grad, q.f2, q.grad_u, q.weights, q.weights, self.e_node,
+                       self.node_rows[1], self.node_rows[2], self.node_flux)
+        ops.gga_points(self.rho, self.grad_rho, self.f2, fac.grad_u, q.far_weights, q.far, self.e_grid,
+                       self.grid_rows[1], self.grid_rows[2], self.grid_flux)
+        q.lift(self.node_rows, add=self.grid_rows, out=self.lumped3)
+        q.lift(self.node_flux, out=self.lifted_flux)
+        ops.divergence(self.lifted_flux, g, self.near_div)
+        ops.divergence(self.grid_flux, g, self.far_div)
+        hartree_weights = self.lumped3[0]
+        ops.gga_combine(self.lumped3[1], self.near_div, self.far_div, self.lumped3[2], g.volume_element, self.xc_weights)
+        scalars = self.pot_scalars
+        ops.reduce(ops.RED_DOT, q.weights, scalars, 1, b=self.e_node)
+        ops.reduce_dots([(hartree_weights, self.rho, None, 0, 0.5, False),
+                         (q.far_weights, self.e_grid, None, 1, 1.0, True),
+                         (self.xc_weights, self.rho, None, 2, 1.0, False)], scalars)
+        ops.ewise(ops.EW_SUMDIV, self.pot_effective, hartree_weights, self.xc_weights, q.mass_weights)
 
     def hamiltonian(self, pot: dict) -> CuspFactoredHamiltonian:
         return CuspFactoredHamiltonian(self.grid, self.factor, self.q, extra_potential=pot["effective"])

@@ -335,6 +335,33 @@ int cdft_hydrogenic_fields_f64(const double* pts, long m, const int* shape, doub
  * (src/cdft/xc/lda.py:122-159,201-217; src/cdft/xc/base.py:115-210). */
 int cdft_lda_vwn_f64(const double* n, long len, double* e, double* v, void* stream);
 
+/* Spin-unpolarised PBE: e = e_x + e_c per volume at (n, sigma = |grad n|^2) with v_n = de/dn and
+ * v_sigma = de/dsigma (analytic), zero below n = 1e-14, sigma / 4 floored at 1e-40 as per spin.
+ * Replaces GGA(PBEExchange, PBECorrelation) through SemiLocalFunctional.evaluate
+ * (src/cdft/xc/gga.py:43-90; src/cdft/xc/lda.py:64-108 for the modified PW92; xc/base.py:115-210). */
+int cdft_pbe_f64(const double* n, const double* sigma, long len, double* e, double* vn, double* vs, void* stream);
+
+/* The pointwise sigma path of the GGA potentials at len points (the nodes, or the grid points):
+ * n = f2 rho, grad n = f2 (grad_rho - 2 rho grad_u), sigma = |grad n|^2, PBE at (n, sigma), then
+ * e, aw = w_s f2 v_n, crossw = w_s (-2 flux . grad_u) and fluxw = w_f flux with flux = 2 v_sigma f2 grad n;
+ * grad_rho, grad_u and fluxw are (3, len). Replaces the flux and cross terms of
+ * KohnShamStep.potentials (src/cdft/scf/step.py:253-279) with QuadratureOperator.density_gradient_at_nodes /
+ * density_gradient_on_grid (src/cdft/operators/quadrature.py:757-773). */
+int cdft_gga_points_f64(const double* rho, const double* grad_rho, const double* f2, const double* grad_u,
+                        const double* w_s, const double* w_f, long len, double* e, double* aw, double* crossw,
+                        double* fluxw, void* stream);
+
+/* out = d0 F0 + d1 F1 + d2 F2 of a (3, N) grid field (central order-`order` taps / h on the device, zero
+ * halo, axes added in order). The divergence of lift_gradient and lumped_gradient_weights
+ * (src/cdft/operators/quadrature.py:640-650,718-729; grid.partial_derivative, grid.py:748-758). */
+int cdft_divergence_f64(const double* field, const int* shape, const double* coef, int order, double* out,
+                        void* stream);
+
+/* out = (a + (-near_div - h3 far_div)) + cross: the GGA XC lumped weights from their scalar, gradient
+ * and cross parts, in the reference's order of addition (src/cdft/scf/step.py:264-271). */
+int cdft_gga_combine_f64(const double* a, const double* near_div, const double* far_div, const double* cross,
+                         double h3, long len, double* out, void* stream);
+
 /* out = op(a, b, c, s) elementwise over len values: 0 a b; 1 (a b) c; 2 s / a; 3 a / b; 4 a s;
  * 5 (a + b) / c; 6 a - b; 7 (a / s) s; 8 s; 9 a + b; 10 a (b . c) with b, c of shape (3, len);
  * 11 (a b_r) c for the two rows r of b and out (shape (2, len)).

@@ -112,6 +112,67 @@ def lda_vwn(density):
     return e, v
 
 
+_KF = (3.0 * math.pi**2) ** (1.0 / 3.0)
+_PBE_KAPPA = 0.804
+_PBE_MU = 0.2195149727645171
+_PBE_BETA = 0.06672455060314922
+_PBE_GAMMA = (1.0 - math.log(2.0)) / math.pi**2
+_PW92_MOD_PARA = (0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294)
+_SIGMA_THRESHOLD = 1.0e-40
+
+
+def _pbe_energy(n_up, n_dn, s_uu, s_ud, s_dd, torch):
+    """Spin-resolved PBE energy density per volume (reference ``xc/gga.py:30-90``, ``xc/lda.py:64-108``,
+    ``xc/base.py:198-235``), written with torch operations so that autograd gives the derivatives."""
+    total = torch.zeros_like(n_up)
+    for n_s, sig_s in ((n_up, s_uu), (n_dn, s_dd)):  # exchange, by spin scaling
+        n2 = 2.0 * n_s
+        s2 = (4.0 * sig_s) / (4.0 * _KF**2 * n2 ** (8.0 / 3.0))
+        fx = 1.0 + _PBE_KAPPA - _PBE_KAPPA / (1.0 + _PBE_MU * s2 / _PBE_KAPPA)
+        total = total + 0.5 * (-_SLATER * n2 ** (4.0 / 3.0)) * fx
+    n = n_up + n_dn
+    rs = (3.0 / (4.0 * torch.pi * n)) ** (1.0 / 3.0)
+    zeta = ((n_up - n_dn) / (n_up + n_dn)).clamp(-1.0 + 1.0e-15, 1.0 - 1.0e-15)
+    a, alpha1, beta1, beta2, beta3, beta4 = _PW92_MOD_PARA
+    sqrt_rs = rs.sqrt()
+    denominator = 2.0 * a * (beta1 * sqrt_rs + beta2 * rs + beta3 * rs * sqrt_rs + beta4 * rs * rs)
+    para = -2.0 * a * (1.0 + alpha1 * rs) * torch.log1p(1.0 / denominator)
+    # zeta = 0 for a restricted density: f(zeta) = 0, so the ferro and stiffness terms vanish exactly
+    fz = ((1.0 + zeta) ** (4.0 / 3.0) + (1.0 - zeta) ** (4.0 / 3.0) - 2.0) / (2.0 ** (4.0 / 3.0) - 2.0)
+    eps = para + 0.0 * fz
+    phi = 0.5 * ((1.0 + zeta) ** (2.0 / 3.0) + (1.0 - zeta) ** (2.0 / 3.0))
+    sigma = s_uu + 2.0 * s_ud + s_dd
+    k_f = _KF * n ** (1.0 / 3.0)
+    k_s_sq = 4.0 * k_f / math.pi
+    t2 = sigma / (4.0 * phi * phi * k_s_sq * n * n)
+    phi3 = phi**3
+    big_a = (_PBE_BETA / _PBE_GAMMA) / (torch.exp(-eps / (_PBE_GAMMA * phi3)) - 1.0)
+    at2 = big_a * t2
+    h = _PBE_GAMMA * phi3 * torch.log1p((_PBE_BETA / _PBE_GAMMA) * t2 * (1.0 + at2) / (1.0 + at2 + at2 * at2))
+    return total + n * (eps + h)
+
+
+def pbe_xc(density, sigma):
+    """``(e_xc, de/dn, de/dsigma)`` per volume of spin-restricted PBE at a total density ``(N,)`` and
+    ``sigma = |grad n|^2``: the reference's spin split (``n_s = n/2``, ``sigma_ss = sigma/4``), clamps
+    and empty-point mask (``xc/base.py:55-71,171-189``), and its derivatives by torch autograd on the
+    CPU, as ``SemiLocalFunctional.evaluate`` takes them (``xc/base.py:121-157``)."""
+    import torch
+
+    dens = torch.as_tensor(np.asarray(density, dtype=np.float64)).clone().requires_grad_(True)
+    sig = torch.as_tensor(np.asarray(sigma, dtype=np.float64)).clone().requires_grad_(True)
+    with torch.enable_grad():
+        n_up = n_dn = 0.5 * dens
+        s_uu = s_ud = s_dd = 0.25 * sig
+        empty = (n_up + n_dn) < DENSITY_THRESHOLD
+        floor = 0.5 * DENSITY_THRESHOLD
+        e = _pbe_energy(n_up.clamp_min(floor), n_dn.clamp_min(floor), s_uu.clamp_min(_SIGMA_THRESHOLD), s_ud,
+                        s_dd.clamp_min(_SIGMA_THRESHOLD), torch)
+        e = torch.where(empty, torch.zeros_like(e), e)
+        vn, vs = torch.autograd.grad(e.sum(), [dens, sig])
+    return e.detach().numpy(), vn.numpy(), vs.numpy()
+
+
 # --------------------------------------------------------------------------------------------
 # Scenario and numerics
 # --------------------------------------------------------------------------------------------
@@ -119,7 +180,7 @@ def lda_vwn(density):
 
 @dataclass
 class Scenario:
-    """Nuclei (bohr), charges, electron count and functional (``"none"`` or ``"lda"``)."""
+    """Nuclei (bohr), charges, electron count and functional (``"none"``, ``"lda"`` or ``"pbe"``)."""
 
     charges: tuple
     positions: tuple
@@ -216,10 +277,28 @@ class KohnShamStep:
         n_node = q.f2 * q.interpolate(rho)
         hartree_weights = q.lumped_weights(q.f2 * v_h_node, self.f2 * v_h_grid)
         hartree_energy = 0.5 * (hartree_weights * rho).sum()
-        e_grid, v_grid = lda_vwn(n_grid)
-        e_node, v_node = lda_vwn(n_node)
+        if self.scenario.xc == "pbe":
+            # the sigma path (reference ``scf/step.py:253-279``)
+            grad_rho = self.grid.gradient(rho)
+            rho_node = q.interpolate(rho)
+            grad_n_node = q.f2[None, :] * (q.interpolate(grad_rho) - 2.0 * rho_node[None, :] * q.grad_u)
+            grad_n_grid = self.f2[None, :] * (grad_rho - 2.0 * rho[None, :] * self.factor.grad_u)
+            sigma_grid = (grad_n_grid * grad_n_grid).sum(axis=0)
+            sigma_node = (grad_n_node * grad_n_node).sum(axis=0)
+            e_grid, v_grid, s_grid = pbe_xc(n_grid, sigma_grid)
+            e_node, v_node, s_node = pbe_xc(n_node, sigma_node)
+            xc_weights = q.lumped_weights(q.f2 * v_node, self.f2 * v_grid)
+            flux_node = 2.0 * s_node[None, :] * q.f2[None, :] * grad_n_node
+            flux_grid = 2.0 * s_grid[None, :] * self.f2[None, :] * grad_n_grid
+            xc_weights = xc_weights + q.lumped_gradient_weights(flux_node, flux_grid)
+            cross_node = -2.0 * (flux_node * q.grad_u).sum(axis=0)
+            cross_grid = -2.0 * (flux_grid * self.factor.grad_u).sum(axis=0)
+            xc_weights = xc_weights + q.lumped_weights(cross_node, cross_grid)
+        else:
+            e_grid, v_grid = lda_vwn(n_grid)
+            e_node, v_node = lda_vwn(n_node)
+            xc_weights = q.lumped_weights(q.f2 * v_node, self.f2 * v_grid)
         xc_energy = q.integrate(e_node, e_grid)
-        xc_weights = q.lumped_weights(q.f2 * v_node, self.f2 * v_grid)
         dc = (xc_weights * rho).sum()
         effective = (hartree_weights + xc_weights) / q.mass_weights
         return {

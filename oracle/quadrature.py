@@ -317,6 +317,23 @@ class CuspQuadrature:
     def lumped_weights(self, node_values, grid_values):
         return self.lift(self.weights * node_values) + self.far_weights * grid_values
 
+    def lift_gradient(self, node_vectors):
+        """``-div_FD (L^T X)`` for a ``(3, m)`` node field (reference ``quadrature.py:640-650``)."""
+        lifted = self.lift(node_vectors)
+        divergence = np.zeros(self.grid.n_points)
+        for d in range(3):
+            divergence = divergence + self.grid.partial_derivative(lifted[d], d)
+        return -divergence
+
+    def lumped_gradient_weights(self, node_vectors, grid_vectors):
+        """``lift_gradient(W X_node) - h^3 div_FD(far X_grid)`` (reference ``quadrature.py:718-729``)."""
+        near = self.lift_gradient(self.weights * node_vectors)
+        weighted = self.far[None, :] * grid_vectors
+        divergence = np.zeros(self.grid.n_points)
+        for d in range(3):
+            divergence = divergence + self.grid.gradient(weighted[d])[d]
+        return near - self.grid.volume_element * divergence
+
     def effective_potential(self, node_values, grid_values):
         tilde = self.lumped_weights(node_values * self.f2, grid_values * self.factor.weight)
         return tilde / self.mass_weights

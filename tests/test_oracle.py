@@ -171,3 +171,61 @@ def test_h2plus_ladder_rung_h05():
     r = solve_noninteracting(sc, Numerics(box_lengths=(14.0, 14.0, 14.0), spacing=0.5), anchor=(0.0, 0.0, -1.0))
     assert r["hamiltonian"].quadrature.far_lumped
     assert abs(r["eigenvalues"][0] - GOLDEN["ladder.h2plus_R2.h0.5.eigenvalue"]) < 1e-11
+
+
+def test_pbe_limits_of_the_oracle():
+    """The oracle's PBE (reference ``xc/gga.py:43-90``) against what its formulas fix independently:
+    no gradient gives Slater exchange plus the modified PW92 correlation, which agrees with VWN5 (a
+    separate fit of the same Ceperley-Alder data, pinned by the LDA fixtures) to 1 mHa per electron;
+    the exchange enhancement tends to 1 + kappa (the Lieb-Oxford bound) at large reduced gradient and
+    to 1 + mu s^2 at small; the correlation vanishes as t -> infinity and its gradient term is
+    beta t^2 at small t (PBE eqs. 7, 8, 14)."""
+    from oracle import physics as op
+
+    kf = (3 * math.pi**2) ** (1 / 3)
+    slater = 0.75 * (3 / math.pi) ** (1 / 3)
+    n = 10 ** np.linspace(-3, 1, 40)
+    e0, _, _ = op.pbe_xc(n, np.zeros_like(n))
+    elda, _ = op.lda_vwn(n)
+    assert np.max(np.abs(e0 - elda) / n) < 1e-3
+    one = np.array([0.37])
+    big, _, _ = op.pbe_xc(one, np.array([1e40]))
+    assert abs(big[0] / (-slater * one[0] ** (4 / 3) * (1 + 0.804)) - 1) < 1e-10
+    # small gradient: the exchange term -C n^{4/3} mu s^2 and the correlation term n beta t^2 cancel
+    # (mu = beta pi^2 / 3 is chosen so, PBE eq. 12), leaving a change of second order in s^2
+    s2 = 1e-7
+    sigma = s2 * 4 * kf**2 * one ** (8 / 3)
+    t2 = sigma * math.pi / (16 * kf * one ** (7 / 3))
+    es, _, _ = op.pbe_xc(one, sigma)
+    e_flat, _, _ = op.pbe_xc(one, np.array([0.0]))
+    x_term = -slater * one[0] ** (4 / 3) * 0.2195149727645171 * s2
+    c_term = one[0] * 0.06672455060314922 * t2[0]
+    assert abs(x_term + c_term) < 1e-12 * abs(x_term)
+    assert abs((es - e_flat)[0]) < 1e-4 * abs(x_term)
+    # and the correlation alone: with exchange held at the uniform gas (s^2 -> 0 only in F), the
+    # gradient term is n beta t^2; recovered from the total by removing the exchange enhancement
+    ex_s = -slater * one[0] ** (4 / 3) * (1 + 0.804 - 0.804 / (1 + 0.2195149727645171 * s2 / 0.804))
+    ex_0 = -slater * one[0] ** (4 / 3)
+    assert abs(((es - e_flat)[0] - (ex_s - ex_0)) / c_term - 1) < 1e-5
+    # correlation vanishes at large t: the total tends to the exchange limit alone
+    huge = np.array([1e12]) * 4 * kf**2 * one ** (8 / 3)
+    eh, _, _ = op.pbe_xc(one, huge)
+    ex_h = -slater * one ** (4 / 3) * (1 + 0.804 - 0.804 / (1 + 0.2195149727645171 * 1e12 / 0.804))
+    assert abs(eh[0] - ex_h[0]) < 1e-8 * abs(e_flat[0])
+
+
+def test_pbe_derivatives_of_the_oracle_match_differences():
+    """``de/dn`` and ``de/dsigma`` (by autograd, as the reference takes them) against central differences
+    at 2000 seeded points, relative step 1e-5, reduced gradients s^2 in [1e-2, 10]."""
+    from oracle import physics as op
+
+    kf = (3 * math.pi**2) ** (1 / 3)
+    rng = np.random.default_rng(0)
+    n = 10 ** rng.uniform(-4, 2, 2000)
+    sig = 10 ** rng.uniform(-2, 1, 2000) * 4 * kf**2 * n ** (8 / 3)
+    e, vn, vs = op.pbe_xc(n, sig)
+    h = 1e-5
+    fds = (op.pbe_xc(n, sig * (1 + h))[0] - op.pbe_xc(n, sig * (1 - h))[0]) / (2 * h * sig)
+    fdn = (op.pbe_xc(n * (1 + h), sig)[0] - op.pbe_xc(n * (1 - h), sig)[0]) / (2 * h * n)
+    assert (np.abs(fds - vs) * sig / np.abs(e)).max() < 1e-9
+    assert (np.abs(fdn - vn) * n / np.abs(e)).max() < 1e-9

@@ -228,7 +228,10 @@ def test_h2_lda_scf_matches_oracle(pkg, device):
     r = scf.solve_self_consistent(sc, num, device, anchor=anchor)
     o = ophys.solve_self_consistent(sc, num, anchor=anchor)
     assert r["scf_iterations"] == o["scf_iterations"]
-    assert abs(r["energies"]["total"] - o["energies"]["total"]) < 1e-10
+    for term in ("total", "harris_foulkes", "kinetic", "external", "hartree", "xc"):
+        print(term, r["energies"][term], float(o["energies"][term]), flush=True)
+    for term in ("total", "harris_foulkes", "kinetic", "external", "hartree", "xc"):
+        assert abs(r["energies"][term] - float(o["energies"][term])) < 1e-10, term
 
 
 def test_captured_potentials_and_energies_match_eager(pkg, device):
