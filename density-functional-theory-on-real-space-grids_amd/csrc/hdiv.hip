@@ -238,18 +238,29 @@ __device__ __forceinline__ void stencil_block(
     for (int j = 0; j < 2 * P; ++j) qphi[j] = pen[2 * P - 1 + j];
   }
 
-  // Per-plane operands, loaded one plane ahead (their latency overlaps the LDS work of a plane).
+  // Per-plane operands used before the plane's first barrier, loaded one plane ahead (their latency
+  // overlaps the LDS work of a plane).
   struct Plane {
     T wxf;        // w0 at the newest x-face of the plane (extended index m + 2P - 1)
-    T phn;        // phi(m + 2P): enters the x queue after the plane
     T hal[MAXH];  // y/z halo values
-    T wy0, wy1, wz0, wz1;
-    T iw, pt, pr;
+    T wy0, wz0, iw;
+  };
+  // Operands first used after the plane's first barrier: loaded at the start of the plane's own
+  // body, ahead of the next plane's prefetch, so only one copy of them is ever live.
+  struct Late {
+    T phn;  // phi(m + 2P): enters the x queue after the plane
+    T wy1, wz1;
+    T pt, pr;
   };
   auto load_plane = [&](int m, Plane& L) {
     const int base = m * plane * S;
     bload_rec<T>(r_gp, rec, m * n1f * n2f * 4 * S, L.wxf, L.wy0, L.wz0, L.iw);
     if (MODE == 3) L.iw = T(0);
+#pragma unroll
+    for (int s = 0; s < MAXH; ++s) L.hal[s] = bload<T>(r_phi, hsrc[s], base);
+  };
+  auto load_late = [&](int m, Late& L) {
+    const int base = m * plane * S;
     {
       // loaded unconditionally (a clamped plane, then a select): a load on only one side of a
       // branch makes the compiler's wait counting assume the worst and wait for every load
@@ -257,15 +268,13 @@ __device__ __forceinline__ void stencil_block(
       const T y = bload<T>(r_phi, col, mp * plane * S);
       L.phn = m + 2 * P < n0 ? y : T(0);
     }
-#pragma unroll
-    for (int s = 0; s < MAXH; ++s) L.hal[s] = bload<T>(r_phi, hsrc[s], base);
     L.wy1 = bload<T>(r_w1, wy1_off, m * pl1 * S);
     L.wz1 = bload<T>(r_w2, wz1_off, m * pl2 * S);
     L.pt = bload<T>(r_pt, col, base);
     L.pr = MODE >= 2 ? bload<T>(r_prev, col, base) : T(0);
   };
 
-  auto body = [&](auto rot, int m, const Plane& cur) __attribute__((always_inline)) {
+  auto body = [&](auto rot, int m, const Plane& cur, const Late& late) __attribute__((always_inline)) {
     // queue entry i of this plane (i = 0 the oldest) lives in slot (SR + i) % 2P
     constexpr int SR = decltype(rot)::value;
 #define QI(i) (((SR) + (i)) % (2 * P))
@@ -302,13 +311,13 @@ __device__ __forceinline__ void stencil_block(
         f = 0;
 #pragma unroll
         for (int i = 0; i < 2 * P; ++i) f += c[i] * s_phi[(ty + TY + i) * LZ + tz + R];
-        s_yf[(ty + TY) * YP + tz] = f * cur.wy1;
+        s_yf[(ty + TY) * YP + tz] = f * late.wy1;
       }
       if (tz < R) {
         f = 0;
 #pragma unroll
         for (int i = 0; i < 2 * P; ++i) f += c[i] * s_phi[(ty + R) * LZ + tz + TZ + i];
-        s_zf[ty * ZP + tz + TZ] = f * cur.wz1;
+        s_zf[ty * ZP + tz + TZ] = f * late.wz1;
       }
       // Tiles narrower than R (grids thinner than 2P-1 points) own further face rows/columns.
       for (int r = ty + 2 * TY; r < TY + R; r += TY) {
@@ -338,68 +347,87 @@ __device__ __forceinline__ void stencil_block(
       for (int i = 0; i < 2 * P; ++i) az += c[i] * s_zf[ty * ZP + tz + R - i];
       const T lap = (ax + ay) + az;
       const T pv = qphi[QI(0)];
-      T val = MODE == 3 ? lap * T(0.5) + cur.pt * cur.pr : lap * T(0.5) * cur.iw + pv * cur.pt;
+      T val = MODE == 3 ? lap * T(0.5) + late.pt * late.pr : lap * T(0.5) * cur.iw + pv * late.pt;
       if (MODE == 1) val = (val - cc * pv) * ca;
-      if (MODE == 2) val = (val - cc * pv) * ca - cb * cur.pr;
+      if (MODE == 2) val = (val - cc * pv) * ca - cb * late.pr;
       if (MODE == 4) {
         const T r = val - ev * pv;
-        racc += (r * r) * cur.pr;
+        racc += (r * r) * late.pr;
       } else {
         bstore<T>(val, r_out, col, m * plane * S);
       }
     }
 
     // advance the x queues and the plane pipeline
-    qphi[QI(0)] = cur.phn;  // becomes logical index 2P-1 of the next plane
+    qphi[QI(0)] = late.phn;  // becomes logical index 2P-1 of the next plane
 #undef QI
   };
   // Two plane buffers used in turn: the next plane's loads land in the other buffer and no register
   // with a load in flight is ever copied (a copy waits for the load). The prefetch past the segment's
-  // last plane reloads that plane, so every iteration issues the same loads. The loop is unrolled
-  // over the x queues' period 2P, so the queues rotate by renaming (the body's compile-time `rot`)
-  // instead of 4P-2 register moves per plane.
+  // last plane reloads that plane, so every iteration issues the same loads. Each plane first issues
+  // its own late operands, then the next plane's prefetch: loads complete in issue order, so the wait
+  // for the late operands at the end of the plane leaves the prefetch in flight. Keeping the late
+  // operands out of the double buffer holds the float64 order-8 kernels at or below 128 VGPRs (four
+  // waves per SIMD instead of three). The loop is unrolled over the x queues' period 2P, so the
+  // queues rotate by renaming (the body's compile-time `rot`) instead of 4P-2 register moves per plane.
   static_assert(2 * P <= 8, "the unrolled plane loop covers queue periods up to 8");
   Plane pa, pb;
   load_plane(X0, pa);
   for (int m = X0; m < X1; m += 2 * P) {
     if constexpr (0 < 2 * P) {
+      Late late;
+      load_late(m + 0, late);
       load_plane(m + 1 < X1 ? m + 1 : X1 - 1, pb);
-      body(std::integral_constant<int, 0>(), m + 0, pa);
+      body(std::integral_constant<int, 0>(), m + 0, pa, late);
     }
     if constexpr (1 < 2 * P) {
       if (m + 1 >= X1) break;
+      Late late;
+      load_late(m + 1, late);
       load_plane(m + 2 < X1 ? m + 2 : X1 - 1, pa);
-      body(std::integral_constant<int, 1>(), m + 1, pb);
+      body(std::integral_constant<int, 1>(), m + 1, pb, late);
     }
     if constexpr (2 < 2 * P) {
       if (m + 2 >= X1) break;
+      Late late;
+      load_late(m + 2, late);
       load_plane(m + 3 < X1 ? m + 3 : X1 - 1, pb);
-      body(std::integral_constant<int, 2>(), m + 2, pa);
+      body(std::integral_constant<int, 2>(), m + 2, pa, late);
     }
     if constexpr (3 < 2 * P) {
       if (m + 3 >= X1) break;
+      Late late;
+      load_late(m + 3, late);
       load_plane(m + 4 < X1 ? m + 4 : X1 - 1, pa);
-      body(std::integral_constant<int, 3>(), m + 3, pb);
+      body(std::integral_constant<int, 3>(), m + 3, pb, late);
     }
     if constexpr (4 < 2 * P) {
       if (m + 4 >= X1) break;
+      Late late;
+      load_late(m + 4, late);
       load_plane(m + 5 < X1 ? m + 5 : X1 - 1, pb);
-      body(std::integral_constant<int, 4>(), m + 4, pa);
+      body(std::integral_constant<int, 4>(), m + 4, pa, late);
     }
     if constexpr (5 < 2 * P) {
       if (m + 5 >= X1) break;
+      Late late;
+      load_late(m + 5, late);
       load_plane(m + 6 < X1 ? m + 6 : X1 - 1, pa);
-      body(std::integral_constant<int, 5>(), m + 5, pb);
+      body(std::integral_constant<int, 5>(), m + 5, pb, late);
     }
     if constexpr (6 < 2 * P) {
       if (m + 6 >= X1) break;
+      Late late;
+      load_late(m + 6, late);
       load_plane(m + 7 < X1 ? m + 7 : X1 - 1, pb);
-      body(std::integral_constant<int, 6>(), m + 6, pa);
+      body(std::integral_constant<int, 6>(), m + 6, pa, late);
     }
     if constexpr (7 < 2 * P) {
       if (m + 7 >= X1) break;
+      Late late;
+      load_late(m + 7, late);
       load_plane(m + 8 < X1 ? m + 8 : X1 - 1, pa);
-      body(std::integral_constant<int, 7>(), m + 7, pb);
+      body(std::integral_constant<int, 7>(), m + 7, pb, late);
     }
   }
   if constexpr (MODE == 4) {

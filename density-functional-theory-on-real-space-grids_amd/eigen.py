@@ -226,7 +226,7 @@ class ChebyshevFilteredSubspace:
         self.filter_precisions: list[str] = []
         self.speculation = {"used": 0, "redone": 0, "wasted": 0}
         self._work: dict[int, RitzWork] = {}
-        self._graphs: dict[tuple, torch.cuda.CUDAGraph] = {}
+        self._graphs: dict[tuple, ops.CapturedGraph] = {}
         n = grid.n_points
         self._pot = torch.empty(n, dtype=torch.float64, device=grid.device)
         self._pot32 = torch.empty(n, dtype=torch.float32, device=grid.device) if filter_dtype != torch.float64 else None

@@ -54,8 +54,45 @@ def workspace_key(device: torch.device) -> tuple:
 
 _capture_streams: dict = {}
 
+# Captures and graph destruction exclude each other process-wide: destroying a graph (its executable
+# and private memory pool) while another host thread records a capture aborts the process. Graphs
+# that are dropped go to `_released` and are destroyed under the lock, when a capture starts or ends.
+_graph_lock = threading.RLock()
+_released: list = []
 
-def capture_graph(body) -> torch.cuda.CUDAGraph:
+
+class CapturedGraph:
+    """A captured HIP graph. Dropping it never destroys the graph at once: the graph is handed to
+    the next capture (in any thread), which destroys it while no capture is being recorded."""
+
+    __slots__ = ("_graph",)
+
+    def __init__(self, graph: torch.cuda.CUDAGraph):
+        self._graph = graph
+
+    def replay(self) -> None:
+        self._graph.replay()
+
+    def __del__(self):
+        graph = getattr(self, "_graph", None)
+        if graph is not None:
+            _released.append(graph)  # a single list append: atomic under the interpreter lock
+            self._graph = None
+
+
+def _destroy_released() -> None:
+    """Destroy the dropped graphs; the caller holds ``_graph_lock`` and records no capture."""
+    while _released:
+        _released.pop()
+
+
+def release_graphs() -> None:
+    """Destroy every dropped graph now (waits for a capture in another thread to finish)."""
+    with _graph_lock:
+        _destroy_released()
+
+
+def capture_graph(body) -> CapturedGraph:
     """``body``'s launches captured into a HIP graph, to be replayed on the current stream.
 
     Captures on a side stream (one per device and origin stream) without draining the device first
@@ -63,25 +100,29 @@ def capture_graph(body) -> torch.cuda.CUDAGraph:
     capture, which leaves the GPU idle while the host records), in thread-local mode, so work that
     host threads enqueue meanwhile (pinned draws) cannot invalidate it. The graph's allocations come
     from its own private pool, as with ``torch.cuda.graph``. Scratch buffers the captured calls take
-    are those of the origin stream (:func:`workspace_key`)."""
+    are those of the origin stream (:func:`workspace_key`). Captures from several host threads take
+    turns, and no graph is destroyed while one is recorded (:class:`CapturedGraph`)."""
     device = torch.cuda.current_device()
     key = workspace_key(torch.device("cuda", device))
-    stream = _capture_streams.get(key)
-    if stream is None:
-        stream = _capture_streams[key] = torch.cuda.Stream(device)
-    graph = torch.cuda.CUDAGraph()
-    outer = getattr(_capture_origin, "stream", None)
-    _capture_origin.stream = key[1]
-    try:
-        with torch.cuda.stream(stream):
-            graph.capture_begin(capture_error_mode="thread_local")
-            try:
-                body()
-            finally:
-                graph.capture_end()
-    finally:
-        _capture_origin.stream = outer
-    return graph
+    with _graph_lock:
+        _destroy_released()
+        stream = _capture_streams.get(key)
+        if stream is None:
+            stream = _capture_streams[key] = torch.cuda.Stream(device)
+        graph = torch.cuda.CUDAGraph()
+        outer = getattr(_capture_origin, "stream", None)
+        _capture_origin.stream = key[1]
+        try:
+            with torch.cuda.stream(stream):
+                graph.capture_begin(capture_error_mode="thread_local")
+                try:
+                    body()
+                finally:
+                    graph.capture_end()
+        finally:
+            _capture_origin.stream = outer
+        _destroy_released()  # graphs the body dropped wait until the capture has ended
+    return CapturedGraph(graph)
 
 
 def require(t: torch.Tensor, shape: tuple, dtype: torch.dtype, device: torch.device, name: str) -> None:

@@ -147,3 +147,48 @@ def test_node_table_is_the_shell_loop(pkg):
             assert np.array_equal(a, b)
         points, weights = gm.sphere_nodes(centres[0], radius, h, 6, ns)
         assert points.shape == (weights.shape[0], 3)
+
+
+def test_dropped_graphs_wait_for_a_capture_free_moment(pkg):
+    """A dropped captured graph is not destroyed where the last reference goes (which may be while
+    another host thread records a capture) but queued, and destroyed under the capture lock."""
+    import threading
+
+    from cdft_amd import ops
+
+    destroyed = []
+
+    class FakeGraph:
+        def __init__(self, name):
+            self.name = name
+
+        def replay(self):
+            pass
+
+        def __del__(self):
+            destroyed.append(self.name)
+
+    ops.release_graphs()
+    g = ops.CapturedGraph(FakeGraph("a"))
+    g.replay()
+    del g
+    assert destroyed == [] and len(ops._released) == 1
+    # while another thread holds the capture lock, release_graphs waits
+    held, done = threading.Event(), threading.Event()
+
+    def hold():
+        with ops._graph_lock:
+            held.set()
+            done.wait(5)
+
+    t = threading.Thread(target=hold)
+    t.start()
+    held.wait(5)
+    waiter = threading.Thread(target=ops.release_graphs)
+    waiter.start()
+    waiter.join(0.2)
+    assert waiter.is_alive() and destroyed == []
+    done.set()
+    t.join(5)
+    waiter.join(5)
+    assert destroyed == ["a"] and not ops._released
