@@ -41,4 +41,22 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned block, unsigned nblocks) {
   return base + local;
 }
 
+// Float64 products, sums and differences rounded one operation at a time, never contracted into a
+// fused multiply-add. (The HIP headers define __dmul_rn / __dadd_rn as the plain operators, which
+// hipcc contracts like any other.) The reference's CPU expressions round every operation, and the
+// geometry compares lattice coordinates and distances against thresholds such as r <= 2h: one ulp
+// moves a point that sits on a threshold across it.
+__device__ __forceinline__ double rn_mul(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ double rn_add(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+__device__ __forceinline__ double rn_sub(double a, double b) {
+#pragma clang fp contract(off)
+  return a - b;
+}
+
 }  // namespace cdft

@@ -134,18 +134,18 @@ __global__ void gga_points_kernel(const double* __restrict__ rho, const double* 
   const double two_r = 2.0 * r;
   double gn[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) gn[d] = f * __dsub_rn(grad_rho[d * len + p], __dmul_rn(two_r, grad_u[d * len + p]));
-  const double sigma = __dadd_rn(__dadd_rn(__dmul_rn(gn[0], gn[0]), __dmul_rn(gn[1], gn[1])), __dmul_rn(gn[2], gn[2]));
+  for (int d = 0; d < 3; ++d) gn[d] = f * rn_sub(grad_rho[d * len + p], rn_mul(two_r, grad_u[d * len + p]));
+  const double sigma = rn_add(rn_add(rn_mul(gn[0], gn[0]), rn_mul(gn[1], gn[1])), rn_mul(gn[2], gn[2]));
   double ep, vn, vs;
   pbe_point(f * r, sigma, &ep, &vn, &vs);
   e[p] = ep;
   const double a = f * vn;
-  const double coef = __dmul_rn(2.0 * vs, f);
+  const double coef = rn_mul(2.0 * vs, f);
   double fl[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) fl[d] = __dmul_rn(coef, gn[d]);
-  const double dot = __dadd_rn(__dadd_rn(__dmul_rn(fl[0], grad_u[p]), __dmul_rn(fl[1], grad_u[len + p])),
-                               __dmul_rn(fl[2], grad_u[2 * len + p]));
+  for (int d = 0; d < 3; ++d) fl[d] = rn_mul(coef, gn[d]);
+  const double dot = rn_add(rn_add(rn_mul(fl[0], grad_u[p]), rn_mul(fl[1], grad_u[len + p])),
+                               rn_mul(fl[2], grad_u[2 * len + p]));
   const double cross = -2.0 * dot;
   const double ws = w_s[p], wf = w_f[p];
   aw[p] = ws * a;
@@ -195,8 +195,8 @@ __global__ void gga_combine_kernel(const double* __restrict__ a, const double* _
                                    long len, double* __restrict__ out) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= len) return;
-  const double grad = __dsub_rn(-near_div[p], __dmul_rn(h3, far_div[p]));
-  out[p] = __dadd_rn(__dadd_rn(a[p], grad), cross[p]);
+  const double grad = rn_sub(-near_div[p], rn_mul(h3, far_div[p]));
+  out[p] = rn_add(rn_add(a[p], grad), cross[p]);
 }
 
 static unsigned gga_blocks(long len) { return (unsigned)((len + 255) / 256); }

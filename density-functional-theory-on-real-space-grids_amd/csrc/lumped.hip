@@ -33,7 +33,7 @@ __global__ void far_fine_kernel(FarSpheres sp, int f0, int f1, int f2, const dou
   for (int a = 0; a < sp.n; ++a) {
     const double dx = x - sp.cx[a], dy = y - sp.cy[a], dz = z - sp.cz[a];
     // separately rounded, as the reference's torch expression (s < radius is a threshold test)
-    const double s = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)));
+    const double s = sqrt(rn_add(rn_add(rn_mul(dx, dx), rn_mul(dy, dy)), rn_mul(dz, dz)));
     const double bump = s < sp.radius[a] ? 0.5 * erfc((s - sp.centre[a]) / sp.width[a]) : 0.0;
     far = far * (1.0 - bump);
     u = u + sp.z[a] * s;
@@ -74,8 +74,10 @@ __global__ void far_blend_kernel(int n0, int n1, int n2, double h, double ox, do
   const int i0 = (int)(p / ((long)n1 * n2));
   // Distances to the lower and upper faces, formed from the point coordinates x = origin + i h as
   // (x - origin) and (origin + (n - 1) h - x).
-  const double x = ox + (double)i0 * h, y = oy + (double)i1 * h, z = oz + (double)i2 * h;
-  const double ux = ox + (double)(n0 - 1) * h, uy = oy + (double)(n1 - 1) * h, uz = oz + (double)(n2 - 1) * h;
+  const double x = rn_add(ox, rn_mul((double)i0, h)), y = rn_add(oy, rn_mul((double)i1, h)),
+               z = rn_add(oz, rn_mul((double)i2, h));
+  const double ux = rn_add(ox, rn_mul((double)(n0 - 1), h)), uy = rn_add(oy, rn_mul((double)(n1 - 1), h)),
+               uz = rn_add(oz, rn_mul((double)(n2 - 1), h));
   const double lo = fmin(fmin(x - ox, y - oy), z - oz);
   const double hi = fmin(fmin(ux - x, uy - y), uz - z);
   const double face = fmin(lo, hi);

@@ -48,12 +48,12 @@ struct Spheres {
 // multiply-add), as the reference's torch CPU expressions round them: a point two spacings from a
 // nucleus sits on the cell-average threshold, and a coordinate one ulp off moves it across.
 __device__ __forceinline__ double dist3(double dx, double dy, double dz) {
-  return sqrt(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)));
+  return sqrt(rn_add(rn_add(rn_mul(dx, dx), rn_mul(dy, dy)), rn_mul(dz, dz)));
 }
 
 // origin + index * h, the reference's grid point coordinate (grid.py:667-670, cusp.py:190-196).
 __device__ __forceinline__ double lattice(double origin, double index, double h) {
-  return __dadd_rn(origin, __dmul_rn(index, h));
+  return rn_add(origin, rn_mul(index, h));
 }
 
 __global__ void cusp_grid_fields_kernel(GridDesc g, Nuclei nu, double reach, CellRule rule,

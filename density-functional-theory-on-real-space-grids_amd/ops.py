@@ -73,7 +73,7 @@ class CapturedGraph:
     def replay(self) -> None:
         self._graph.replay()
 
-    def __del__(self):
+    def __del__(self, _released=_released):  # bound here: module globals are gone at interpreter exit
         graph = getattr(self, "_graph", None)
         if graph is not None:
             _released.append(graph)  # a single list append: atomic under the interpreter lock
