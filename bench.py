@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the fused step on this workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
 # `bench.py --kernel-only`, each scaled by the same counter on a 1 GiB copy in the same run (summary and
 # commands: profiles/hdiv_fused_step_he65_k5.md). Re-measured whenever the stencil kernel changes.
-FUSED_STEP_TRAFFIC_BYTES = (27880 / 0.500 + 11689 / 1.000) * 1024
+FUSED_STEP_TRAFFIC_BYTES = (27961 / 0.500 + 11462 / 1.000) * 1024
 METRIC = "He-LDA SCF s/solve & H-apply GB/s vs HBM peak; mixed vs FP64 at <=1e-8 Ha"
 PUBLISHED_CPU_S_PER_SOLVE = 109.5  # reference, He LDA 65^3, CPU float64 2 threads (convergence_ladders.json)
 
