@@ -57,8 +57,16 @@ _capture_streams: dict = {}
 # Captures and graph destruction exclude each other process-wide: destroying a graph (its executable
 # and private memory pool) while another host thread records a capture aborts the process. Graphs
 # that are dropped go to `_released` and are destroyed under the lock, when a capture starts or ends.
+# Other device-side releases (FFT plans) go there too, as callables: a finaliser runs wherever the
+# last reference goes, including inside a capture, where a free or a synchronisation invalidates it.
 _graph_lock = threading.RLock()
 _released: list = []
+
+
+def release_later(release) -> None:
+    """Run ``release()`` (a device-side free) at the next capture-free moment (:func:`capture_graph`
+    or :func:`release_graphs`) instead of now. Safe to call from a finaliser."""
+    _released.append(release)
 
 
 class CapturedGraph:
@@ -81,13 +89,21 @@ class CapturedGraph:
 
 
 def _destroy_released() -> None:
-    """Destroy the dropped graphs; the caller holds ``_graph_lock`` and records no capture."""
+    """Destroy the dropped graphs and run the deferred releases; the caller holds ``_graph_lock`` and
+    records no capture."""
     while _released:
-        _released.pop()
+        item = _released.pop()
+        if callable(item):
+            try:
+                item()
+            except Exception:  # noqa: BLE001 - a failed free must not stop the others
+                pass
+        del item
 
 
 def release_graphs() -> None:
-    """Destroy every dropped graph now (waits for a capture in another thread to finish)."""
+    """Destroy every dropped graph and run every deferred release now (waits for a capture in another
+    thread to finish)."""
     with _graph_lock:
         _destroy_released()
 
@@ -107,6 +123,11 @@ def capture_graph(body) -> CapturedGraph:
     with _graph_lock:
         _destroy_released()
         stream = _capture_streams.get(key)
+        if stream is not None:
+            with torch.cuda.stream(stream):
+                stuck = torch.cuda.is_current_stream_capturing()
+            if stuck:  # a capture that failed was left open on it: never begin another there
+                stream = None
         if stream is None:
             stream = _capture_streams[key] = torch.cuda.Stream(device)
         graph = torch.cuda.CUDAGraph()

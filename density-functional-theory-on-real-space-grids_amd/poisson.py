@@ -15,7 +15,7 @@ import math
 
 import torch
 
-from . import _lib
+from . import _lib, ops
 from .ops import stream_of, workspace_key
 
 
@@ -53,10 +53,8 @@ class _Plan:
     def __del__(self):
         handle = getattr(self, "handle", None)
         if handle is not None and handle.value:
-            try:
-                _lib.call("cdft_poisson_destroy", handle)
-            except Exception:  # noqa: BLE001 - interpreter shutdown
-                pass
+            # destroying the plans frees device memory: deferred to a moment no capture is recorded
+            ops.release_later(lambda h=handle: _lib.call("cdft_poisson_destroy", h))
             self.handle = None
 
 

@@ -472,12 +472,28 @@ def subspace_solver(grid, config: EigenConfig, filter_dtype, filter_hook) -> Che
     return solver
 
 
+def two_solver_check(hamiltonian, grid, config, n_states: int, seed: int, first_eigenvalues) -> dict:
+    """LOBPCG on the same operator, seeded with ``seed + 1``, against the CheFSI eigenvalues
+    ``first_eigenvalues`` (reference ``scf/loop.py:485-494``, ``scf/noninteracting.py:182-189``; gate
+    G2.5 holds the largest difference to ``cross_check_tol``)."""
+    from .lobpcg import LOBPCG
+
+    second = LOBPCG(grid, config).solve(hamiltonian, n_states, generator=torch.Generator(device="cpu").manual_seed(seed + 1))
+    first = np.asarray(first_eigenvalues, dtype=np.float64)[:n_states]
+    delta = np.abs(first - second.eigenvalues[: first.shape[0]])
+    return {"two_solver_max_delta": float(delta.max()), "two_solver_converged": bool(second.converged),
+            "lobpcg_iterations": float(second.n_iterations), "lobpcg_eigenvalues": second.eigenvalues.tolist()}
+
+
 def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None, filter_dtype=torch.float64,
-                          filter_hook=None):
+                          filter_hook=None, cross_check: bool | None = None):
     """The SCF driver (reference ``scf/loop.py:255-460``). ``filter_dtype=float32`` runs the Chebyshev
     filter of every warm SCF pass in float32 (and the cold first solve until its residual is small),
     and the final diagonalisation in float64 with at least ``REFINEMENT_PASSES`` passes; Rayleigh-Ritz,
-    orthonormalisation, densities, Poisson, XC and energies stay float64 throughout."""
+    orthonormalisation, densities, Poisson, XC and energies stay float64 throughout. ``cross_check``
+    (default ``numerics.eigen.cross_check``) also solves the final operator by LOBPCG and by a second
+    CheFSI diagonalisation from the converged vectors, and reports their largest eigenvalue difference
+    in ``measurements`` (reference ``scf/loop.py:485-494``)."""
     started = time.perf_counter()
     store: dict = {}
     if geometry is None:
@@ -598,10 +614,19 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
     k = eigen.eigenvectors.shape[0]
     ne = k * k + 3
     terms = energies_from_host(k, occ, eigen.eigenvalues, host[:ne], host[ne:ne + 3], host[ne + 3:ne + 6], step.ion_ion)
+    vectors = eigen.eigenvectors
+    measurements: dict = {}
+    do_cross = getattr(numerics.eigen, "cross_check", False) if cross_check is None else cross_check
+    if do_cross:
+        vectors = vectors.clone()  # the cached solver's buffers are reused by the second diagonalisation
+        first = step.diagonalise(solver, h_final, initial=vectors, precision="f64")
+        measurements.update(two_solver_check(h_final, grid, numerics.eigen, step.n_states, numerics.seed,
+                                             first.eigenvalues))
     return {
         "energies": terms,
         "eigenvalues": eigen.eigenvalues,
-        "eigenvectors": eigen.eigenvectors,
+        "eigenvectors": vectors,
+        "measurements": measurements,
         "density": n_out,
         "scf_iterations": iteration,
         "converged": converged and eigen.converged,
@@ -620,9 +645,10 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
     }
 
 
-def solve_noninteracting(scenario, numerics, device, anchor=None, geometry=None):
+def solve_noninteracting(scenario, numerics, device, anchor=None, geometry=None, cross_check: bool | None = None):
     """Fixed-potential eigenproblem (reference ``scf/noninteracting.py:118-330``): CheFSI with the
-    weak-form Ritz stop, then ``T_s`` and ``E_ext = band - T``."""
+    weak-form Ritz stop, then ``T_s`` and ``E_ext = band - T``. ``cross_check`` (default
+    ``numerics.eigen.cross_check``) also solves the operator by LOBPCG (``noninteracting.py:182-189``)."""
     grid, factor, q = geometry if geometry is not None else build_geometry(scenario, numerics, device, anchor)
     h = CuspFactoredHamiltonian(grid, factor, q)
     n_states = max(1, int(-(-float(scenario.n_electrons) // 2.0)))
@@ -641,6 +667,9 @@ def solve_noninteracting(scenario, numerics, device, anchor=None, geometry=None)
     kinetic = float((occ[:k] * np.diag(form)).sum()) - escal[k * k] - escal[k * k + 1]
     band = float((occ * result.eigenvalues).sum())
     ion = nuclear_repulsion(scenario.charges, scenario.positions)
+    measurements: dict = {}
+    if getattr(numerics.eigen, "cross_check", False) if cross_check is None else cross_check:
+        measurements.update(two_solver_check(h, grid, numerics.eigen, n_states, numerics.seed, result.eigenvalues))
     return {"eigenvalues": result.eigenvalues, "eigenvectors": result.eigenvectors, "n_iterations": result.n_iterations,
             "converged": result.converged, "kinetic": kinetic, "band": band, "total": band + ion, "info": result.info,
-            "hamiltonian": h}
+            "hamiltonian": h, "measurements": measurements}

@@ -429,6 +429,26 @@ int cdft_poisson_kernel_f64(void* handle, double h, double radius, double* kerne
 int cdft_poisson_solve_f64(void* handle, const double* density, double* padded_in, void* slabs, void* spectrum,
                            double* padded_out, const double* kernel, double* potential, void* stream);
 
+/* ---- the LOBPCG cross-check's kinetic preconditioner (rocFFT through hipFFT) ---- */
+
+/* 3-D real-to-complex and complex-to-real plans of an (n0, n1, n2) box; *handle owns them. Replaces
+ * the lazily built spectrum of LOBPCG._apply_preconditioner (src/cdft/eigen/lobpcg.py:207-228). */
+int cdft_tpa_create(int n0, int n1, int n2, void** handle);
+int cdft_tpa_destroy(void* handle);
+
+/* The Teter-Payne-Allan filter K(x) = P / (P + 16 x^4), P = 27 + 18x + 12x^2 + 8x^3, x = (G^2/2) / E_ref,
+ * on the n0 x n1 x (n2/2+1) half spectrum, divided by n0 n1 n2 (so the inverse transform needs no
+ * normalisation); E_ref = max(mean of G^2/2 over the half spectrum, 1e-6), written to *e_ref when it
+ * is not NULL (src/cdft/eigen/lobpcg.py:219-236). */
+int cdft_tpa_filter_f64(void* handle, double h, double* filter, double* e_ref, void* stream);
+
+/* out[i] = f^-1 K (f in[i]) for the k rows of `in`, f = max(exp(-u), 1e-150) (the similarity frame,
+ * src/cdft/eigen/lobpcg.py:129-133,181-192); u = NULL applies K alone (the "transformed" frame).
+ * real_work: n0 n1 n2 doubles; spectrum: n0 n1 (n2/2+1) complex doubles; out may alias in. Replaces
+ * LOBPCG._apply_preconditioner (src/cdft/eigen/lobpcg.py:207-239). */
+int cdft_tpa_apply_f64(void* handle, const double* in, int k, const double* u, const double* filter,
+                       double* real_work, void* spectrum, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

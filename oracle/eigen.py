@@ -145,6 +145,7 @@ class EigenConfig:
     bound_safety: float = 1.02
     max_iterations: int = 300
     residual_tol: float = 1.0e-8
+    cross_check: bool = False
     cross_check_tol: float = 1.0e-8
 
 
@@ -249,3 +250,132 @@ def chefsi_solve(hamiltonian, grid, config: EigenConfig, n_states, initial=None,
         "history": history,
     }
     return EigenResult(evals[:n_states], vectors[:n_states], residual[:n_states], iteration, converged, info)
+
+
+# ------------------------------------------------------------------------------------------------
+# LOBPCG, the second eigensolver of the two-solver cross-check (reference ``eigen/lobpcg.py``)
+# ------------------------------------------------------------------------------------------------
+
+
+def tpa_filter(shape, spacing):
+    """The Teter-Payne-Allan kinetic filter on the real-FFT half spectrum of the box (reference
+    ``eigen/lobpcg.py:207-239``): ``K(x) = P(x) / (P(x) + 16 x^4)``, ``P = 27 + 18x + 12x^2 + 8x^3``,
+    ``x = (G^2/2) / E_ref`` with ``E_ref`` the mean of ``G^2/2`` over the half spectrum (at least 1e-6)."""
+    n0, n1, n2 = shape
+    two_pi = 2.0 * np.pi
+    kx = two_pi * np.fft.fftfreq(n0, d=spacing)
+    ky = two_pi * np.fft.fftfreq(n1, d=spacing)
+    kz = two_pi * np.fft.rfftfreq(n2, d=spacing)
+    kinetic = 0.5 * (kx[:, None, None] ** 2 + ky[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    reference = max(float(kinetic.mean()), 1.0e-6)
+    x = kinetic / reference
+    poly = 27.0 + 18.0 * x + 12.0 * x**2 + 8.0 * x**3
+    return poly / (poly + 16.0 * x**4)
+
+
+def tpa_precondition(block, grid, cusp_factor=None):
+    """The filter applied to each row of ``block`` by FFT over the box. With the cusp factor
+    ``f = max(exp(-u), 1e-150)`` it acts in the wavefunction frame: ``f^-1 K (f r)`` (reference
+    ``lobpcg.py:129-133``, the ``"similarity"`` frame)."""
+    flat = np.asarray(block, dtype=np.float64).reshape(-1, grid.n_points)
+    if cusp_factor is not None:
+        flat = flat * cusp_factor
+    box = flat.reshape(-1, *grid.shape)
+    spectrum = np.fft.rfftn(box, axes=(-3, -2, -1)) * tpa_filter(grid.shape, grid.spacing)
+    out = np.fft.irfftn(spectrum, s=grid.shape, axes=(-3, -2, -1)).reshape(-1, grid.n_points)
+    return out / cusp_factor if cusp_factor is not None else out
+
+
+def stable_basis(block, measure, threshold=1e-12):
+    """An orthonormal basis of ``block``'s span in the measure, directions with Gram eigenvalue at
+    most ``threshold`` times the largest dropped (reference ``lobpcg.py:254-271``); returns the basis
+    and the number dropped."""
+    flat = np.asarray(block).reshape(-1, measure.grid.n_points)
+    if flat.shape[0] == 0:
+        return flat, 0
+    gram = measure.gram(flat)
+    evals, evecs = np.linalg.eigh(0.5 * (gram + gram.T))
+    largest = evals[-1]
+    if largest <= 0.0:
+        return flat[:0], flat.shape[0]
+    keep = evals > largest * threshold
+    scaled = evecs[:, keep] / np.sqrt(evals[keep])
+    return scaled.T @ flat, int((~keep).sum())
+
+
+def project_out(block, basis, measure):
+    """``block`` with the span of the orthonormal ``basis`` removed by two classical Gram-Schmidt
+    passes, then made orthonormal itself (reference ``lobpcg.py:242-251``)."""
+    for _ in range(2):
+        block = block - measure.cross(block, basis) @ basis
+    return stable_basis(block, measure)[0]
+
+
+def lobpcg_solve(hamiltonian, grid, config: EigenConfig, n_states, initial=None, generator=None,
+                 frame="similarity"):
+    """Block LOBPCG in the operator's measure (reference ``eigen/lobpcg.py:71-165``).
+
+    Stops when every residual is below ``residual_tol``, or when the Ritz values move less than
+    ``0.1 cross_check_tol`` for two iterations in a row. Residuals at most ``0.1 residual_tol`` are
+    deflated out of the search directions. The preconditioner is the TPA filter in ``frame``
+    (``"similarity"``, ``"transformed"`` or ``"none"``)."""
+    measure = hamiltonian.measure
+    factor = getattr(hamiltonian, "factor", None)
+    cusp = None
+    if frame == "similarity" and factor is not None:
+        cusp = np.maximum(np.exp(-np.asarray(factor.u)), 1.0e-150)
+    if initial is None:
+        x = seeded_randn((n_states, grid.n_points), generator)
+    else:
+        x = np.asarray(initial, dtype=np.float64).reshape(-1, grid.n_points)[:n_states].copy()
+    x = orthonormalise(x, measure)
+    hx = hamiltonian.apply(x)
+    sub = measure.cross(x, hx)
+    evals, rotation = np.linalg.eigh(0.5 * (sub + sub.T))
+    x = rotation.T @ x
+    hx = rotation.T @ hx
+    p = None
+    converged = False
+    previous = evals.copy()
+    stable = 0
+    tol = 0.1 * config.cross_check_tol
+    history = []
+    residual = measure.norm(hx - evals[:, None] * x)
+    iteration = 0
+    for iteration in range(1, config.max_iterations + 1):
+        r = hx - evals[:, None] * x
+        residual = measure.norm(r)
+        keep = residual > 0.1 * config.residual_tol
+        worst = float(residual.max())
+        drift = float(np.abs(evals - previous).max()) if iteration > 1 else float("inf")
+        previous = evals.copy()
+        history.append((iteration, worst, drift))
+        if worst < config.residual_tol:
+            converged = True
+            break
+        stable = stable + 1 if drift < tol else 0
+        if stable >= 2:
+            converged = True
+            break
+        w = r[keep] if keep.any() else r
+        if cusp is not None:
+            w = tpa_precondition(w, grid, cusp)
+        elif frame == "transformed":
+            w = tpa_precondition(w, grid)
+        w = project_out(w, x, measure)
+        if w.shape[0] == 0:
+            converged = True
+            break
+        blocks = [x, w] if p is None else [x, w, p]
+        basis, _ = stable_basis(np.concatenate(blocks, axis=0), measure)
+        h_basis = hamiltonian.apply(basis)
+        sub = measure.cross(basis, h_basis)
+        sub_evals, sub_vectors = np.linalg.eigh(0.5 * (sub + sub.T))
+        coeff = sub_vectors[:, :n_states]
+        x_new = coeff.T @ basis
+        hx_new = coeff.T @ h_basis
+        p = x_new - measure.cross(x_new, x) @ x
+        x, hx = x_new, hx_new
+        evals = sub_evals[:n_states]
+    return EigenResult(evals[:n_states], x[:n_states], residual[:n_states], iteration, converged,
+                       {"history": history, "eigenvalue_drift": history[-1][2] if history else float("inf")})

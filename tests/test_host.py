@@ -192,3 +192,9 @@ def test_dropped_graphs_wait_for_a_capture_free_moment(pkg):
     t.join(5)
     waiter.join(5)
     assert destroyed == ["a"] and not ops._released
+    # deferred device-side releases run at the same moment, in the lock
+    calls = []
+    ops.release_later(lambda: calls.append("plan"))
+    assert calls == []
+    ops.release_graphs()
+    assert calls == ["plan"] and not ops._released

@@ -229,3 +229,68 @@ def test_pbe_derivatives_of_the_oracle_match_differences():
     fdn = (op.pbe_xc(n * (1 + h), sig)[0] - op.pbe_xc(n * (1 - h), sig)[0]) / (2 * h * n)
     assert (np.abs(fds - vs) * sig / np.abs(e)).max() < 1e-9
     assert (np.abs(fdn - vn) * n / np.abs(e)).max() < 1e-9
+
+
+def test_tpa_filter_on_a_plane_wave():
+    """One Fourier mode of the box is an eigenvector of the preconditioner with eigenvalue
+    ``K(x) = P/(P + 16 x^4)``, ``x = (G^2/2)/E_ref``; ``K(0) = 1`` and K falls as ``27/(16 x^4)``."""
+    from oracle.eigen import tpa_filter, tpa_precondition
+
+    grid = Grid.from_box((7.0, 9.0, 5.0), 0.5)  # 15 x 19 x 11: odd and mixed sizes
+    n0, n1, n2 = grid.shape
+    i, j, k = np.meshgrid(np.arange(n0), np.arange(n1), np.arange(n2), indexing="ij")
+    m = (2, 3, 1)
+    wave = np.cos(2 * np.pi * (m[0] * i / n0 + m[1] * j / n1 + m[2] * k / n2)).reshape(1, -1)
+    g2 = sum((2 * np.pi * mm / (n * grid.spacing)) ** 2 for mm, n in zip(m, grid.shape))
+    # E_ref: the mean of G^2/2 over the half spectrum, from the frequencies' closed forms
+    f0 = np.fft.fftfreq(n0, grid.spacing)
+    f1 = np.fft.fftfreq(n1, grid.spacing)
+    f2 = np.arange(n2 // 2 + 1) / (n2 * grid.spacing)
+    e_ref = 0.5 * (2 * np.pi) ** 2 * ((f0**2).mean() + (f1**2).mean() + (f2**2).mean())
+    x = 0.5 * g2 / e_ref
+    poly = 27 + 18 * x + 12 * x**2 + 8 * x**3
+    out = tpa_precondition(wave, grid)
+    assert np.abs(out - wave * poly / (poly + 16 * x**4)).max() < 1e-13
+    k_table = tpa_filter(grid.shape, grid.spacing)
+    assert k_table[0, 0, 0] == 1.0 and np.all(np.diff(k_table[0, 0, :]) < 0)
+
+
+def test_similarity_frame_preconditioner_is_symmetric_in_the_weighted_measure():
+    """The reference's check (``tests/test_cusp_factorisation.py:241-262``): ``<a, f^-1 K f b>_{f^2}``
+    is symmetric, the plain frame ``<a, K b>_{f^2}`` is not."""
+    from oracle.eigen import tpa_precondition
+
+    grid = Grid.from_box((8.0, 8.0, 8.0), 0.5)
+    factor = CuspFactor(grid, (1.0,), [[0.0, 0.0, 0.0]])
+    f = np.maximum(np.exp(-factor.u), 1e-150)
+    rng = np.random.default_rng(11)
+    a, b = rng.standard_normal((2, 1, grid.n_points))
+    w = factor.weight
+    left = float((w * a * tpa_precondition(b, grid, f)).sum())
+    right = float((w * b * tpa_precondition(a, grid, f)).sum())
+    assert abs(left - right) < 1e-10 * max(1.0, abs(left))
+    plain_left = float((w * a * tpa_precondition(b, grid)).sum())
+    plain_right = float((w * b * tpa_precondition(a, grid)).sum())
+    assert abs(plain_left - plain_right) > 1e-6 * max(1.0, abs(plain_left))
+
+
+def test_lobpcg_agrees_with_chefsi_on_bare_hydrogen():
+    """The two-solver check (G2.5, ``contract.py:436-449``: |Δε| < ``cross_check_tol`` = 1e-8) on the
+    bare-H golden case (33^3, h = 1), LOBPCG seeded with ``seed + 1`` as the reference's SCF does
+    (``scf/loop.py:485-494``); and the preconditioner frames rank as the reference's decision D-56
+    records (similarity fewest iterations, then none, then transformed)."""
+    from oracle import physics as ophys
+    from oracle.eigen import lobpcg_solve, make_generator
+
+    sc = ophys.Scenario((1.0,), ((0.0, 0.0, 0.0),), 1.0)
+    num = ophys.Numerics(box_lengths=(32.0,) * 3, spacing=1.0)
+    geometry = ophys.build_geometry(sc, num)
+    grid, factor, q = geometry
+    h = CuspHamiltonian(grid, factor, quadrature=q)
+    iterations = {}
+    for frame in ("similarity", "none", "transformed"):
+        r = lobpcg_solve(h, grid, num.eigen, 1, generator=make_generator(num.seed + 1), frame=frame)
+        assert r.converged
+        assert abs(r.eigenvalues[0] - GOLDEN["eigenvalue.ground.h_atom"]) < 1e-8
+        iterations[frame] = r.n_iterations
+    assert iterations["similarity"] < iterations["none"] < iterations["transformed"]
