@@ -294,3 +294,26 @@ def test_lobpcg_agrees_with_chefsi_on_bare_hydrogen():
         assert abs(r.eigenvalues[0] - GOLDEN["eigenvalue.ground.h_atom"]) < 1e-8
         iterations[frame] = r.n_iterations
     assert iterations["similarity"] < iterations["none"] < iterations["transformed"]
+
+
+def test_rank_deficient_block_falls_back_to_eigh():
+    """``orthonormalise`` (reference ``eigen/rayleigh_ritz.py:51-72``) on a block whose rows repeat:
+    the Cholesky fails, the eigh fallback keeps the independent directions (eigenvalues above 1e-14 of
+    the largest) and returns fewer vectors, orthonormal in the measure and spanning the block; the
+    Rayleigh-Ritz values of that span are those of the distinct rows."""
+    from oracle.eigen import orthonormalise, rayleigh_ritz
+
+    grid, h = _small_operator((1.0,))
+    rng = np.random.default_rng(2)
+    rows = rng.standard_normal((3, grid.n_points))
+    block = np.concatenate([rows, rows[:1], 2.0 * rows[1:2]])  # rank 3 in 5 rows
+    basis = orthonormalise(block, h.measure)
+    assert basis.shape[0] == 3
+    assert np.abs(h.measure.gram(basis) - np.eye(3)).max() < 1e-12
+    # the span is the rows' span: projecting the rows on the basis loses nothing
+    proj = h.measure.cross(rows, basis) @ basis
+    assert np.abs(proj - rows).max() < 1e-10 * np.abs(rows).max()
+    evals, _, orth = rayleigh_ritz(h, block, h.measure)
+    evals_rows, _, _ = rayleigh_ritz(h, rows, h.measure)
+    assert evals.shape == (3,) and orth < 1e-12
+    assert np.allclose(evals, evals_rows, rtol=1e-10, atol=1e-12)
