@@ -28,10 +28,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the fused step on this workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
-# `bench.py --kernel-only`, each scaled by the same counter on a 1 GiB copy in the same run (summary and
-# commands: profiles/hdiv_fused_step_he65_k5.md). Re-measured whenever the stencil kernel changes.
-FUSED_STEP_TRAFFIC_BYTES = (27961 / 0.500 + 11462 / 1.000) * 1024
+# Bytes between L2 and memory per launch of the fused step on this workload, from the rocprofv3
+# TCC_EA0_RDREQ_{32B,64B,128B} / TCC_EA0_WRREQ{,_64B} request counts of `bench.py --kernel-only`, all
+# L2 channels summed (the 1 GiB copy of the same run reads and writes exactly 1 GiB by the same
+# formula; summary and commands: profiles/hdiv_fused_step_he65_k5.md). Re-measured whenever the
+# stencil kernel changes.
+FUSED_STEP_TRAFFIC_BYTES = (444268 * 128 + 2095 * 64) + (179896 * 64 + (186905 - 179896) * 32)
 METRIC = "He-LDA SCF s/solve & H-apply GB/s vs HBM peak; mixed vs FP64 at <=1e-8 Ha"
 PUBLISHED_CPU_S_PER_SOLVE = 109.5  # reference, He LDA 65^3, CPU float64 2 threads (convergence_ladders.json)
 
@@ -192,6 +194,23 @@ def kernel_only(pkg, device, k=5, filters=3):
     bytes_step = (19 * stencil_bytes(grid.shape, k, 8, 8) + stencil_bytes(grid.shape, k, 8, 8, first=True)) / 20
     return {"kernel_only": True, "grid": list(grid.shape), "k": k, "us_per_step": us, "algorithmic_bytes_per_step": bytes_step,
             "GB_s": bytes_step / us / 1e3, "copy_bytes": 2**30, "points": n}
+
+
+def apply_only(pkg, device, k=8, reps=60):
+    """The k = 8 apply microbench alone (cache-defeating rotation of input and output blocks) on the bare
+    He 65^3 geometry, for counter runs, then a 1 GiB copy of known size."""
+    import torch
+    from cdft_amd import scf
+    from cdft_amd.hamiltonian import CuspFactoredHamiltonian
+
+    grid, factor, quad = scf.build_geometry(he_lda_scenario(), numerics(), device)
+    geom = CuspFactoredHamiltonian(grid, factor, quad).geometry
+    result = h_apply_microbench(pkg, device, (geom, grid.shape), k=k, reps=reps)
+    a = torch.ones(2**30 // 8, dtype=torch.float64, device=device)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    return {"apply_only": True, **result, "copy_bytes": 2**30}
 
 
 def compare_precisions(device, scenario, numerics_, anchor=None, reference=None):
@@ -393,6 +412,8 @@ def main(argv=None):
     ap.add_argument("--kernel-only", action="store_true",
                     help="run only the fused Chebyshev step on the He 65^3 geometry (k=5, three degree-20 filters) "
                          "and a 1 GiB device copy that calibrates the HBM byte counters; for rocprofv3 --pmc runs")
+    ap.add_argument("--apply-only", action="store_true",
+                    help="run only the k=8 apply microbench on the He 65^3 geometry and a 1 GiB copy; for rocprofv3 runs")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the Kohn-Sham path here runs on a "
@@ -408,6 +429,9 @@ def main(argv=None):
     device = torch.device("cuda:0")
     if args.kernel_only:
         print(json.dumps(kernel_only(pkg, device)), flush=True)
+        return 0
+    if args.apply_only:
+        print(json.dumps(apply_only(pkg, device)), flush=True)
         return 0
     for _ in range(args.warmup):
         run_solve(pkg, device, torch.float64)
