@@ -367,7 +367,7 @@ def batched_scan(device, count=64, seed=0, spacing=0.25):
             "speedup_batched_vs_sequential": t_solo / t_batch, "solves_per_s_batched": count / t_batch,
             "lockstep_iterations": solver.iterations, "blocks_per_filter_launch": solver.blocks_per_launch,
             "batched_phases_s": solver.timings,
-            "autotuned_segment": solver.tuning,
+            "autotuned_launch": solver.tuning,
             "sequential_iterations_total": int(sum(s["n_iterations"] for s in solo)),
             "graph": solver.graph_record,
             "max_abs_eigenvalue_diff_vs_sequential": max(diffs),
@@ -412,6 +412,8 @@ def main(argv=None):
     ap.add_argument("--kernel-only", action="store_true",
                     help="run only the fused Chebyshev step on the He 65^3 geometry (k=5, three degree-20 filters) "
                          "and a 1 GiB device copy that calibrates the HBM byte counters; for rocprofv3 --pmc runs")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the 64-geometry batched H2+ scan and print its record; for rocprofv3 runs")
     ap.add_argument("--apply-only", action="store_true",
                     help="run only the k=8 apply microbench on the He 65^3 geometry and a 1 GiB copy; for rocprofv3 runs")
     args = ap.parse_args(argv)
@@ -432,6 +434,9 @@ def main(argv=None):
         return 0
     if args.apply_only:
         print(json.dumps(apply_only(pkg, device)), flush=True)
+        return 0
+    if args.batch_only:
+        print(json.dumps(batched_scan(device)), flush=True)
         return 0
     for _ in range(args.warmup):
         run_solve(pkg, device, torch.float64)

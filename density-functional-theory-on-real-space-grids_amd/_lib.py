@@ -40,6 +40,7 @@ SIGNATURES: dict[str, list] = {
     "cdft_cheb_filter_batched_f32": [P, I, P, I, I, P],
     "cdft_hdiv_apply_batched_f32": [P, I, P, I, P],
     "cdft_hdiv_set_segment": [I],
+    "cdft_hdiv_set_tile": [I, I],
     "cdft_hdiv_launch_shape": [I, I, I, I, I, P],
     "cdft_wgram_f64": [P, I, P, I, P, L, D, I, P, L, P, P],
     "cdft_wgram_noalloc_f64": [P, I, P, I, P, L, D, I, P, P],

@@ -222,29 +222,29 @@ class BatchedCheFSI:
         self.timings["start_lanczos_s"] = t_lanczos
         geoms = [(it.h.geometry, it.work) for it in insts]
         filter_rows = [(g, w.vectors, w.filtered, w.work1, w.work2, w.table) for g, w in geoms]
-        segment = 0
+        from . import autotune
+
+        tuning = None
         if self.autotune:
             # The tables are still zero here, so the timed filters only overwrite scratch blocks.
-            from . import autotune
-
             signature = ("batched-filter", tuple(it.h.grid.shape for it in insts), k, degree)
-            self.tuning = autotune.tune(signature, lambda: BatchPlan(filter_rows, k).filter(degree, insts[0].work.vectors))
-            segment = self.tuning["best"]
-        ops.hdiv_set_segment(segment)
+            self.tuning = tuning = autotune.tune(signature,
+                                                 lambda: BatchPlan(filter_rows, k).filter(degree, insts[0].work.vectors))
+        autotune.apply(tuning)
         try:
             plan_filter = BatchPlan(filter_rows, k)
         finally:
-            ops.hdiv_set_segment(0)
+            autotune.apply(None)
         plan_rr = BatchPlan([(g, w.basis, w.applied, w.work1, w.work2, w.table) for g, w in geoms], k)
         plan_res = BatchPlan([(g, w.vectors, w.applied, w.work1, w.work2, w.table) for g, w in geoms], k)
         plan32 = None
         if self.mixed:
-            ops.hdiv_set_segment(segment)
+            autotune.apply(tuning)
             try:
                 plan32 = BatchPlan([(it.h.geometry_in(self.filter_dtype), it.work.x32, it.work.y32, it.work.a32,
                                      it.work.b32, it.work.table) for it in insts], k)
             finally:
-                ops.hdiv_set_segment(0)
+                autotune.apply(None)
         self.blocks_per_launch = plan_filter.blocks
         graphs: dict = {}
         stalled32 = False

@@ -552,14 +552,18 @@ static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 // Segment length override (0 = automatic); set by cdft_hdiv_set_segment.
 static int g_segment = 0;
+// Tile extent overrides (0 = automatic): the largest tile extent along y (1..TMAX; the tiles then
+// split n1 evenly) and the tile width along z (13 or 16); set by cdft_hdiv_set_tile. No tiling
+// changes an output bit: every point's sums take the same operands in the same order.
+static int g_tile_y = 0, g_tile_z = 0;
 static March march_dims(int n0, int n1, int n2, int k, int reach) {
   March d;
   // z tiles are 13 or 16 wide (the kernel's LDS pitches are built for those); take the one that
   // leaves fewer idle columns (65 = 5 x 13, 64 = 4 x 16).
   const int idle13 = ceil_div(n2, 13) * 13 - n2, idle16 = ceil_div(n2, 16) * 16 - n2;
-  d.tz = idle13 < idle16 ? 13 : 16;
+  d.tz = g_tile_z ? g_tile_z : (idle13 < idle16 ? 13 : 16);
   d.tiles_z = ceil_div(n2, d.tz);
-  d.tiles_y = ceil_div(n1, TMAX);
+  d.tiles_y = ceil_div(n1, g_tile_y ? g_tile_y : TMAX);
   d.ty = ceil_div(n1, d.tiles_y);
   const int per_plane = d.tiles_y * d.tiles_z;
   if (g_segment > 0) {
@@ -882,6 +886,15 @@ extern "C" int cdft_hdiv_pack_f32(const float* w0, const float* w1, const float*
 extern "C" int cdft_hdiv_set_segment(int planes) {
   if (planes < 0) return fail(CDFT_E_ARG, "hdiv_set_segment: planes must be >= 0 (0 = automatic)");
   g_segment = planes;
+  return CDFT_OK;
+}
+
+extern "C" int cdft_hdiv_set_tile(int tile_y, int tile_z) {
+  if (tile_y < 0 || tile_y > TMAX) return fail(CDFT_E_ARG, "hdiv_set_tile: tile_y must be 0 (automatic) or 1..%d", TMAX);
+  if (tile_z != 0 && tile_z != 13 && tile_z != 16)
+    return fail(CDFT_E_ARG, "hdiv_set_tile: tile_z must be 0 (automatic), 13 or 16");
+  g_tile_y = tile_y;
+  g_tile_z = tile_z;
   return CDFT_OK;
 }
 

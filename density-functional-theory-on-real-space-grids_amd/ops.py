@@ -283,6 +283,12 @@ def hdiv_set_segment(planes: int) -> None:
     _lib.call("cdft_hdiv_set_segment", int(planes))
 
 
+def hdiv_set_tile(tile_y: int = 0, tile_z: int = 0) -> None:
+    """Tile extents of the marching stencil kernels: the largest along y (1..16) and the width along
+    z (13 or 16); 0 restores the automatic choice. Outputs do not depend on them."""
+    _lib.call("cdft_hdiv_set_tile", int(tile_y), int(tile_z))
+
+
 def hdiv_launch_shape(shape, k: int, order: int = 8) -> dict:
     """The launch the stencil kernels use for a ``(k, *shape)`` block (host-side, no device work)."""
     buf = (ctypes.c_int * 6)()

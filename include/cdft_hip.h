@@ -220,6 +220,13 @@ int cdft_cheb_table_f64(const double* evals, int index, const double* bounds, in
 
 /* Planes per x-segment of the marching stencil kernels (0 = automatic choice). Process-wide. */
 int cdft_hdiv_set_segment(int planes);
+
+/* Tile extents of the marching stencil kernels for later launches and batch plans (0 = automatic):
+ * the largest tile extent along y, 1..16 (the tiles then split n1 evenly), and the tile width along z,
+ * 13 or 16. Outputs are bitwise independent of the tiling (and of the segment length); a launch
+ * tuning knob with no counterpart in the reference, whose stencil is torch slicing
+ * (src/cdft/operators/divergence.py:84-197). Process-wide, like cdft_hdiv_set_segment. */
+int cdft_hdiv_set_tile(int tile_y, int tile_z);
 /* Host-only: the launch the stencil kernels use for a (k, n0, n1, n2) block of the given order:
  * shape6 (host) = {tile_y, tile_z, longest segment, segments, threads per block, blocks}. */
 int cdft_hdiv_launch_shape(int n0, int n1, int n2, int k, int order, int* shape6);

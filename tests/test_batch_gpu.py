@@ -114,3 +114,25 @@ def test_mixed_batched_chefsi_matches_float64_batch(pkg, device):
     for a, b in zip(mixed, f64):
         assert a["converged"]
         assert abs(a["eigenvalues"][0] - b["eigenvalues"][0]) <= 1e-12 * abs(b["eigenvalues"][0])
+
+
+def test_autotuned_launch_leaves_the_eigenvalues_bitwise_unchanged(pkg, device):
+    """The batch autotuner (segment length, then tile extents) changes only the launch: the tuned
+    batched solve gives bitwise the eigenvalues and iteration counts of the untuned one."""
+    from cdft_amd import autotune
+    from cdft_amd.batch import BatchedCheFSI, bond_scan_geometries
+    from cdft_amd.contract import Numerics
+    from cdft_amd.eigen import EigenConfig
+    from cdft_amd.hamiltonian import CuspFactoredHamiltonian
+    from cdft_amd.scf import build_geometry
+
+    scan = bond_scan_geometries(count=3, seed=5, spacing=0.5)
+    hams = [CuspFactoredHamiltonian(*build_geometry(sc, Numerics(box_lengths=box, spacing=0.5), device))
+            for _, sc, box in scan]
+    tuned_solver = BatchedCheFSI(hams, EigenConfig(), n_states=1, autotune=True)
+    tuned = tuned_solver.solve()
+    plain = BatchedCheFSI(hams, EigenConfig(), n_states=1, autotune=False).solve()
+    assert "tile" in tuned_solver.tuning and len(tuned_solver.tuning["tile_us"]) == len(autotune.TILES)
+    for a, b in zip(tuned, plain):
+        assert a["n_iterations"] == b["n_iterations"]
+        assert np.array_equal(a["eigenvalues"], b["eigenvalues"])

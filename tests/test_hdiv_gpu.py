@@ -197,3 +197,48 @@ def test_geometry_records_hold_the_separate_arrays(pkg, device, dtype):
     expect[:, :n1, :, 2] = w2
     expect[:, :n1, :n2, 3] = iw
     assert np.array_equal(rec, expect)
+
+
+def test_outputs_do_not_depend_on_the_tiling(pkg, device):
+    """Every tiling (largest tile rows along y 1..16, width 13 or 16 along z) and segment length gives
+    bitwise the same apply, transpose and Chebyshev steps (each point's sums take the same operands
+    in the same order); the residual norms, summed over per-block partials, agree to round-off."""
+    from cdft_amd.chefsi import scaled_chebyshev_table
+
+    ops = pkg.ops
+    for shape, k in (((61, 57, 57), 2), ((20, 23, 31), 3), ((33, 33, 33), 5)):
+        grid, ref = _case(shape, 0.4, (1.0, 1.0), [[-0.7, 0.05, 0.0], [0.7, 0.05, 0.0]])
+        geom = pkg.geometry_from_arrays(grid.shape, 8, grid.spacing, ref.face, ref.inverse_weight, ref.potential,
+                                        device)
+        gen = torch.Generator().manual_seed(shape[0])
+        x = (torch.rand((k, grid.n_points), generator=gen, dtype=torch.float64) * 2 - 1).to(device)
+        p = (torch.rand((k, grid.n_points), generator=gen, dtype=torch.float64) * 2 - 1).to(device)
+        q = torch.rand(grid.n_points, generator=gen, dtype=torch.float64).to(device)
+        evals = torch.linspace(-1.0, 1.0, k, dtype=torch.float64, device=device)
+        table = torch.tensor(scaled_chebyshev_table(3, -0.3, 30.0, -1.2), dtype=torch.float64, device=device)
+
+        def run():
+            norms = torch.empty(k, dtype=torch.float64, device=device)
+            ops.hdiv_resnorm(x, geom, evals, q, norms)
+            return (ops.hdiv_apply(x, geom), ops.hdiv_apply_adjoint(x, geom),
+                    ops.cheb_step(x, None, geom, table, 0, torch.empty_like(x)),
+                    ops.cheb_step(x, p, geom, table, 1, torch.empty_like(x)), norms)
+
+        base = run()
+        try:
+            for tile, segment in (((16, 13), 0), ((13, 13), 5), ((8, 16), 0), ((3, 13), 4), ((1, 16), 11)):
+                ops.hdiv_set_tile(*tile)
+                ops.hdiv_set_segment(segment)
+                shape_used = ops.hdiv_launch_shape(grid.shape, k)
+                assert shape_used["tile_z"] == tile[1] and shape_used["tile_y"] <= tile[0]
+                got = run()
+                for a, b in zip(got[:4], base[:4]):
+                    assert torch.equal(a, b), (shape, tile, segment)
+                assert torch.allclose(got[4], base[4], rtol=1e-13, atol=0), (shape, tile, segment)
+        finally:
+            ops.hdiv_set_tile(0, 0)
+            ops.hdiv_set_segment(0)
+    with pytest.raises(Exception):
+        ops.hdiv_set_tile(17, 0)
+    with pytest.raises(Exception):
+        ops.hdiv_set_tile(0, 14)
