@@ -16,6 +16,8 @@
 // loads of plane m+1 are issued before the LDS work of plane m, so their latency overlaps it.
 // Tile extents are chosen per grid (ceil(n / ceil(n / 16))), so n = 65 runs 13-wide tiles rather
 // than four full tiles and a one-column tail.
+#include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 #include "common.h"
@@ -556,6 +558,56 @@ static int g_segment = 0;
 // split n1 evenly) and the tile width along z (13 or 16); set by cdft_hdiv_set_tile. No tiling
 // changes an output bit: every point's sums take the same operands in the same order.
 static int g_tile_y = 0, g_tile_z = 0;
+// Compute units of the current device, read once per device: the unit of the fill rule below.
+// 256 (MI355X) when no device is visible, as for the host-only launch-shape query.
+static int device_cus() {
+  static std::atomic<int> cached[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    (void)hipGetLastError();  // leave no error behind for the next launch check
+    return 256;
+  }
+  int cus = cached[dev].load(std::memory_order_relaxed);
+  if (cus > 0) return cus;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) {
+    (void)hipGetLastError();
+    cus = 256;
+  }
+  cached[dev].store(cus, std::memory_order_relaxed);
+  return cus;
+}
+
+// Segment count of the fill rule. The dispatcher deals a launch's blocks round-robin over the CUs
+// and, up to 16 waves per CU (four per SIMD, the order-8 float64 kernels' occupancy), runs them all
+// at once; the launch ends when the most loaded CU ends. Per-block timestamps show a block's life
+// growing with the blocks beside it on its CU (65^3, k = 5: 14.1 us beside two others, 16.4 us
+// beside three; 81^3: 28.5 us at four per CU, 32.1 us at five), so a block count just above a
+// multiple of the CU count is the worst case: at 81^3, k = 5, 1,050 blocks (26 CUs with five)
+// take 39.7 us and 1,260 blocks take 35.6 us. For each fill b = 1..5 blocks per CU the rule takes
+// the most segments that fit (segments of at least 4 planes) and scores them by (4 + planes per
+// segment) x g(b): 4 planes stand for the segment's x warm-up, and g is the relative block life
+// at b per CU fitted to launch sweeps at 65^3 (k = 5, 8), 81^3 and 101 x 81 x 81 (float64 and
+// float32, tilings 8-16 x 13-16, 1-16 segments). The choice is the best or within 3% of the best
+// segment count measured on each of those shapes and on 57^3.
+static int fill_segments(int n0, long columns, int threads) {
+  static const double g[6] = {0.0, 0.57, 0.68, 0.90, 1.0, 1.29};
+  const long cus = device_cus();
+  const int cap = std::min(5, std::max(1, 16 / std::max(1, threads / 64)));
+  const int most = ceil_div(n0, 4);
+  int best = 1;
+  double best_cost = 1e300;
+  for (int b = 1; b <= cap; ++b) {
+    const int nseg = (int)std::max(1L, std::min<long>((long)b * cus / columns, most));
+    const long per_cu = (columns * nseg + cus - 1) / cus;  // the most loaded CU
+    const double cost = (4.0 + ceil_div(n0, nseg)) * g[std::min<long>(std::max(per_cu, 1L), 5)];
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = nseg;
+    }
+  }
+  return best;
+}
+
 static March march_dims(int n0, int n1, int n2, int k, int reach) {
   March d;
   // z tiles are 13 or 16 wide (the kernel's LDS pitches are built for those); take the one that
@@ -565,28 +617,16 @@ static March march_dims(int n0, int n1, int n2, int k, int reach) {
   d.tiles_z = ceil_div(n2, d.tz);
   d.tiles_y = ceil_div(n1, g_tile_y ? g_tile_y : TMAX);
   d.ty = ceil_div(n1, d.tiles_y);
-  const int per_plane = d.tiles_y * d.tiles_z;
-  if (g_segment > 0) {
-    d.nseg = ceil_div(n0, g_segment);
-  } else {
-    // About 1100 blocks (four or more per CU), but segments of at least 8 planes so the 4P-2
-    // plane warm-up stays a minor part of the reads. Measured on 65^3 (k = 5, 8) and 101 x 81 x 81
-    // (k = 5): the best segment counts were 8, 5 and 5-6.
-    // A launch that would still hold fewer than two blocks per CU (small grids at k = 1 or 2) takes
-    // segments down to 4 planes instead: k = 1 on 65^3 ran 8.5 us with 17 segments against 10.9 us
-    // with 8 (the outputs do not depend on the segmentation).
-    d.nseg = (1100 + per_plane * k / 2) / (per_plane * k);
-    const int wide = n0 / 8;
-    if (d.nseg > wide) d.nseg = (long)per_plane * k * wide < 512 ? std::min(d.nseg, ceil_div(n0, 4)) : wide;
-  }
-  if (d.nseg < 1) d.nseg = 1;
-  if (d.nseg > n0) d.nseg = n0;
-  d.seg = ceil_div(n0, d.nseg);
-  d.tiles = (unsigned)per_plane * d.nseg;
   const int cover = d.ty * d.tz;
   const int halo = ceil_div(2 * reach * (d.ty + d.tz), MAXH);  // every halo value has a slot
   int th = cover > halo ? cover : halo;
   d.threads = ceil_div(th, 64) * 64;
+  const int per_plane = d.tiles_y * d.tiles_z;
+  d.nseg = g_segment > 0 ? ceil_div(n0, g_segment) : fill_segments(n0, (long)per_plane * k, d.threads);
+  if (d.nseg < 1) d.nseg = 1;
+  if (d.nseg > n0) d.nseg = n0;
+  d.seg = ceil_div(n0, d.nseg);
+  d.tiles = (unsigned)per_plane * d.nseg;
   return d;
 }
 

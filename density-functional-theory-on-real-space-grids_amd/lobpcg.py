@@ -70,10 +70,10 @@ class KineticPreconditioner:
                   self._real.data_ptr(), self._spectrum.data_ptr(), out.data_ptr(), ops.stream_of(rows))
         return out
 
-    def __del__(self):
+    def __del__(self, _release_later=ops.release_later, _lib=_lib):
         handle = getattr(self, "_handle", None)
         if handle is not None and handle.value:
-            ops.release_later(lambda h=handle: _lib.call("cdft_tpa_destroy", h))
+            _release_later(lambda h=handle: _lib.call("cdft_tpa_destroy", h))
             self._handle = None
 
 

@@ -63,9 +63,10 @@ _graph_lock = threading.RLock()
 _released: list = []
 
 
-def release_later(release) -> None:
+def release_later(release, _released=_released) -> None:
     """Run ``release()`` (a device-side free) at the next capture-free moment (:func:`capture_graph`
-    or :func:`release_graphs`) instead of now. Safe to call from a finaliser."""
+    or :func:`release_graphs`) instead of now. Safe to call from a finaliser, also while the
+    interpreter shuts down (the list is bound at definition, not looked up in the module)."""
     _released.append(release)
 
 

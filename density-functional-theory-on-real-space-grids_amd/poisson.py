@@ -50,11 +50,12 @@ class _Plan:
         self.kernel = torch.empty((p0, p1, p2 // 2 + 1), dtype=torch.float64, device=device)
         _lib.call("cdft_poisson_kernel_f64", handle, spacing, radius, self.kernel.data_ptr(), stream_of(self.kernel))
 
-    def __del__(self):
+    def __del__(self, _release_later=ops.release_later, _lib=_lib):
         handle = getattr(self, "handle", None)
         if handle is not None and handle.value:
             # destroying the plans frees device memory: deferred to a moment no capture is recorded
-            ops.release_later(lambda h=handle: _lib.call("cdft_poisson_destroy", h))
+            # (the helpers are bound as defaults: module globals are gone at interpreter shutdown)
+            _release_later(lambda h=handle: _lib.call("cdft_poisson_destroy", h))
             self.handle = None
 
 

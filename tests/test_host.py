@@ -198,3 +198,12 @@ def test_dropped_graphs_wait_for_a_capture_free_moment(pkg):
     assert calls == []
     ops.release_graphs()
     assert calls == ["plan"] and not ops._released
+    # finalisers run during interpreter shutdown, when module globals are already None
+    queue = ops._released
+    try:
+        ops._released = None
+        ops.release_later(lambda: calls.append("late"))
+    finally:
+        ops._released = queue
+    ops.release_graphs()
+    assert calls == ["plan", "late"] and not ops._released
