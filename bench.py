@@ -96,6 +96,7 @@ def h_apply_microbench(pkg, device, geometry_arrays, k=8, reps=200):
     n = int(np.prod(shape))
     s = 8
     nbuf = max(2, int(math.ceil(4 * 256e6 / (2 * k * n * s))))
+    tuning = autotune_launch(geom, k)
     gen = torch.Generator().manual_seed(0)
     xs = [(torch.rand((k, n), generator=gen, dtype=torch.float64) * 2 - 1).to(device) for _ in range(nbuf)]
     ys = [torch.empty_like(xs[0]) for _ in range(nbuf)]
@@ -112,7 +113,20 @@ def h_apply_microbench(pkg, device, geometry_arrays, k=8, reps=200):
     b = 2 * k * n * s + s * ((shape[0] + 8) * shape[1] * shape[2] + shape[0] * (shape[1] + 8) * shape[2]
                              + shape[0] * shape[1] * (shape[2] + 8) + 2 * n)
     return {"k": k, "grid": list(shape), "dtype": "f64", "us_per_apply": t * 1e6, "bytes_per_apply": b,
-            "GB_s": b / t / 1e9, "frac_of_hbm_peak": b / t / 1e9 / HBM_PEAK_GBS, "buffers_rotated": nbuf}
+            "GB_s": b / t / 1e9, "frac_of_hbm_peak": b / t / 1e9 / HBM_PEAK_GBS, "buffers_rotated": nbuf,
+            "launch": tuning}
+
+
+def autotune_launch(geom, k):
+    """Put the tuned vector-kernel launch for ``(geom.shape, k)`` in force (the sweep the solvers run
+    on a shape's second solve) and return it with the launch shape it gives."""
+    from cdft_amd import autotune, ops
+
+    tuning = autotune.tune_stencil(geom, k)
+    shape = ops.hdiv_vec_shape(geom.shape, k, geom.dtype)
+    return {"setting": None if tuning is None or tuning["best"] is None else list(tuning["best"]),
+            "tile": [shape["tile_y"], shape["tile_z"]], "segment": shape["segment"], "blocks": shape["blocks"],
+            "threads": shape["threads"], "candidates_us": None if tuning is None else tuning["us"]}
 
 
 def cpu_model() -> str:
@@ -177,6 +191,7 @@ def kernel_only(pkg, device, k=5, filters=3):
     x = (torch.rand((k, grid.n_points), generator=gen, dtype=torch.float64) * 2 - 1).to(device)
     table = torch.tensor(chebyshev_coefficients(20, -1.0, 40.0, -1.5), dtype=torch.float64, device=device)
     out, w1, w2 = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
+    launch = autotune_launch(geom, k)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     pkg.ops.cheb_filter(x, geom, table, 20, out=out, work=(w1, w2))
     torch.cuda.synchronize()
@@ -193,7 +208,8 @@ def kernel_only(pkg, device, k=5, filters=3):
     n = grid.n_points
     bytes_step = (19 * stencil_bytes(grid.shape, k, 8, 8) + stencil_bytes(grid.shape, k, 8, 8, first=True)) / 20
     return {"kernel_only": True, "grid": list(grid.shape), "k": k, "us_per_step": us, "algorithmic_bytes_per_step": bytes_step,
-            "GB_s": bytes_step / us / 1e3, "copy_bytes": 2**30, "points": n}
+            "GB_s": bytes_step / us / 1e3, "copy_bytes": 2**30, "points": n, "launch": launch,
+            "timed_dispatches": 20 * filters}
 
 
 def apply_only(pkg, device, k=8, reps=60):
@@ -454,6 +470,7 @@ def main(argv=None):
     achieved = bytes_per_launch / avg_launch / 1e9
     step = r["step"]
     shape = step.grid.shape
+    vec = pkg.ops.hdiv_vec_shape(shape, 5, torch.float64)  # the fused step's launch during the timed solves
     micro = h_apply_microbench(pkg, device, (step.bare.geometry, shape))
     line = {
         "metric": METRIC,
@@ -477,7 +494,8 @@ def main(argv=None):
         "energy_reference_cpu": -2.8348195294207272,
         "scf_iterations": r["scf_iterations"],
         "block_applies": int(sum(r["applications"])),
-        "roofline": {"kernel": "hdiv_kernel<double,4,2> (fused Chebyshev step, k=5)", "bound": "hbm",
+        "roofline": {"kernel": f"hdiv_vec_kernel<double,2,{vec['tile_z']}> (fused Chebyshev step, k=5, "
+                               f"{vec['tile_y']}x{vec['tile_z']} tiles, {vec['segment']}-plane segments)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": FUSED_STEP_TRAFFIC_BYTES, "avg_launch_us": avg_launch * 1e6, "algorithmic_bytes_per_launch": bytes_per_launch,
                      "launches_timed": timer.launches},

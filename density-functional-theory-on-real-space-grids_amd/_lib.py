@@ -42,6 +42,9 @@ SIGNATURES: dict[str, list] = {
     "cdft_hdiv_set_segment": [I],
     "cdft_hdiv_set_tile": [I, I],
     "cdft_hdiv_launch_shape": [I, I, I, I, I, P],
+    "cdft_hdiv_set_kernel": [I],
+    "cdft_hdiv_vec_set": [I, I, I, I, I, I, I, I],
+    "cdft_hdiv_vec_shape": [I, I, I, I, I, P],
     "cdft_wgram_f64": [P, I, P, I, P, L, D, I, P, L, P, P],
     "cdft_wgram_noalloc_f64": [P, I, P, I, P, L, D, I, P, P],
     "cdft_rotate_f64": [P, I, I, I, P, P, L, P],

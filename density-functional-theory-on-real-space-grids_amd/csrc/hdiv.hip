@@ -18,6 +18,8 @@
 // than four full tiles and a one-column tail.
 #include <algorithm>
 #include <atomic>
+#include <iterator>
+#include <mutex>
 #include <type_traits>
 
 #include "common.h"
@@ -462,6 +464,399 @@ __global__ __launch_bounds__(NTMAX) void hdiv_kernel(
                                 step, md, tile, v, lds);
 }
 
+// ---- the vector-column kernel: K1 and K2 (modes 0-2) at order 8 ----
+//
+// The same marching scheme with each thread owning CZ = 16 / sizeof(T) adjacent z-columns (two
+// doubles or four floats) of a (TY, TZ) tile: every plane-tile, face-tile and global access of the
+// plane loop is one 16-byte vector. The column kernel above moves 8-byte values through LDS, whose
+// 8-byte reads run at half the bytes per clock of 16-byte reads on this part; a thread here reads
+// 5 vectors for the z-stencil of its CZ columns where a column thread reads 8 values for one. Each
+// output is computed with the operations of stencil_block in the same order (the z-stencil takes its
+// operands from a window of whole vectors), so the two kernels give bitwise the same results.
+template <typename T>
+struct V16;
+template <>
+struct V16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+  static constexpr int n = 2;
+};
+template <>
+struct V16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  static constexpr int n = 4;
+};
+
+template <typename T>
+__device__ __forceinline__ typename V16<T>::type vload(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(typename V16<T>::type, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+template <typename T>
+__device__ __forceinline__ void vstore(typename V16<T>::type v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  typedef __attribute__((ext_vector_type(4))) unsigned int u4;
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, voff, soff, 0);
+}
+
+constexpr int VEC_TMAX_Y = 17;  // largest tile extent along y of the vector kernel
+
+// LDS of one vector-kernel block. Columns are staged from z = Z0 - 8 (a whole number of vectors of
+// left halo); every pitch is congruent to the tile width modulo one 256-byte bank row, so the lanes of
+// a wave, numbered row-major over the tile's vectors, read consecutive 16-byte bank groups.
+template <typename T, int TZ>
+struct VecLds {
+  static constexpr int HZ = 8;                          // left halo columns staged
+  static constexpr int LZ = TZ + 256 / (int)sizeof(T);  // plane-tile pitch (>= TZ + 16)
+  static constexpr int YP = TZ;                         // y-face pitch
+  static constexpr int ZP = TZ + 256 / (int)sizeof(T);  // z-face pitch (>= TZ + 8)
+  T phi[(VEC_TMAX_Y + 14) * LZ];
+  T yf[(VEC_TMAX_Y + 7) * YP];
+  T zf[VEC_TMAX_Y * ZP];
+};
+
+template <typename T, int MODE, int TZ>
+__device__ __forceinline__ void stencil_block_vec(
+    const T* __restrict__ phi, const T* __restrict__ prev, T* __restrict__ out,
+    const T* __restrict__ w0, const T* __restrict__ w1, const T* __restrict__ w2,
+    const T* __restrict__ invw, const T* __restrict__ pot, int n0, int n1, int n2,
+    const Taps<4>& taps, const double* __restrict__ table, int step, const March& md, unsigned tile, int v,
+    VecLds<T, TZ>& lds) {
+  static_assert(MODE <= 2, "the vector kernel covers the apply and the Chebyshev steps");
+  typedef typename V16<T>::type V;
+  constexpr int P = 4, R = 7;                    // order 8: reach 2P - 1 = 7
+  constexpr int CZ = V16<T>::n;
+  constexpr int HZ = VecLds<T, TZ>::HZ;
+  constexpr int LZ = VecLds<T, TZ>::LZ;
+  constexpr int YP = VecLds<T, TZ>::YP;
+  constexpr int ZP = VecLds<T, TZ>::ZP;
+  constexpr int TZV = TZ / CZ;                   // vectors per tile row
+  constexpr int HL = HZ / CZ;                    // left halo vectors per row
+  constexpr int HRV = (R + CZ - 1) / CZ;         // right halo vectors per row (= threads owning edge z-faces)
+  constexpr int NW = (8 + CZ + CZ - 1) / CZ;     // vectors of a z-face window (columns c - 8 .. c + CZ - 1)
+  constexpr int NF = (R + CZ + CZ - 1) / CZ;     // vectors of a z-divergence window (faces c .. c + CZ + R - 1)
+  static_assert(TZ % CZ == 0 && TZV >= HRV, "tile width must hold whole vectors and the edge faces");
+  T* s_phi = lds.phi;
+  T* s_yf = lds.yf;
+  T* s_zf = lds.zf;
+
+  const int TY = md.ty;
+  const int bz = (int)(tile % md.tiles_z);
+  const int bs = (int)((tile / md.tiles_z) % md.nseg);
+  const int by = (int)(tile / ((unsigned)md.tiles_z * md.nseg));
+  const int X0 = (int)((long)bs * n0 / md.nseg), X1 = (int)((long)(bs + 1) * n0 / md.nseg);
+  const int Y0 = by * TY, Z0 = bz * TZ;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const bool act = t < TY * TZV;
+  const int ty = act ? t / TZV : 0, pz = act ? t - (t / TZV) * TZV : 0;
+  const int y = Y0 + ty, z = Z0 + CZ * pz;  // this thread's columns are z .. z + CZ - 1
+  const int nz = act && y < n1 ? (n2 - z < CZ ? n2 - z : CZ) : 0;  // how many of them are grid points
+  constexpr int S = (int)sizeof(T);
+  constexpr int NONE = (int)0x80000000;
+  const int plane = n1 * n2;
+  const int n1f = n1 + 2 * P, n2f = n2 + 2 * P;
+  const int pl1 = n1f * n2, pl2 = n1 * n2f;
+  const long vbase = (long)v * plane * n0;
+  const int vbytes = plane * n0 * S;
+  const auto r_phi = make_rsrc(phi + vbase, vbytes);
+  const auto r_prev = make_rsrc(MODE == 2 ? prev + vbase : phi, vbytes);
+  const auto r_out = make_rsrc(out + vbase, vbytes);
+  const auto r_w0 = make_rsrc(w0, (n0 + 2 * P) * plane * S);
+  const auto r_w1 = make_rsrc(w1, n0 * pl1 * S);
+  const auto r_w2 = make_rsrc(w2, n0 * pl2 * S);
+  const auto r_iw = make_rsrc(invw, n0 * plane * S);
+  const auto r_pt = make_rsrc(pot, n0 * plane * S);
+  // A vector whose first column is inside the grid is loaded whole; its components past n2 (the
+  // next row's values) are zeroed where they are staged as phi and otherwise only reach outputs
+  // that are never stored.
+  const int col = nz > 0 ? (y * n2 + z) * S : NONE;
+
+  T c[2 * P];
+#pragma unroll
+  for (int i = 0; i < 2 * P; ++i) c[i] = (T)taps.c[i];
+  T cc = 0, ca = 0, cb = 0;
+  if (MODE == 1 || MODE == 2) {
+    cc = (T)table[3 * step];
+    ca = (T)table[3 * step + 1];
+    cb = (T)table[3 * step + 2];
+  }
+  auto keep = [&](V a, int valid) {  // zero the components from `valid` on
+#pragma unroll
+    for (int j = 0; j < CZ; ++j) a[j] = j < valid ? a[j] : T(0);
+    return a;
+  };
+
+  // Halo vectors of this thread (the same in every plane): source byte offset (NONE = outside the
+  // grid), LDS destination (-1 = none) and how many leading components are grid points.
+  const int nhy = 2 * R * TZV, nhalo = nhy + (HL + HRV) * TY;
+  int hsrc[MAXH], hdst[MAXH], hval[MAXH];
+#pragma unroll
+  for (int s = 0; s < MAXH; ++s) {
+    const int h = t + s * nt;
+    hsrc[s] = NONE;
+    hdst[s] = -1;
+    hval[s] = 0;
+    if (h < nhy) {
+      const int hr = h / TZV, hc = h - (h / TZV) * TZV;
+      const int rl = hr < R ? hr : TY + hr;
+      const int gy = Y0 - R + rl, gz = Z0 + CZ * hc;
+      hdst[s] = rl * LZ + HZ + CZ * hc;
+      if (gy >= 0 && gy < n1 && gz < n2) {
+        hsrc[s] = (gy * n2 + gz) * S;
+        hval[s] = n2 - gz;
+      }
+    } else if (h < nhalo) {
+      const int hh = h - nhy;
+      const int r = hh / (HL + HRV), hc = hh - (hh / (HL + HRV)) * (HL + HRV);
+      const int cl = hc < HL ? CZ * hc : HZ + TZ + CZ * (hc - HL);  // LDS column
+      const int gy = Y0 + r, gz = Z0 - HZ + cl;
+      hdst[s] = (R + r) * LZ + cl;
+      if (gy < n1 && gz >= 0 && gz < n2) {
+        hsrc[s] = (gy * n2 + gz) * S;
+        hval[s] = n2 - gz;
+      }
+    }
+  }
+  const bool zin = act && z < n2, yin = act && y < n1;
+  const int ay1 = Y0 + ty + TY;       // the edge y-face row (extended index) of threads ty < R
+  const int az1 = Z0 + TZ + CZ * pz;  // the edge z-face columns (extended index) of threads pz < HRV
+  const int wy_off = (zin && y < n1f) ? (y * n2 + z) * S : NONE;
+  const int wz_off = (yin && z < n2f) ? (y * n2f + z) * S : NONE;
+  const int wy1_off = (zin && ty < R && ay1 < n1f) ? (ay1 * n2 + z) * S : NONE;
+  const int wz1_off = (yin && pz < HRV && az1 < n2f) ? (y * n2f + az1) * S : NONE;
+
+  // ---- x warm-up: phi(X0-7 .. X0+7) and the faces F(X0-4 .. X0+2) ----
+  V qphi[2 * P], qf[2 * P];
+  {
+    V pen[4 * P - 1];
+#pragma unroll
+    for (int q = 0; q < 4 * P - 1; ++q) {
+      const int gx = X0 - 2 * P + 1 + q;  // block-uniform
+      pen[q] = (gx >= 0 && gx < n0) ? vload<T>(r_phi, col, gx * plane * S) : V(0);
+    }
+#pragma unroll
+    for (int j = 0; j < 2 * P - 1; ++j) {
+      V f = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) f += c[i] * pen[j + i];
+      qf[j] = f * vload<T>(r_w0, col, (X0 + j) * plane * S);
+    }
+    qf[2 * P - 1] = 0;
+#pragma unroll
+    for (int j = 0; j < 2 * P; ++j) qphi[j] = pen[2 * P - 1 + j];
+  }
+
+  // Operands of a plane: those used before its first barrier loaded one plane ahead into two
+  // alternating buffers; the rest at the start of the plane's own body (as in stencil_block).
+  struct Plane {
+    V wxf;        // w0 at the plane's newest x-face
+    V hal[MAXH];  // halo vectors
+  };
+  struct Late {
+    V wy0, wz0, iw;
+    V phn;  // phi(m + 8)
+    V wy1, wz1;
+    V pt, pr;
+  };
+  auto load_plane = [&](int m, Plane& L) {
+    const int base = m * plane * S;
+    L.wxf = vload<T>(r_w0, col, (m + 2 * P - 1) * plane * S);
+#pragma unroll
+    for (int s = 0; s < MAXH; ++s) L.hal[s] = vload<T>(r_phi, hsrc[s], base);
+  };
+  auto load_late = [&](int m, Late& L) {
+    const int base = m * plane * S;
+    L.wy0 = vload<T>(r_w1, wy_off, m * pl1 * S);
+    L.wz0 = vload<T>(r_w2, wz_off, m * pl2 * S);
+    L.iw = vload<T>(r_iw, col, base);
+    {
+      const int mp = m + 2 * P < n0 ? m + 2 * P : n0 - 1;
+      const V yv = vload<T>(r_phi, col, mp * plane * S);
+      L.phn = m + 2 * P < n0 ? yv : V(0);
+    }
+    L.wy1 = vload<T>(r_w1, wy1_off, m * pl1 * S);
+    L.wz1 = vload<T>(r_w2, wz1_off, m * pl2 * S);
+    L.pt = vload<T>(r_pt, col, base);
+    L.pr = MODE == 2 ? vload<T>(r_prev, col, base) : V(0);
+  };
+  auto lv = [](const T* p) { return *reinterpret_cast<const V*>(p); };
+  auto sv = [](T* p, V a) { *reinterpret_cast<V*>(p) = a; };
+  // the z-faces of CZ consecutive face columns from the plane-tile window starting at LDS column w0c
+  auto zfaces = [&](int row, int w0c) {
+    T win[NW * CZ];
+#pragma unroll
+    for (int u = 0; u < NW; ++u) {
+      const V a = lv(&s_phi[row * LZ + w0c + CZ * u]);
+#pragma unroll
+      for (int j = 0; j < CZ; ++j) win[CZ * u + j] = a[j];
+    }
+    V g;
+#pragma unroll
+    for (int j = 0; j < CZ; ++j) {
+      T s = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) s += c[i] * win[1 + j + i];
+      g[j] = s;
+    }
+    return g;
+  };
+
+  auto body = [&](auto rot, int m, const Plane& cur, const Late& late) __attribute__((always_inline)) {
+    constexpr int SR = decltype(rot)::value;
+#define QI(i) (((SR) + (i)) % (2 * P))
+    {
+      V f = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) f += c[i] * qphi[QI(i)];
+      qf[QI(2 * P - 1)] = f * cur.wxf;
+    }
+    V ax = 0;
+#pragma unroll
+    for (int i = 0; i < 2 * P; ++i) ax += c[i] * qf[QI(2 * P - 1 - i)];
+
+    const V pv = qphi[QI(0)];
+    if (act) sv(&s_phi[(ty + R) * LZ + HZ + CZ * pz], keep(pv, nz));
+#pragma unroll
+    for (int s = 0; s < MAXH; ++s)
+      if (hdst[s] >= 0) sv(&s_phi[hdst[s]], keep(cur.hal[s], hval[s]));
+    __syncthreads();
+
+    if (act) {
+      // y-face F_y(y - 4) of the thread's columns, from rows y - 7 .. y
+      V f = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) f += c[i] * lv(&s_phi[(ty + i) * LZ + HZ + CZ * pz]);
+      sv(&s_yf[ty * YP + CZ * pz], f * late.wy0);
+      // z-faces F_z(z + j - 4), from columns z + j - 7 .. z + j
+      sv(&s_zf[ty * ZP + CZ * pz], zfaces(ty + R, HZ - 8 + CZ * pz) * late.wz0);
+      if (ty < R) {
+        V e = 0;
+#pragma unroll
+        for (int i = 0; i < 2 * P; ++i) e += c[i] * lv(&s_phi[(ty + TY + i) * LZ + HZ + CZ * pz]);
+        sv(&s_yf[(ty + TY) * YP + CZ * pz], e * late.wy1);
+      }
+      if (pz < HRV) sv(&s_zf[ty * ZP + TZ + CZ * pz], zfaces(ty + R, HZ - 8 + TZ + CZ * pz) * late.wz1);
+      // tiles narrower than R rows (grids thinner than 7 points) own further face rows
+      for (int r = ty + 2 * TY; r < TY + R; r += TY) {
+        V e = 0;
+#pragma unroll
+        for (int i = 0; i < 2 * P; ++i) e += c[i] * lv(&s_phi[(r + i) * LZ + HZ + CZ * pz]);
+        const int ay = Y0 + r;
+        const V w = vload<T>(r_w1, (zin && ay < n1f) ? (ay * n2 + z) * S : NONE, m * pl1 * S);
+        sv(&s_yf[r * YP + CZ * pz], e * w);
+      }
+    }
+    __syncthreads();
+
+    if (nz > 0) {
+      V ay = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * P; ++i) ay += c[i] * lv(&s_yf[(ty + R - i) * YP + CZ * pz]);
+      // az_j = sum_i c_i F_z(z + j + 3 - i): face columns CZ pz + j + R - i
+      T win[NF * CZ];
+#pragma unroll
+      for (int u = 0; u < NF; ++u) {
+        const V a = lv(&s_zf[ty * ZP + CZ * pz + CZ * u]);
+#pragma unroll
+        for (int j = 0; j < CZ; ++j) win[CZ * u + j] = a[j];
+      }
+      V az;
+#pragma unroll
+      for (int j = 0; j < CZ; ++j) {
+        T s = 0;
+#pragma unroll
+        for (int i = 0; i < 2 * P; ++i) s += c[i] * win[j + R - i];
+        az[j] = s;
+      }
+      const V lap = (ax + ay) + az;
+      V val = lap * T(0.5) * late.iw + pv * late.pt;
+      if (MODE == 1) val = (val - cc * pv) * ca;
+      if (MODE == 2) val = (val - cc * pv) * ca - cb * late.pr;
+      if (nz >= CZ) {
+        vstore<T>(val, r_out, col, m * plane * S);
+      } else {
+#pragma unroll
+        for (int j = 0; j < CZ; ++j)
+          if (j < nz) bstore<T>(val[j], r_out, col + j * S, m * plane * S);
+      }
+    }
+    qphi[QI(0)] = late.phn;
+#undef QI
+  };
+
+  // the plane loop of stencil_block: two prefetch buffers, unrolled over the x queues' period
+  Plane pa, pb;
+  load_plane(X0, pa);
+  for (int m = X0; m < X1; m += 2 * P) {
+    {
+      Late late;
+      load_late(m + 0, late);
+      load_plane(m + 1 < X1 ? m + 1 : X1 - 1, pb);
+      body(std::integral_constant<int, 0>(), m + 0, pa, late);
+    }
+    if (m + 1 >= X1) break;
+    {
+      Late late;
+      load_late(m + 1, late);
+      load_plane(m + 2 < X1 ? m + 2 : X1 - 1, pa);
+      body(std::integral_constant<int, 1>(), m + 1, pb, late);
+    }
+    if (m + 2 >= X1) break;
+    {
+      Late late;
+      load_late(m + 2, late);
+      load_plane(m + 3 < X1 ? m + 3 : X1 - 1, pb);
+      body(std::integral_constant<int, 2>(), m + 2, pa, late);
+    }
+    if (m + 3 >= X1) break;
+    {
+      Late late;
+      load_late(m + 3, late);
+      load_plane(m + 4 < X1 ? m + 4 : X1 - 1, pa);
+      body(std::integral_constant<int, 3>(), m + 3, pb, late);
+    }
+    if (m + 4 >= X1) break;
+    {
+      Late late;
+      load_late(m + 4, late);
+      load_plane(m + 5 < X1 ? m + 5 : X1 - 1, pb);
+      body(std::integral_constant<int, 4>(), m + 4, pa, late);
+    }
+    if (m + 5 >= X1) break;
+    {
+      Late late;
+      load_late(m + 5, late);
+      load_plane(m + 6 < X1 ? m + 6 : X1 - 1, pa);
+      body(std::integral_constant<int, 5>(), m + 5, pb, late);
+    }
+    if (m + 6 >= X1) break;
+    {
+      Late late;
+      load_late(m + 6, late);
+      load_plane(m + 7 < X1 ? m + 7 : X1 - 1, pb);
+      body(std::integral_constant<int, 6>(), m + 6, pa, late);
+    }
+    if (m + 7 >= X1) break;
+    {
+      Late late;
+      load_late(m + 7, late);
+      load_plane(m + 8 < X1 ? m + 8 : X1 - 1, pa);
+      body(std::integral_constant<int, 7>(), m + 7, pb, late);
+    }
+  }
+}
+
+template <typename T, int MODE, int TZ>
+__global__ __launch_bounds__(NTMAX) void hdiv_vec_kernel(
+    const T* __restrict__ phi, const T* __restrict__ prev, T* __restrict__ out,
+    const T* __restrict__ w0, const T* __restrict__ w1, const T* __restrict__ w2,
+    const T* __restrict__ invw, const T* __restrict__ pot, int n0, int n1, int n2, int k,
+    Taps<4> taps, const double* __restrict__ table, int step, March md) {
+  __shared__ VecLds<T, TZ> lds;
+  const unsigned lin = xcd_tile(blockIdx.x, gridDim.x);
+  const unsigned tile = lin / (unsigned)k;
+  const int v = (int)(lin - tile * (unsigned)k);
+  stencil_block_vec<T, MODE, TZ>(phi, prev, out, w0, w1, w2, invw, pot, n0, n1, n2, taps, table, step, md, tile, v,
+                                 lds);
+}
+
 // ---- batched launches: many grids (one descriptor each) in one launch ----
 
 // One grid of a batch. The filter rotates through out / work1 / work2 exactly as the single-grid
@@ -589,10 +984,10 @@ static int device_cus() {
 // at b per CU fitted to launch sweeps at 65^3 (k = 5, 8), 81^3 and 101 x 81 x 81 (float64 and
 // float32, tilings 8-16 x 13-16, 1-16 segments). The choice is the best or within 3% of the best
 // segment count measured on each of those shapes and on 57^3.
-static int fill_segments(int n0, long columns, int threads) {
+static int fill_segments(int n0, long columns, int threads, int waves_per_cu = 16) {
   static const double g[6] = {0.0, 0.57, 0.68, 0.90, 1.0, 1.29};
   const long cus = device_cus();
-  const int cap = std::min(5, std::max(1, 16 / std::max(1, threads / 64)));
+  const int cap = std::min(5, std::max(1, waves_per_cu / std::max(1, threads / 64)));
   const int most = ceil_div(n0, 4);
   int best = 1;
   double best_cost = 1e300;
@@ -630,12 +1025,137 @@ static March march_dims(int n0, int n1, int n2, int k, int reach) {
   return d;
 }
 
+// ---- launch shape of the vector kernel ----
+
+// Which kernel the order-8 apply and Chebyshev steps use: 0 = the vector kernel (default), 1 = the
+// column kernel (set by cdft_hdiv_set_kernel; both give bitwise the same outputs).
+static std::atomic<int> g_kernel{0};
+
+// Tile widths the vector kernel is built for, by element size.
+static const int VEC_TZ_F64[] = {16, 20, 22, 24, 28};
+static const int VEC_TZ_F32[] = {16, 20, 24, 28};
+
+static bool vec_width_ok(int tz, int elem) {
+  if (elem == 8) return std::find(std::begin(VEC_TZ_F64), std::end(VEC_TZ_F64), tz) != std::end(VEC_TZ_F64);
+  return std::find(std::begin(VEC_TZ_F32), std::end(VEC_TZ_F32), tz) != std::end(VEC_TZ_F32);
+}
+
+// Per-shape launch settings (tile rows, tile width, planes per segment) put in force by
+// cdft_hdiv_vec_set, typically from a timing sweep on the device: a launch whose grid, block width and
+// element size match an entry uses it. A small table under a lock; the oldest entry makes room.
+struct VecSetting {
+  int n0, n1, n2, k, elem, ty, tz, seg;
+};
+static std::mutex g_vec_mu;
+static VecSetting g_vec_table[64];
+static int g_vec_count = 0, g_vec_next = 0;
+
+static bool vec_lookup(int n0, int n1, int n2, int k, int elem, VecSetting& out) {
+  std::lock_guard<std::mutex> lock(g_vec_mu);
+  for (int i = 0; i < g_vec_count; ++i) {
+    const VecSetting& e = g_vec_table[i];
+    if (e.n0 == n0 && e.n1 == n1 && e.n2 == n2 && e.k == k && e.elem == elem) {
+      out = e;
+      return true;
+    }
+  }
+  return false;
+}
+
+// The vector kernel's launch: by default the tile width with the fewest idle columns (ties: the
+// wider), the largest tile rows up to VEC_TMAX_Y that split n1 evenly, and the segment count of the
+// CU-fill rule at the kernel's occupancy (three waves per SIMD in float64, two in float32).
+static March vec_dims(int n0, int n1, int n2, int k, int elem) {
+  VecSetting set{};
+  const bool tuned = vec_lookup(n0, n1, n2, k, elem, set);
+  const int cz = 16 / elem;
+  March d;
+  if (tuned) {
+    d.tz = set.tz;
+  } else {
+    int best = 0, idle = 1 << 30;
+    const int* w = elem == 8 ? VEC_TZ_F64 : VEC_TZ_F32;
+    const int nw = elem == 8 ? (int)std::size(VEC_TZ_F64) : (int)std::size(VEC_TZ_F32);
+    for (int i = 0; i < nw; ++i) {
+      const int id = ceil_div(n2, w[i]) * w[i] - n2;
+      if (id <= idle) {
+        idle = id;
+        best = w[i];
+      }
+    }
+    d.tz = best;
+  }
+  d.tiles_z = ceil_div(n2, d.tz);
+  d.tiles_y = ceil_div(n1, tuned ? set.ty : VEC_TMAX_Y);
+  d.ty = ceil_div(n1, d.tiles_y);
+  const int tzv = d.tz / cz;
+  const int reach = 7;
+  const int halo = ceil_div(2 * reach * tzv + (8 / cz + ceil_div(reach, cz)) * d.ty, MAXH);
+  const int cover = d.ty * tzv;
+  d.threads = ceil_div(std::max(cover, halo), 64) * 64;
+  const int per_plane = d.tiles_y * d.tiles_z;
+  if (tuned && set.seg > 0)
+    d.nseg = ceil_div(n0, set.seg);
+  else
+    d.nseg = fill_segments(n0, (long)per_plane * k, d.threads, elem == 8 ? 12 : 8);
+  d.nseg = std::max(1, std::min(d.nseg, n0));
+  d.seg = ceil_div(n0, d.nseg);
+  d.tiles = (unsigned)per_plane * d.nseg;
+  return d;
+}
+
+template <typename T, int MODE, int TZ>
+static int launch_vec_tz(const T* phi, const T* prev, T* out, const T* w0, const T* w1, const T* w2, const T* invw,
+                         const T* pot, int k, int n0, int n1, int n2, const Taps<4>& taps, const double* table,
+                         int step, const March& d, hipStream_t stream) {
+  const unsigned long blocks = (unsigned long)d.tiles * (unsigned long)k;
+  if (blocks > 0x7fffffffUL) return fail(CDFT_E_ARG, "hdiv: too many blocks");
+  hipLaunchKernelGGL((hdiv_vec_kernel<T, MODE, TZ>), dim3((unsigned)blocks), dim3(d.threads), 0, stream, phi, prev,
+                     out, w0, w1, w2, invw, pot, n0, n1, n2, k, taps, table, step, d);
+  return check_launch("hdiv_vec_kernel");
+}
+
+template <typename T, int MODE>
+static int launch_vec(const T* phi, const T* prev, T* out, const T* w0, const T* w1, const T* w2, const T* invw,
+                      const T* pot, int k, int n0, int n1, int n2, const Taps<4>& taps, const double* table, int step,
+                      hipStream_t stream) {
+  const March d = vec_dims(n0, n1, n2, k, (int)sizeof(T));
+  if (d.ty > VEC_TMAX_Y || d.threads > NTMAX) return fail(CDFT_E_ARG, "hdiv: vector tile needs %d threads", d.threads);
+#define CDFT_VEC_CASE(W) \
+  case W: return launch_vec_tz<T, MODE, W>(phi, prev, out, w0, w1, w2, invw, pot, k, n0, n1, n2, taps, table, step, d, stream);
+  if constexpr (sizeof(T) == 8) {
+    switch (d.tz) {
+      CDFT_VEC_CASE(16)
+      CDFT_VEC_CASE(20)
+      CDFT_VEC_CASE(22)
+      CDFT_VEC_CASE(24)
+      CDFT_VEC_CASE(28)
+    }
+  } else {
+    switch (d.tz) {
+      CDFT_VEC_CASE(16)
+      CDFT_VEC_CASE(20)
+      CDFT_VEC_CASE(24)
+      CDFT_VEC_CASE(28)
+    }
+  }
+#undef CDFT_VEC_CASE
+  return fail(CDFT_E_ARG, "hdiv: no vector kernel of tile width %d", d.tz);
+}
+
 template <typename T, int P, int MODE>
 static int launch_p(const T* phi, const T* prev, T* out, const T* w0, const T* w1, const T* w2,
                     const T* invw, const T* pot, const T* gp, int k, int n0, int n1, int n2, const double* coef,
                     const double* table, int step, hipStream_t stream) {
   Taps<P> taps;
   for (int i = 0; i < 2 * P; ++i) taps.c[i] = coef[i];
+  if constexpr (P == 4 && MODE <= 2) {
+    if (g_kernel.load(std::memory_order_relaxed) == 0) {
+      if ((long)(n0 + 2 * P) * (n1 + 2 * P) * (n2 + 2 * P) * (long)sizeof(T) >= 0x7fffffffL)
+        return fail(CDFT_E_ARG, "hdiv: grid %d x %d x %d too large for one launch", n0, n1, n2);
+      return launch_vec<T, MODE>(phi, prev, out, w0, w1, w2, invw, pot, k, n0, n1, n2, taps, table, step, stream);
+    }
+  }
   const March d = march_dims(n0, n1, n2, k, 2 * P - 1);
   if (d.threads > NTMAX) return fail(CDFT_E_ARG, "hdiv: tile needs %d threads", d.threads);
   // Offsets inside one vector and inside the extended face-weight arrays are 32-bit.
@@ -943,6 +1463,61 @@ extern "C" int cdft_hdiv_launch_shape(int n0, int n1, int n2, int k, int order, 
   if (order != 2 && order != 4 && order != 6 && order != 8)
     return fail(CDFT_E_ARG, "hdiv_launch_shape: unsupported stencil order %d", order);
   const March d = march_dims(n0, n1, n2, k, order - 1);
+  shape6[0] = d.ty;
+  shape6[1] = d.tz;
+  shape6[2] = d.seg;
+  shape6[3] = d.nseg;
+  shape6[4] = d.threads;
+  shape6[5] = (int)(d.tiles * (unsigned)k);
+  return CDFT_OK;
+}
+
+extern "C" int cdft_hdiv_set_kernel(int which) {
+  if (which != 0 && which != 1) return fail(CDFT_E_ARG, "hdiv_set_kernel: 0 (vector kernel) or 1 (column kernel)");
+  g_kernel.store(which, std::memory_order_relaxed);
+  return CDFT_OK;
+}
+
+extern "C" int cdft_hdiv_vec_set(int n0, int n1, int n2, int k, int elem_bytes, int tile_y, int tile_z, int segment) {
+  if (n0 < 1 || n1 < 1 || n2 < 1 || k < 1 || (elem_bytes != 4 && elem_bytes != 8))
+    return fail(CDFT_E_ARG, "hdiv_vec_set: bad shape, block width or element size");
+  if (tile_y == 0) {  // remove the entry
+    std::lock_guard<std::mutex> lock(g_vec_mu);
+    for (int i = 0; i < g_vec_count; ++i) {
+      VecSetting& e = g_vec_table[i];
+      if (e.n0 == n0 && e.n1 == n1 && e.n2 == n2 && e.k == k && e.elem == elem_bytes) {
+        e = g_vec_table[--g_vec_count];
+        g_vec_next = g_vec_count % 64;
+        break;
+      }
+    }
+    return CDFT_OK;
+  }
+  if (tile_y < 1 || tile_y > VEC_TMAX_Y) return fail(CDFT_E_ARG, "hdiv_vec_set: tile_y must be 1..%d", VEC_TMAX_Y);
+  if (!vec_width_ok(tile_z, elem_bytes)) return fail(CDFT_E_ARG, "hdiv_vec_set: no vector kernel of tile width %d", tile_z);
+  if (segment < 0) return fail(CDFT_E_ARG, "hdiv_vec_set: segment must be >= 0 (0 = automatic)");
+  const VecSetting entry{n0, n1, n2, k, elem_bytes, tile_y, tile_z, segment};
+  std::lock_guard<std::mutex> lock(g_vec_mu);
+  for (int i = 0; i < g_vec_count; ++i) {
+    VecSetting& e = g_vec_table[i];
+    if (e.n0 == n0 && e.n1 == n1 && e.n2 == n2 && e.k == k && e.elem == elem_bytes) {
+      e = entry;
+      return CDFT_OK;
+    }
+  }
+  if (g_vec_count < 64) {
+    g_vec_table[g_vec_count++] = entry;
+  } else {
+    g_vec_table[g_vec_next] = entry;
+    g_vec_next = (g_vec_next + 1) % 64;
+  }
+  return CDFT_OK;
+}
+
+extern "C" int cdft_hdiv_vec_shape(int n0, int n1, int n2, int k, int elem_bytes, int* shape6) {
+  if (!shape6 || n0 < 1 || n1 < 1 || n2 < 1 || k < 1 || (elem_bytes != 4 && elem_bytes != 8))
+    return fail(CDFT_E_ARG, "hdiv_vec_shape: bad arguments");
+  const March d = vec_dims(n0, n1, n2, k, elem_bytes);
   shape6[0] = d.ty;
   shape6[1] = d.tz;
   shape6[2] = d.seg;

@@ -19,7 +19,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import ops
+from . import autotune, ops
 from .hamiltonian import DeviceMeasure, lanczos_enqueue, lanczos_finish, seeded_randn
 
 BOUND_REUSE_RITZ_FRACTION = 0.9
@@ -417,6 +417,14 @@ class ChebyshevFilteredSubspace:
         block = block.contiguous()
         work = self.work_for(size)
         self._bind(hamiltonian)
+        # launch settings of the stencil kernels for this shape: timed once per process, on one of
+        # its first solves (autotune.TUNE_ON_SOLVE), before that solve captures any graph; no result
+        # depends on them
+        retuned = autotune.tune_for_solve(self._geom64, size)
+        if self.mixed:
+            retuned |= autotune.tune_for_solve(self._geom32, size)
+        if retuned:
+            self._graphs.clear()  # captured with the earlier launches
 
         def lanczos():
             return self._lanczos(hamiltonian, generator)

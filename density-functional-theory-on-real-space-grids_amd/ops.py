@@ -291,9 +291,39 @@ def hdiv_set_tile(tile_y: int = 0, tile_z: int = 0) -> None:
 
 
 def hdiv_launch_shape(shape, k: int, order: int = 8) -> dict:
-    """The launch the stencil kernels use for a ``(k, *shape)`` block (host-side, no device work)."""
+    """The launch the column stencil kernel uses for a ``(k, *shape)`` block (host-side, no device
+    work): the transpose, the residual-norm epilogue, batched launches and orders other than 8."""
     buf = (ctypes.c_int * 6)()
     _lib.call("cdft_hdiv_launch_shape", int(shape[0]), int(shape[1]), int(shape[2]), int(k), int(order),
+              ctypes.addressof(buf))
+    return dict(zip(("tile_y", "tile_z", "segment", "segments", "threads", "blocks"), list(buf)))
+
+
+def hdiv_set_kernel(which: int) -> None:
+    """Kernel of the order-8 apply and Chebyshev steps: 0 = the vector kernel (default), 1 = the
+    column kernel. Both give bitwise the same outputs."""
+    _lib.call("cdft_hdiv_set_kernel", int(which))
+
+
+def _elem_bytes(dtype) -> int:
+    if dtype == torch.float64:
+        return 8
+    if dtype == torch.float32:
+        return 4
+    raise ValueError(f"stencil kernels take float64 or float32, not {dtype}")
+
+
+def hdiv_vec_set(shape, k: int, dtype, tile_y: int, tile_z: int, segment: int) -> None:
+    """Launch settings of the vector kernel for ``(k, *shape)`` blocks of ``dtype`` (``tile_y = 0``
+    removes them). Outputs do not depend on them."""
+    _lib.call("cdft_hdiv_vec_set", int(shape[0]), int(shape[1]), int(shape[2]), int(k), _elem_bytes(dtype),
+              int(tile_y), int(tile_z), int(segment))
+
+
+def hdiv_vec_shape(shape, k: int, dtype) -> dict:
+    """The vector kernel's launch for ``(k, *shape)`` blocks of ``dtype`` (host-side)."""
+    buf = (ctypes.c_int * 6)()
+    _lib.call("cdft_hdiv_vec_shape", int(shape[0]), int(shape[1]), int(shape[2]), int(k), _elem_bytes(dtype),
               ctypes.addressof(buf))
     return dict(zip(("tile_y", "tile_z", "segment", "segments", "threads", "blocks"), list(buf)))
 

@@ -227,9 +227,26 @@ int cdft_hdiv_set_segment(int planes);
  * tuning knob with no counterpart in the reference, whose stencil is torch slicing
  * (src/cdft/operators/divergence.py:84-197). Process-wide, like cdft_hdiv_set_segment. */
 int cdft_hdiv_set_tile(int tile_y, int tile_z);
-/* Host-only: the launch the stencil kernels use for a (k, n0, n1, n2) block of the given order:
+/* Host-only: the launch the column stencil kernel uses for a (k, n0, n1, n2) block of the given order
+ * (the transpose, the residual-norm epilogue, the batched launches, and every order but 8):
  * shape6 (host) = {tile_y, tile_z, longest segment, segments, threads per block, blocks}. */
 int cdft_hdiv_launch_shape(int n0, int n1, int n2, int k, int order, int* shape6);
+
+/* The kernel of the order-8 apply and Chebyshev steps (cdft_hdiv_apply_*, cdft_cheb_step_*,
+ * cdft_cheb_filter_*): 0 = the vector kernel (each thread owns a 16-byte vector of adjacent
+ * z-columns; the default), 1 = the column kernel (one column per thread). Bitwise the same outputs;
+ * process-wide. No reference counterpart (a launch choice of the K1/K2 stencil). */
+int cdft_hdiv_set_kernel(int which);
+/* Launch settings of the vector kernel for one problem shape: blocks of k vectors on an n0 x n1 x n2
+ * grid of elem_bytes-byte values (8 or 4) use tiles of at most tile_y rows (1..17; the tiles then
+ * split n1 evenly), tile_z columns (16, 20, 22, 24 or 28 in float64; 16, 20, 24 or 28 in float32)
+ * and `segment` planes per x-segment (0 = the CU-fill rule). tile_y = 0 removes the entry. A
+ * process-wide table of 64 entries (the oldest makes room), read at every launch, so a setting
+ * applies to graphs captured after it; outputs do not depend on it. No reference counterpart. */
+int cdft_hdiv_vec_set(int n0, int n1, int n2, int k, int elem_bytes, int tile_y, int tile_z, int segment);
+/* Host-only: the vector kernel's launch for that shape: shape6 (host) = {tile_y, tile_z, longest
+ * segment, segments, threads per block, blocks}. */
+int cdft_hdiv_vec_shape(int n0, int n1, int n2, int k, int elem_bytes, int* shape6);
 
 /* ---- K7: geometry of the cusp factor and the cusp-aware quadrature (once per geometry) ----
  * Grid points sit at origin + h * (i0, i1, i2); shape (host, 3 ints) = (n0, n1, n2); origin (host,

@@ -57,3 +57,30 @@ def test_null_arguments_are_rejected_before_any_launch(pkg):
     """Validation runs on the host: a null pointer returns CDFT_E_ARG without touching a device."""
     with pytest.raises(pkg._lib.HipLibraryError, match="null pointer"):
         pkg._lib.call("cdft_hdiv_apply_f64", 0, 0, 0, 0, 0, 0, 0, 0, 1, 4, 4, 4, 0, 8, 0)
+
+
+def test_vector_launch_shape_and_settings_are_host_only(pkg):
+    """The vector kernel's launch (a host computation): 65 columns take 22-wide float64 tiles (three
+    tiles, one idle column) and 24-wide float32 tiles; a per-shape setting is used and removed; widths
+    the kernel is not built for are rejected."""
+    import torch
+
+    ops = pkg.ops
+    shape = ops.hdiv_vec_shape((65, 65, 65), 5, torch.float64)
+    assert shape["tile_z"] == 22 and shape["tile_y"] == 17 and shape["threads"] % 64 == 0
+    assert shape["blocks"] == 4 * 3 * 5 * shape["segments"]
+    assert ops.hdiv_vec_shape((65, 65, 65), 5, torch.float32)["tile_z"] == 24
+    ops.hdiv_vec_set((65, 65, 65), 5, torch.float64, 11, 24, 6)
+    try:
+        s = ops.hdiv_vec_shape((65, 65, 65), 5, torch.float64)
+        assert (s["tile_y"], s["tile_z"], s["segment"]) == (11, 24, 6)
+        assert ops.hdiv_vec_shape((65, 65, 65), 4, torch.float64)["tile_z"] == 22  # other k: untouched
+    finally:
+        ops.hdiv_vec_set((65, 65, 65), 5, torch.float64, 0, 0, 0)
+    assert ops.hdiv_vec_shape((65, 65, 65), 5, torch.float64)["tile_z"] == 22
+    with pytest.raises(pkg._lib.HipLibraryError):
+        ops.hdiv_vec_set((65, 65, 65), 5, torch.float64, 11, 13, 0)
+    with pytest.raises(pkg._lib.HipLibraryError):
+        ops.hdiv_vec_set((65, 65, 65), 5, torch.float32, 11, 22, 0)
+    with pytest.raises(pkg._lib.HipLibraryError):
+        ops.hdiv_set_kernel(2)

@@ -226,6 +226,7 @@ def test_outputs_do_not_depend_on_the_tiling(pkg, device):
 
         base = run()
         try:
+            ops.hdiv_set_kernel(1)  # the column kernel for every launch (its tiling is what varies)
             for tile, segment in (((16, 13), 0), ((13, 13), 5), ((8, 16), 0), ((3, 13), 4), ((1, 16), 11)):
                 ops.hdiv_set_tile(*tile)
                 ops.hdiv_set_segment(segment)
@@ -236,9 +237,90 @@ def test_outputs_do_not_depend_on_the_tiling(pkg, device):
                     assert torch.equal(a, b), (shape, tile, segment)
                 assert torch.allclose(got[4], base[4], rtol=1e-13, atol=0), (shape, tile, segment)
         finally:
+            ops.hdiv_set_kernel(0)
             ops.hdiv_set_tile(0, 0)
             ops.hdiv_set_segment(0)
     with pytest.raises(Exception):
         ops.hdiv_set_tile(17, 0)
     with pytest.raises(Exception):
         ops.hdiv_set_tile(0, 14)
+
+
+def _random_geometry(pkg, device, shape, seed, dtype):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(shape))
+    faces = [rng.uniform(0.5, 1.5, size=tuple(s + (8 if d == a else 0) for d, s in enumerate(shape))) for a in range(3)]
+    invw = rng.uniform(0.5, 2.0, size=n)
+    pot = rng.uniform(-1.0, 1.0, size=n)
+    geom = pkg.geometry_from_arrays(shape, 8, 0.3, faces, invw, pot, device)
+    if dtype == torch.float32:
+        geom = geom.to(torch.float32)
+    return geom
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_vector_kernel_is_the_column_kernel_bitwise(pkg, device, dtype):
+    """The vector kernel (each thread owns a 16-byte vector of adjacent z-columns) gives bitwise the
+    column kernel's apply and Chebyshev steps (modes 0, 1, 2), for grids whose rows end inside a
+    vector (n2 odd or not a multiple of four), grids thinner than the stencil's reach along y, and
+    forced launch settings of every tile width, several tile heights and segment lengths."""
+    from cdft_amd.autotune import VEC_WIDTHS
+    from cdft_amd.chefsi import scaled_chebyshev_table
+
+    ops = pkg.ops
+    cases = [((65, 65, 65), 5), ((5, 3, 70), 3), ((11, 18, 23), 1), ((20, 7, 33), 2), ((9, 41, 57), 8)]
+    for shape, k in cases:
+        geom = _random_geometry(pkg, device, shape, shape[2], dtype)
+        gen = torch.Generator().manual_seed(shape[1])
+        x = (torch.rand((k, geom.n_points), generator=gen, dtype=torch.float64) * 2 - 1).to(device, dtype)
+        p = (torch.rand((k, geom.n_points), generator=gen, dtype=torch.float64) * 2 - 1).to(device, dtype)
+        table = torch.tensor(scaled_chebyshev_table(3, -0.3, 30.0, -1.2), dtype=torch.float64, device=device)
+
+        def run():
+            return (ops.hdiv_apply(x, geom), ops.cheb_step(x, None, geom, table, 0, torch.empty_like(x)),
+                    ops.cheb_step(x, p, geom, table, 1, torch.empty_like(x)))
+
+        try:
+            ops.hdiv_set_kernel(1)
+            column = run()
+            ops.hdiv_set_kernel(0)
+            settings = [None] + [(ty, tz, seg) for tz in VEC_WIDTHS[dtype] for ty, seg in ((17, 0), (5, 3), (11, 7))]
+            for setting in settings:
+                if setting is None:
+                    ops.hdiv_vec_set(shape, k, dtype, 0, 0, 0)
+                else:
+                    ops.hdiv_vec_set(shape, k, dtype, *setting)
+                    used = ops.hdiv_vec_shape(shape, k, dtype)
+                    assert used["tile_z"] == setting[1] and used["tile_y"] <= setting[0]
+                for a, b in zip(run(), column):
+                    assert torch.equal(a, b), (shape, k, dtype, setting)
+        finally:
+            ops.hdiv_set_kernel(0)
+            ops.hdiv_vec_set(shape, k, dtype, 0, 0, 0)
+
+
+def test_tuned_launch_is_bitwise_the_default(pkg, device):
+    """The single-grid tuner times candidate vector-kernel settings on the device and puts the fastest
+    in force for the shape; the apply and the steps are bitwise those of the built-in setting."""
+    from cdft_amd import autotune
+    from cdft_amd.chefsi import scaled_chebyshev_table
+
+    ops = pkg.ops
+    shape, k = (33, 29, 37), 5
+    geom = _random_geometry(pkg, device, shape, 7, torch.float64)
+    gen = torch.Generator().manual_seed(3)
+    x = (torch.rand((k, geom.n_points), generator=gen, dtype=torch.float64) * 2 - 1).to(device)
+    p = (torch.rand((k, geom.n_points), generator=gen, dtype=torch.float64) * 2 - 1).to(device)
+    table = torch.tensor(scaled_chebyshev_table(3, -0.3, 30.0, -1.2), dtype=torch.float64, device=device)
+    ops.hdiv_vec_set(shape, k, torch.float64, 0, 0, 0)
+    base = (ops.hdiv_apply(x, geom), ops.cheb_step(x, p, geom, table, 1, torch.empty_like(x)))
+    try:
+        tuning = autotune.tune_stencil(geom, k)
+        assert tuning is not None and len(tuning["us"]) >= 2
+        assert all(t > 0 for t in tuning["us"].values())
+        got = (ops.hdiv_apply(x, geom), ops.cheb_step(x, p, geom, table, 1, torch.empty_like(x)))
+        for a, b in zip(got, base):
+            assert torch.equal(a, b)
+    finally:
+        autotune._stencil_cache.pop((shape, k, torch.float64, geom.device), None)
+        ops.hdiv_vec_set(shape, k, torch.float64, 0, 0, 0)
