@@ -29,11 +29,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # Bytes between L2 and memory per launch of the fused step on this workload, from the rocprofv3
-# TCC_EA0_RDREQ_{32B,64B,128B} / TCC_EA0_WRREQ{,_64B} request counts of `bench.py --kernel-only`, all
+# TCC_EA0_RDREQ_{32B,64B,128B} / TCC_EA0_WRREQ{,_64B} request counts of `bench.py --kernel-only --launch 11:22:6`, all
 # L2 channels summed (the 1 GiB copy of the same run reads and writes exactly 1 GiB by the same
 # formula; summary and commands: profiles/hdiv_fused_step_he65_k5.md). Re-measured whenever the
 # stencil kernel changes.
-FUSED_STEP_TRAFFIC_BYTES = (444268 * 128 + 2095 * 64) + (179896 * 64 + (186905 - 179896) * 32)
+FUSED_STEP_TRAFFIC_BYTES = (464360 * 128 + 2900 * 64) + (175365 * 64 + (183310 - 175365) * 32)
 METRIC = "He-LDA SCF s/solve & H-apply GB/s vs HBM peak; mixed vs FP64 at <=1e-8 Ha"
 PUBLISHED_CPU_S_PER_SOLVE = 109.5  # reference, He LDA 65^3, CPU float64 2 threads (convergence_ladders.json)
 
@@ -118,8 +118,9 @@ def h_apply_microbench(pkg, device, geometry_arrays, k=8, reps=200):
 
 
 def autotune_launch(geom, k):
-    """Put the tuned vector-kernel launch for ``(geom.shape, k)`` in force (the sweep the solvers run
-    on a shape's second solve) and return it with the launch shape it gives."""
+    """Put the tuned vector-kernel launch for ``(geom.shape, k)`` in force (the sweep a solver runs on a
+    shape's fifth solve; a ``--launch`` setting instead, when given) and return it with the launch shape
+    it gives."""
     from cdft_amd import autotune, ops
 
     tuning = autotune.tune_stencil(geom, k)
@@ -210,6 +211,25 @@ def kernel_only(pkg, device, k=5, filters=3):
     return {"kernel_only": True, "grid": list(grid.shape), "k": k, "us_per_step": us, "algorithmic_bytes_per_step": bytes_step,
             "GB_s": bytes_step / us / 1e3, "copy_bytes": 2**30, "points": n, "launch": launch,
             "timed_dispatches": 20 * filters}
+
+
+def solves_only(pkg, device, n):
+    """One warm-up He-LDA 65^3 solve, a 1 GiB device copy (a marker in a kernel trace), then ``n`` timed
+    solves: a trace's dispatches after the copy are the timed solves' alone."""
+    import torch
+
+    run_solve(pkg, device, torch.float64)
+    a = torch.ones(2**30 // 8, dtype=torch.float64, device=device)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    del a, b
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        r = run_solve(pkg, device, torch.float64)
+    torch.cuda.synchronize()
+    return {"solves_only": n, "s_per_solve": (time.perf_counter() - t0) / n, "energy_total": r["energies"]["total"],
+            "scf_iterations": r["scf_iterations"], "launch": pkg.ops.hdiv_vec_shape((65, 65, 65), 5, torch.float64)}
 
 
 def apply_only(pkg, device, k=8, reps=60):
@@ -432,6 +452,13 @@ def main(argv=None):
                     help="run only the 64-geometry batched H2+ scan and print its record; for rocprofv3 runs")
     ap.add_argument("--apply-only", action="store_true",
                     help="run only the k=8 apply microbench on the He 65^3 geometry and a 1 GiB copy; for rocprofv3 runs")
+    ap.add_argument("--solves-only", type=int, default=0, metavar="N",
+                    help="one warm-up He 65^3 solve, a 1 GiB device copy (the marker after which a kernel trace "
+                         "holds only the timed solves), then N timed solves; for rocprofv3 runs")
+    ap.add_argument("--launch", default=None, metavar="TY:TZ:SEG",
+                    help="vector-kernel launch (tile rows, tile width, planes per segment) for the He 65^3 grid's "
+                         "blocks, in place of the timing sweep (--kernel-only, --apply-only, --solves-only), so "
+                         "that several profiler runs measure one launch")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the Kohn-Sham path here runs on a "
@@ -445,6 +472,16 @@ def main(argv=None):
     pkg = ge.load_package()
     pkg.load_library()
     device = torch.device("cuda:0")
+    if args.launch:
+        from cdft_amd import autotune
+
+        autotune.ENABLED = False
+        setting = [int(v) for v in args.launch.split(":")]
+        for k in (1, 5, 8):
+            pkg.ops.hdiv_vec_set((65, 65, 65), k, torch.float64, *setting)
+    if args.solves_only:
+        print(json.dumps(solves_only(pkg, device, args.solves_only)), flush=True)
+        return 0
     if args.kernel_only:
         print(json.dumps(kernel_only(pkg, device)), flush=True)
         return 0
