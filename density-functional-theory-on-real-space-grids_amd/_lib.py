@@ -70,6 +70,7 @@ SIGNATURES: dict[str, list] = {
     "cdft_interp_transpose_axis_f64": [P, P, I, I, P, P, P, P, P],
     "cdft_far_blend_f64": [P, D, P, D, D, P, P, P, P],
     "cdft_mt19937_states": [ctypes.c_uint64, I, P, P, P],
+    "cdft_node_table": [I, P, P, P, P, P, D, P, I, P, P, P, P, P, L, P, P, P, P, P, P, P],
     "cdft_hydrogenic_fields_f64": [P, L, P, D, P, I, P, P, P, P, P],
     "cdft_lda_vwn_f64": [P, L, P, P, P],
     "cdft_pbe_f64": [P, P, L, P, P, P, P],

@@ -243,30 +243,76 @@ def sphere_nodes(centre, radius_sphere, spacing, radial_order, node_spacing):
     return rows.T, weights
 
 
+_dir_table: dict = {"max": 0}
+
+
+def _direction_table(max_theta: int) -> dict:
+    """The unit directions and solid-angle weights of every direction count 4..``max_theta`` (those of
+    :func:`sphere_directions`), concatenated as rows x, y, z, w with the start of each count at
+    ``offset[n]``; built once per process and grown when a larger count is asked for."""
+    if max_theta > _dir_table["max"]:
+        top = max(max_theta, 2 * _dir_table["max"], 48)
+        offset = np.zeros(top + 1, dtype=np.int64)
+        parts, weights = [], []
+        at = 0
+        for n in range(4, top + 1):
+            offset[n] = at
+            parts.append(_direction_rows(n))
+            weights.append(sphere_directions(n)[1])
+            at += 2 * n * n
+        rows = np.ascontiguousarray(np.concatenate(parts, axis=1))
+        _dir_table.update(max=top, offset=offset, x=rows[0].copy(), y=rows[1].copy(), z=rows[2].copy(),
+                          w=np.ascontiguousarray(np.concatenate(weights)))
+    return _dir_table
+
+
 def node_table(centres, radii, spacing, radial_order, node_spacings, origin) -> dict:
     """The quadrature's nodes, numbered in the order of their interpolation base cell (stable), so
     the nodes a grid point gathers from in the transpose are contiguous in memory (only the order of
     the node sums changes): coordinate rows ``(3, m)``, owners, bare weights, base-cell rows ``(3,
     m)`` (``floor((p - origin) / h)``), the permutation from the sphere-by-sphere enumeration, the
-    cell box (``lo``, ``ext``) and the node count per cell of the box."""
-    rows_list, w_list, owner_list = [], [], []
+    cell box (``lo``, ``ext``) and the node count per cell of the box.
+
+    The shells (radius, radial weight and direction count of each composite Gauss-Legendre node, as
+    :func:`sphere_node_rows` computes them) are computed here; ``cdft_node_table`` places the nodes
+    and sorts them on the host, with the same IEEE operations as :func:`sphere_node_rows`."""
+    r_all, wr_all, nt_all, atom_all = [], [], [], []
+    gl_nodes, gl_weights = gauss_legendre(radial_order)
     for atom, centre in enumerate(centres):
-        p, w = sphere_node_rows(centre, radii[atom], spacing, radial_order, node_spacings[atom])
-        rows_list.append(p)
-        w_list.append(w)
-        owner_list.append(np.full(p.shape[1], atom, dtype=np.int32))
-    rows = np.concatenate(rows_list, axis=1)
-    owners = np.concatenate(owner_list)
-    bare = np.concatenate(w_list)
-    base = np.floor((rows - np.asarray(origin, dtype=np.float64)[:, None]) / spacing).astype(np.int32)
-    lo = base.min(axis=1)
-    ext = base.max(axis=1) - lo + 1
-    key = ((base[0] - lo[0]).astype(np.int64) * int(ext[1]) + (base[1] - lo[1])) * int(ext[2]) + (base[2] - lo[2])
-    # 16-bit keys (cell boxes up to 65,536 cells) take numpy's radix sort; the same stable order
-    perm = np.argsort(key.astype(np.uint16) if int(ext.prod()) <= 65536 else key, kind="stable")
-    return {"point_rows": np.take(rows, perm, axis=1), "owners": owners[perm], "bare_weights": bare[perm],
-            "base_rows": np.take(base, perm, axis=1), "perm": perm, "lo": lo, "ext": ext,
-            "counts": np.bincount(key, minlength=int(ext.prod()))}
+        radius_sphere, node_spacing = radii[atom], node_spacings[atom]
+        n_panels = max(2, int(math.ceil(radius_sphere / spacing)))
+        edges = np.linspace(0.0, radius_sphere, n_panels + 1)
+        mid = (0.5 * (edges[1:] + edges[:-1]))[:, None]
+        half = (0.5 * (edges[1:] - edges[:-1]))[:, None]
+        r = (mid + half * gl_nodes[None, :]).ravel()
+        wr = (half * gl_weights[None, :]).ravel() * r * r
+        n_theta = np.maximum(4, np.ceil(math.sqrt(2.0 * math.pi) * r / (node_spacing * spacing)).astype(np.int64))
+        r_all.append(r)
+        wr_all.append(wr)
+        nt_all.append(n_theta)
+        atom_all.append(np.full(r.shape[0], atom, dtype=np.int32))
+    r = np.ascontiguousarray(np.concatenate(r_all))
+    wr = np.ascontiguousarray(np.concatenate(wr_all))
+    n_theta = np.ascontiguousarray(np.concatenate(nt_all).astype(np.int32))
+    atoms = np.ascontiguousarray(np.concatenate(atom_all))
+    m = int((2 * n_theta.astype(np.int64) ** 2).sum())
+    table = _direction_table(int(n_theta.max()))
+    cen = np.ascontiguousarray(np.asarray(centres, dtype=np.float64).reshape(-1, 3))
+    org = np.ascontiguousarray(np.asarray(origin, dtype=np.float64))
+    points = np.empty((3, m), dtype=np.float64)
+    owners = np.empty(m, dtype=np.int32)
+    bare = np.empty(m, dtype=np.float64)
+    base = np.empty((3, m), dtype=np.int32)
+    perm = np.empty(m, dtype=np.int64)
+    keys = np.empty(m, dtype=np.int64)
+    box = np.empty(6, dtype=np.int32)
+    ptr = lambda a: a.ctypes.data  # noqa: E731
+    _lib.call("cdft_node_table", r.shape[0], ptr(r), ptr(wr), ptr(n_theta), ptr(atoms), ptr(cen), float(spacing),
+              ptr(org), int(table["max"]), ptr(table["offset"]), ptr(table["x"]), ptr(table["y"]), ptr(table["z"]),
+              ptr(table["w"]), m, ptr(points), ptr(owners), ptr(bare), ptr(base), ptr(perm), ptr(keys), ptr(box))
+    lo, ext = box[:3].copy(), box[3:].copy()
+    return {"point_rows": points, "owners": owners, "bare_weights": bare, "base_rows": base, "perm": perm,
+            "lo": lo, "ext": ext, "counts": np.bincount(keys, minlength=int(ext.prod()))}
 
 
 class CuspQuadrature:

@@ -63,7 +63,9 @@ class SeededDraws:
 
 _draw_pool = None
 _chunk_pool = None
-_DRAW_THREADS = 8
+#: Host threads drawing chunks: the CPUs this process may run on, less two (the solve's own thread
+#: and one for the rest), at most 14 (a GPU box's CPU share is 16).
+_DRAW_THREADS = None
 _CHUNK_VALUES = 1 << 16  # values per chunk; a multiple of 16
 _state_layout_ok = None
 
@@ -72,9 +74,16 @@ def _pools():
     global _draw_pool, _chunk_pool
     if _draw_pool is None:
         import concurrent.futures
+        import os
 
-        _draw_pool = concurrent.futures.ThreadPoolExecutor(max_workers=2, thread_name_prefix="seeded-draws")
-        _chunk_pool = concurrent.futures.ThreadPoolExecutor(max_workers=_DRAW_THREADS, thread_name_prefix="draw-chunks")
+        try:
+            cpus = len(os.sched_getaffinity(0))
+        except (AttributeError, OSError):
+            cpus = os.cpu_count() or 4
+        threads = _DRAW_THREADS or max(2, min(14, cpus - 2))
+        # One sequence at a time: the first one a solve needs is drawn before the later ones start.
+        _draw_pool = concurrent.futures.ThreadPoolExecutor(max_workers=1, thread_name_prefix="seeded-draws")
+        _chunk_pool = concurrent.futures.ThreadPoolExecutor(max_workers=threads, thread_name_prefix="draw-chunks")
     return _draw_pool, _chunk_pool
 
 

@@ -269,6 +269,22 @@ int cdft_face_weights_f64(int n_atoms, const double* charges, const double* posi
                           const int* shape, double h, const double* origin, int order, double* w0,
                           double* w1, double* w2, unsigned long long* umax, void* stream);
 
+/* Host-only: the quadrature's node table. For n_shells composite Gauss-Legendre shells (radius r,
+ * radial weight wr, direction count n_theta >= 4 and owning atom; centres 3 per atom) places the
+ * 2 n_theta^2 nodes of each shell at r d + centre with bare weight wr dw (each product and sum rounded
+ * on its own), d and dw the unit directions and weights of count n (rows dir_x/y/z and dir_w from
+ * dir_offset[n], n <= max_theta), and numbers all m nodes by their base cell floor((p - origin) /
+ * spacing) with a stable counting sort. Outputs in that order: points (3, m) rows, owners, bare
+ * weights, base cells (3, m) rows, perm (sorted position -> enumeration index), the sorted cell keys
+ * and box6 = {lo, ext} of the base cells. Replaces the node construction of CuspQuadrature
+ * (src/cdft/operators/quadrature.py:129-180, sphere_nodes); the base-cell numbering has no
+ * counterpart (the reference stores a CSR interpolation matrix instead). */
+int cdft_node_table(int n_shells, const double* r, const double* wr, const int* n_theta, const int* atom,
+                    const double* centres, double spacing, const double* origin, int max_theta,
+                    const long* dir_offset, const double* dir_x, const double* dir_y, const double* dir_z,
+                    const double* dir_w, long m, double* points, int* owners, double* bare, int* base, long* perm,
+                    long* keys, int* box6);
+
 /* Per quadrature node (pts (m, 3), owner atom, base weight): partition P = (1 - far) * (Becke
  * share of the owner plus the lune of the other cells), weights = base_w * P, f^2, grad u (3, m),
  * W = -|grad u|^2 / 2 and P itself. spheres (host): the sphere table of the quadrature (centres,
