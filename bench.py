@@ -418,6 +418,52 @@ def batched_scan(device, count=64, seed=0, spacing=0.25):
                       "max_abs_eigenvalue_diff_vs_float64_batch": mixed_diff}}
 
 
+def gate_suite(device):
+    """The re-solve gates (G3.1, G3.2, G3.5, G3.4, G5.2) on the two bare molecules and the He atom
+    (LDA) at their derived production grids: per scenario the production solve and each gate's wall
+    time, verdicts and measured values, for three passes of the whole suite: the first (with the
+    per-shape launch tuning and the graph captures), then one with the G3.4 and G5.2 re-solves one
+    after another and one with them at once on side streams. Beside it, G3.1's three H2 rungs solved
+    one after another instead of as one batch."""
+    import torch
+
+    from cdft_amd import fingerprint, gates, scf
+
+    out = {"scenarios": ["h2plus_R2", "h2_R1.4", "he_atom_lda"], "passes": []}
+    for label, concurrent in (("first", True), ("sequential", False), ("concurrent", True)):
+        record = {}
+        t_pass = time.perf_counter()
+        for sid in out["scenarios"]:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            run = scf.solve_scenario(fingerprint.REGISTRY[sid], device=device)
+            t_solve = time.perf_counter() - t0
+            results, wall = gates.resolve_gates(run, concurrent=concurrent)
+            torch.cuda.synchronize()
+            record[sid] = {
+                "production_solve_s": t_solve,
+                "grid": list(run.result["hamiltonian"].grid.shape if "hamiltonian" in run.result
+                             else run.result["step"].grid.shape),
+                "gates": {r.gate_id: {"verdict": r.verdict,
+                                      "measured": r.measured if math.isfinite(r.measured) else None,
+                                      **({"reason": r.reason} if r.reason else {})} for r in results},
+                "gate_wall_s": {k: v for k, v in wall.items() if k != "rung_batches"},
+                "rung_batches": wall["rung_batches"],
+                "scenario_wall_s": time.perf_counter() - t0,
+            }
+        out["passes"].append({"pass": label, "concurrent_resolves": concurrent, "wall_s": time.perf_counter() - t_pass,
+                              "scenarios": record})
+    sc = fingerprint.REGISTRY["h2_R1.4"]
+    num, _ = fingerprint.derived_numerics(sc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for h in [num.spacing / (gates.LADDER_RATIO**r) for r in range(gates.LADDER_RUNGS)]:
+        scf.solve_scenario(sc, num.with_grid(h, tuple(num.box_lengths)), device, derive_grid=False, n_states=1)
+    torch.cuda.synchronize()
+    out["h2_G3.1_rungs_one_by_one_s"] = time.perf_counter() - t0
+    return out
+
+
 def grid_points(grid):
     n0, n1, n2 = grid.shape
     o = np.asarray(grid.origin)
@@ -445,6 +491,10 @@ def main(argv=None):
     ap.add_argument("--skip-mixed", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-gates", action="store_true")
+    ap.add_argument("--gates-only", action="store_true",
+                    help="run only the re-solve gate suite (G3.1, G3.2, G3.5, G3.4, G5.2) on H2+, H2 and He LDA "
+                         "and print its record")
     ap.add_argument("--kernel-only", action="store_true",
                     help="run only the fused Chebyshev step on the He 65^3 geometry (k=5, three degree-20 filters) "
                          "and a 1 GiB device copy that calibrates the HBM byte counters; for rocprofv3 --pmc runs")
@@ -490,6 +540,9 @@ def main(argv=None):
         return 0
     if args.batch_only:
         print(json.dumps(batched_scan(device)), flush=True)
+        return 0
+    if args.gates_only:
+        print(json.dumps(gate_suite(device)), flush=True)
         return 0
     for _ in range(args.warmup):
         run_solve(pkg, device, torch.float64)
@@ -565,6 +618,8 @@ def main(argv=None):
         line["adjoint_inversion_step"] = adjoint_inversion_step(device)
     if not args.skip_batch:
         line["h2plus_batched_scan"] = batched_scan(device)
+    if not args.skip_gates:
+        line["gate_suite"] = gate_suite(device)
     if not args.skip_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

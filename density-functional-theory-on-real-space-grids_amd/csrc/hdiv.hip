@@ -1003,12 +1003,17 @@ static int fill_segments(int n0, long columns, int threads, int waves_per_cu = 1
   return best;
 }
 
-static March march_dims(int n0, int n1, int n2, int k, int reach) {
-  March d;
-  // z tiles are 13 or 16 wide (the kernel's LDS pitches are built for those); take the one that
-  // leaves fewer idle columns (65 = 5 x 13, 64 = 4 x 16).
+// The column kernel's z tiles are 13 or 16 wide (its LDS pitches are built for those): the one that
+// leaves fewer idle columns (65 = 5 x 13, 64 = 4 x 16).
+static int column_tile_width(int n2) {
   const int idle13 = ceil_div(n2, 13) * 13 - n2, idle16 = ceil_div(n2, 16) * 16 - n2;
-  d.tz = g_tile_z ? g_tile_z : (idle13 < idle16 ? 13 : 16);
+  return idle13 < idle16 ? 13 : 16;
+}
+
+// tz: the tile width (0 = the override of cdft_hdiv_set_tile, else column_tile_width).
+static March march_dims(int n0, int n1, int n2, int k, int reach, int tz = 0) {
+  March d;
+  d.tz = tz ? tz : g_tile_z ? g_tile_z : column_tile_width(n2);
   d.tiles_z = ceil_div(n2, d.tz);
   d.tiles_y = ceil_div(n1, g_tile_y ? g_tile_y : TMAX);
   d.ty = ceil_div(n1, d.tiles_y);
@@ -1584,16 +1589,27 @@ static int batch_plan(int n_grids, const int* shapes, int k, int order, const do
   if (k < 1 || k > 65535) return fail(CDFT_E_ARG, "batch_plan: bad block width %d", k);
   BatchGrid<T>* g = (BatchGrid<T>*)desc;
   unsigned long start = 0;
-  int threads = 64, tz = 0;
+  int threads = 64;
+  // One launch runs one tile width: the override of cdft_hdiv_set_tile, else the width that leaves the
+  // fewest idle points over the whole batch (grids of different z extents may each prefer another;
+  // the outputs do not depend on the tiling).
+  int tz = g_tile_z;
+  if (tz == 0) {
+    long idle[2] = {0, 0};
+    for (int i = 0; i < n_grids; ++i) {
+      const long n0 = shapes[3 * i], n1 = shapes[3 * i + 1], n2 = shapes[3 * i + 2];
+      idle[0] += n0 * n1 * (ceil_div((int)n2, 13) * 13 - n2);
+      idle[1] += n0 * n1 * (ceil_div((int)n2, 16) * 16 - n2);
+    }
+    tz = idle[0] < idle[1] ? 13 : 16;
+  }
   for (int i = 0; i < n_grids; ++i) {
     const int n0 = shapes[3 * i], n1 = shapes[3 * i + 1], n2 = shapes[3 * i + 2];
     if (n0 < 1 || n1 < 1 || n2 < 1) return fail(CDFT_E_ARG, "batch_plan: grid %d has a bad shape", i);
     if ((long)(n0 + order) * (n1 + order) * (n2 + order) * 4 * (long)sizeof(T) >= 0x7fffffffL)
       return fail(CDFT_E_ARG, "batch_plan: grid %d too large for 32-bit offsets", i);
-    const March d = march_dims(n0, n1, n2, k, order - 1);
+    const March d = march_dims(n0, n1, n2, k, order - 1, tz);
     if (d.threads > NTMAX) return fail(CDFT_E_ARG, "batch_plan: grid %d needs %d threads", i, d.threads);
-    if (tz == 0) tz = d.tz;
-    if (d.tz != tz) return fail(CDFT_E_ARG, "batch_plan: grid %d has tile width %d, the batch %d", i, d.tz, tz);
     threads = d.threads > threads ? d.threads : threads;
     const void* const* p = ptrs + 11 * i;
     for (int j = 0; j < 11; ++j)

@@ -398,12 +398,15 @@ class ChebyshevFilteredSubspace:
 
     def solve(self, hamiltonian, n_states: int, initial: torch.Tensor | None = None, generator=None,
               bounds: SpectralBoundsHint | None = None, precision: str = "auto",
-              min_fp64_passes: int = 0) -> EigenResult:
+              min_fp64_passes: int = 0, max_iterations: int | None = None) -> EigenResult:
         """``precision`` (mixed solvers only): ``"auto"`` filters in float32 while the worst residual
         is above ``switch_residual``; ``"f32"`` filters every pass in float32 (the warm passes of an
         SCF iteration, whose vectors the outer loop refines); ``"f64"`` filters in float64 and
-        runs at least ``min_fp64_passes`` passes (the refinement of a mixed solve)."""
+        runs at least ``min_fp64_passes`` passes (the refinement of a mixed solve).
+        ``max_iterations`` caps this solve's passes in place of ``config.max_iterations`` (the SCF's
+        warm iterations); the configuration, which concurrent solves may share, is only read."""
         cfg = self.config
+        limit = cfg.max_iterations if max_iterations is None else int(max_iterations)
         g = self.grid
         extra = max(cfg.n_extra_states, int(cfg.extra_states_fraction * n_states))
         size = min(n_states + extra, g.n_points)
@@ -459,7 +462,7 @@ class ChebyshevFilteredSubspace:
         drift = float("inf")
         iteration = 0
         speculative = None  # precision of a speculative filter of the coming pass, if one is enqueued
-        for iteration in range(1, cfg.max_iterations + 1):
+        for iteration in range(1, limit + 1):
             lower = next_lower
             fraction = _fraction(lower, spectrum_min, spectrum_max)
             if source == "reused" and not fraction < BOUND_REUSE_RITZ_FRACTION:
@@ -473,7 +476,7 @@ class ChebyshevFilteredSubspace:
                 use32 = False
             else:
                 use32 = (self.mixed and not stalled32 and worst > self.switch_residual
-                         and iteration <= cfg.max_iterations - self.fp64_tail)
+                         and iteration <= limit - self.fp64_tail)
             # the device-built table takes the last Ritz value (clamped into the bounds) as its lower
             # edge; the host copy of it above decides only the bound reuse
             degree = cfg.chebyshev_degree
@@ -491,7 +494,7 @@ class ChebyshevFilteredSubspace:
             # (the last residual above SPECULATE_RESIDUAL x tol) and the weak-form stop impossible
             # (it needs two stable passes in a row and the last pass was not stable).
             speculative = None
-            if (self.speculate and iteration < cfg.max_iterations and worst > SPECULATE_RESIDUAL * cfg.residual_tol
+            if (self.speculate and iteration < limit and worst > SPECULATE_RESIDUAL * cfg.residual_tol
                     and stable == 0):
                 self._filter(work, use32, degree)
                 speculative = use32
@@ -517,7 +520,7 @@ class ChebyshevFilteredSubspace:
                 converged = True
                 stop_reason = "Ritz values stable on the weak-form path"
                 break
-        if speculative is not None and (converged or iteration == cfg.max_iterations):
+        if speculative is not None and (converged or iteration == limit):
             self.speculation["wasted"] += 1
         info = {"spectral_bounds": (spectrum_min, spectrum_max), "bounds_source": source, "bounds_shift": shift_used,
                 "top_ritz_fraction": top_fraction, "stop_reason": stop_reason, "eigenvalue_drift": drift,

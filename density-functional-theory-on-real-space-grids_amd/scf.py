@@ -10,6 +10,7 @@ iteration's scalars once, as the reference does.
 
 from __future__ import annotations
 
+import ctypes
 import math
 import time
 from dataclasses import dataclass
@@ -243,14 +244,8 @@ class KohnShamStep:
     def diagonalise(self, solver: ChebyshevFilteredSubspace, h, initial=None, max_iterations=None, bounds=None,
                     precision="auto", min_fp64_passes=0):
         gen = SeededDraws(self.numerics.seed, self.draws)
-        saved = solver.config.max_iterations
-        if max_iterations is not None:
-            solver.config.max_iterations = max_iterations
-        try:
-            return solver.solve(h, self.n_states, initial=initial, generator=gen, bounds=bounds, precision=precision,
-                                min_fp64_passes=min_fp64_passes)
-        finally:
-            solver.config.max_iterations = saved
+        return solver.solve(h, self.n_states, initial=initial, generator=gen, bounds=bounds, precision=precision,
+                            min_fp64_passes=min_fp64_passes, max_iterations=max_iterations)
 
     def output_density(self, eigen, occ) -> tuple[torch.Tensor, torch.Tensor]:
         g = self.grid
@@ -432,11 +427,28 @@ def build_geometry(scenario, numerics, device, anchor=None):
     return grid, factor, quadrature
 
 
-def initial_density(step: KohnShamStep) -> torch.Tensor:
-    """Hydrogenic superposition normalised in the cusp-aware measure (reference ``scf/loop.py:173-193``)."""
+#: The starting densities of the SCF and their exponent scales (reference ``scf/loop.py:68,181``):
+#: three genuinely different starting points for the initial-guess gate.
+INITIAL_GUESSES = {"hydrogenic": 1.0, "diffuse": 0.5, "compact": 2.0}
+
+
+def initial_density(step: KohnShamStep, kind: str = "hydrogenic") -> torch.Tensor:
+    """Superposition of ``(zeta^3/pi) exp(-2 zeta s_a)``, ``zeta = Z_a`` times 1 (hydrogenic), 0.5
+    (diffuse) or 2 (compact), normalised in the cusp-aware measure (reference ``scf/loop.py:173-193``).
+    The hydrogenic fields are the step's own; the other two are evaluated for the scaled charges."""
+    if kind not in INITIAL_GUESSES:
+        raise ValueError(f"unknown initial guess {kind!r}; choose from {tuple(INITIAL_GUESSES)}")
     g = step.grid
-    coefs = torch.tensor([z**3 / math.pi for z in step.charges], dtype=torch.float64, device=g.device)
-    total = ops.atom_combine(torch.zeros(g.n_points, dtype=torch.float64, device=g.device), coefs, step.cusp_grid, +1.0,
+    zetas = [z * INITIAL_GUESSES[kind] for z in step.charges]
+    fields = step.cusp_grid
+    if kind != "hydrogenic":
+        na = len(zetas)
+        fields = torch.empty((na, g.n_points), dtype=torch.float64, device=g.device)
+        scaled = (ctypes.c_double * na)(*zetas)
+        _lib.call("cdft_hydrogenic_fields_f64", None, g.n_points, g.c_shape, g.spacing, g.c_origin, na,
+                  ctypes.addressof(scaled), step.factor.c_positions, fields.data_ptr(), None, stream_of(fields))
+    coefs = torch.tensor([z**3 / math.pi for z in zetas], dtype=torch.float64, device=g.device)
+    total = ops.atom_combine(torch.zeros(g.n_points, dtype=torch.float64, device=g.device), coefs, fields, +1.0,
                              g.empty())
     rho = ops.ewise(ops.EW_DIV, g.empty(), total, step.f2)
     charge = torch.empty(1, dtype=torch.float64, device=g.device)
@@ -486,14 +498,17 @@ def two_solver_check(hamiltonian, grid, config, n_states: int, seed: int, first_
 
 
 def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None, filter_dtype=torch.float64,
-                          filter_hook=None, cross_check: bool | None = None):
+                          filter_hook=None, cross_check: bool | None = None, initial_guess: str = "hydrogenic"):
     """The SCF driver (reference ``scf/loop.py:255-460``). ``filter_dtype=float32`` runs the Chebyshev
     filter of every warm SCF pass in float32 (and the cold first solve until its residual is small),
     and the final diagonalisation in float64 with at least ``REFINEMENT_PASSES`` passes; Rayleigh-Ritz,
     orthonormalisation, densities, Poisson, XC and energies stay float64 throughout. ``cross_check``
     (default ``numerics.eigen.cross_check``) also solves the final operator by LOBPCG and by a second
     CheFSI diagonalisation from the converged vectors, and reports their largest eigenvalue difference
-    in ``measurements`` (reference ``scf/loop.py:485-494``)."""
+    in ``measurements`` (reference ``scf/loop.py:485-494``). ``initial_guess`` names the starting
+    density (``INITIAL_GUESSES``)."""
+    if initial_guess not in INITIAL_GUESSES:
+        raise ValueError(f"unknown initial guess {initial_guess!r}; choose from {tuple(INITIAL_GUESSES)}")
     started = time.perf_counter()
     store: dict = {}
     if geometry is None:
@@ -514,7 +529,7 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
     g = grid
     inner_weights = ops.ewise(ops.EW_DIV, g.empty(), q.mass_weights, ops.ewise(ops.EW_MUL, g.empty(), step.f2, step.f2))
     solver = subspace_solver(grid, numerics.eigen, filter_dtype, filter_hook)
-    n_in = initial_density(step)
+    n_in = initial_density(step, initial_guess)
     mixer = PulayMixer(numerics.alpha, numerics.history, numerics.pulay_period, inner_weights)
     ladder = [
         lambda: PulayMixer(numerics.alpha, numerics.history, numerics.pulay_period, inner_weights),
@@ -645,13 +660,16 @@ def solve_self_consistent(scenario, numerics, device, anchor=None, geometry=None
     }
 
 
-def solve_noninteracting(scenario, numerics, device, anchor=None, geometry=None, cross_check: bool | None = None):
+def solve_noninteracting(scenario, numerics, device, anchor=None, geometry=None, cross_check: bool | None = None,
+                         n_states: int | None = None):
     """Fixed-potential eigenproblem (reference ``scf/noninteracting.py:118-330``): CheFSI with the
     weak-form Ritz stop, then ``T_s`` and ``E_ext = band - T``. ``cross_check`` (default
-    ``numerics.eigen.cross_check``) also solves the operator by LOBPCG (``noninteracting.py:182-189``)."""
+    ``numerics.eigen.cross_check``) also solves the operator by LOBPCG (``noninteracting.py:182-189``).
+    ``n_states`` asks for more states than the electrons occupy, never fewer (``noninteracting.py:148-152``)."""
     grid, factor, q = geometry if geometry is not None else build_geometry(scenario, numerics, device, anchor)
     h = CuspFactoredHamiltonian(grid, factor, q)
-    n_states = max(1, int(-(-float(scenario.n_electrons) // 2.0)))
+    needed = max(1, int(-(-float(scenario.n_electrons) // 2.0)))
+    n_states = needed if n_states is None else max(int(n_states), needed)
     gen = torch.Generator(device="cpu").manual_seed(numerics.seed)
     solver = ChebyshevFilteredSubspace(grid, numerics.eigen, ritz_stop=True)
     result = solver.solve(h, n_states, generator=gen)
@@ -673,3 +691,55 @@ def solve_noninteracting(scenario, numerics, device, anchor=None, geometry=None,
     return {"eigenvalues": result.eigenvalues, "eigenvectors": result.eigenvectors, "n_iterations": result.n_iterations,
             "converged": result.converged, "kinetic": kinetic, "band": band, "total": band + ion, "info": result.info,
             "hamiltonian": h, "measurements": measurements}
+
+
+@dataclass
+class Run:
+    """A completed solve as the gates read it (the fields of the reference's ``RunArtifact`` that
+    ``gates/tier3.py`` and ``gates/tier5.py`` use): the scenario, the numerics with the grid the run
+    used, its lattice anchor, the device, the eigenvalues, the total energy and the raw result."""
+
+    scenario: object
+    numerics: object
+    anchor: tuple | None
+    device: torch.device
+    eigenvalues: np.ndarray
+    total: float
+    converged: bool
+    n_iterations: int
+    initial_guess: str
+    wall_s: float
+    result: dict
+
+
+def solve_scenario(scenario, numerics=None, device=None, *, derive_grid: bool = True, n_states: int | None = None,
+                   cross_check: bool = False, initial_guess: str = "hydrogenic") -> Run:
+    """Solve one scenario by the path it needs (reference ``scf/solve.py:32-64``): the SCF for an
+    interacting one, the fixed-potential eigenproblem otherwise. ``derive_grid`` replaces the grid of
+    ``numerics`` by the one the reference derives from the nuclei, anchored on the first nucleus
+    (``grid.py:294-457``); ``derive_grid=False`` solves on ``numerics``' own spacing and box with the
+    coordinate origin on the lattice, as the convergence gates need (``grid.py:448-457``)."""
+    from .fingerprint import derived_numerics  # the reference's grid rule lives beside the scenarios
+
+    device = torch.device("cuda") if device is None else torch.device(device)
+    if derive_grid:
+        numerics, anchor = derived_numerics(scenario, numerics)
+    elif numerics is None:
+        raise ValueError("derive_grid=False needs the numerics whose grid it solves on")
+    else:
+        anchor = None
+    started = time.perf_counter()
+    if scenario.interacting:
+        if n_states is not None:
+            raise ValueError("the SCF solves for the occupied states; n_states is for the fixed-potential path")
+        r = solve_self_consistent(scenario, numerics, device, anchor=anchor, cross_check=cross_check,
+                                  initial_guess=initial_guess)
+        total, iterations = float(r["energies"]["total"]), int(r["scf_iterations"])
+    else:
+        if initial_guess != "hydrogenic":
+            raise ValueError("a fixed-potential solve has no starting density")
+        r = solve_noninteracting(scenario, numerics, device, anchor=anchor, cross_check=cross_check,
+                                 n_states=n_states)
+        total, iterations = float(r["total"]), int(r["n_iterations"])
+    return Run(scenario, numerics, anchor, device, np.asarray(r["eigenvalues"], dtype=np.float64).copy(), total,
+               bool(r["converged"]), iterations, initial_guess, time.perf_counter() - started, r)

@@ -375,13 +375,21 @@ def build_geometry(scenario: Scenario, numerics: Numerics, anchor=None):
     return grid, factor, quadrature
 
 
-def initial_density(step: KohnShamStep):
-    """Hydrogenic superposition normalised in the cusp-aware measure (reference ``scf/loop.py:173-193``)."""
+#: The starting densities of the SCF and their exponent scales (reference ``scf/loop.py:68,181``).
+INITIAL_GUESSES = {"hydrogenic": 1.0, "diffuse": 0.5, "compact": 2.0}
+
+
+def initial_density(step: KohnShamStep, kind: str = "hydrogenic"):
+    """Superposition of ``(zeta^3/pi) exp(-2 zeta s_a)``, ``zeta = Z_a`` times 1 (hydrogenic), 0.5
+    (diffuse) or 2 (compact), normalised in the cusp-aware measure (reference ``scf/loop.py:173-193``)."""
+    if kind not in INITIAL_GUESSES:
+        raise ValueError(f"unknown initial guess {kind!r}; choose from {tuple(INITIAL_GUESSES)}")
     pts = step.grid.points()
     total = np.zeros(step.grid.n_points)
     for a, z in enumerate(step.charges):
+        zeta = z * INITIAL_GUESSES[kind]
         d = np.sqrt(((pts - step.positions[a]) ** 2).sum(-1))
-        total = total + (z**3 / math.pi) * np.exp(-2.0 * z * d)
+        total = total + (zeta**3 / math.pi) * np.exp(-2.0 * zeta * d)
     rho = total / step.f2
     charge = float((step.q.mass_weights * rho).sum())
     total = total * (step.n_electrons / charge)
@@ -476,7 +484,8 @@ class BoundReusePolicy:
             self.history.append(f"lanczos: {reason}")
 
 
-def solve_self_consistent(scenario: Scenario, numerics: Numerics, anchor=None, geometry=None):
+def solve_self_consistent(scenario: Scenario, numerics: Numerics, anchor=None, geometry=None,
+                          initial_guess: str = "hydrogenic"):
     """The SCF driver (reference ``scf/loop.py:255-460``): periodic Pulay, 4 filter steps per warm
     iteration, Weyl-widened bound reuse, fallback ladder, final full diagonalisation."""
     grid, factor, q = geometry if geometry is not None else build_geometry(scenario, numerics, anchor)
@@ -491,7 +500,7 @@ def solve_self_consistent(scenario: Scenario, numerics: Numerics, anchor=None, g
     def inner(a, b):
         return (inner_weights * a * b).sum()
 
-    n_in = initial_density(step)
+    n_in = initial_density(step, initial_guess)
     mixer = PulayMixer(numerics.alpha, numerics.history, numerics.pulay_period, inner)
     ladder = [
         lambda: PulayMixer(numerics.alpha, numerics.history, numerics.pulay_period, inner),

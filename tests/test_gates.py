@@ -1,0 +1,121 @@
+"""Host logic of the re-solve gates (no kernel runs): the verdict rule, the ladder and box rungs on
+the reference's grids, and the starting densities of the initial-guess gate in the oracle.
+
+Pins: the reference's ladder point counts for H2+ at R = 2 (``docs/figures/convergence_ladders.json``:
+185,193 points at h = 0.25 and 357,911 at h = 0.2 on the 14-bohr box) fix the rung grids; the verdict
+rule is the reference's ``gates/base.py:134-160``; the starting densities are checked against their
+closed-form charge (the electron count) and against each other.
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import pathlib
+
+import numpy as np
+import pytest
+
+GOLDEN = json.loads((pathlib.Path(__file__).parent / "golden" / "reference_values.json").read_text())
+
+
+def _run(pkg, sid):
+    from cdft_amd import fingerprint, scf
+
+    sc = fingerprint.REGISTRY[sid]
+    num, anchor = fingerprint.derived_numerics(sc)
+    return scf.Run(sc, num, anchor, None, np.zeros(1), 0.0, True, 0, "hydrogenic", 0.0, {})
+
+
+def test_verdict_rule(pkg):
+    from cdft_amd import gates
+
+    spec = gates.SPECS["G3.1"]
+    t = spec.threshold
+    assert gates.make_result(spec, 0.5 * t).verdict == gates.PASS
+    assert gates.make_result(spec, 0.95 * t).verdict == gates.MARGINAL
+    assert gates.make_result(spec, -0.95 * t).verdict == gates.MARGINAL
+    assert gates.make_result(spec, t).verdict == gates.MARGINAL  # at the threshold: not above it
+    assert gates.make_result(spec, 1.01 * t).verdict == gates.FAIL
+    assert gates.make_result(spec, math.nan).verdict == gates.FAIL
+    assert gates.make_result(spec, math.inf).verdict == gates.FAIL
+    r = gates.make_result(spec, 0.5 * t, {"x": 1})
+    assert r.margin == pytest.approx(0.5) and r.detail == {"x": 1}
+    s = gates.skipped(spec, "because")
+    assert s.verdict == gates.SKIPPED and math.isnan(s.measured) and s.reason == "because"
+    with pytest.raises(ValueError):
+        gates.skipped(spec, "")
+    assert gates.MILLI_EV == pytest.approx(1e-3 / 27.211386, rel=1e-6)
+
+
+def test_ladder_rungs_are_the_reference_grids(pkg):
+    from cdft_amd import gates
+    from cdft_amd.geometry import UniformGrid
+
+    run = _run(pkg, "h2plus_R2")
+    assert run.numerics.spacing == 0.25 and run.numerics.box_lengths == (14.0, 14.0, 14.0)
+    spacings = gates.ladder_spacings(run)
+    assert spacings[0] == 0.25 and spacings[1] == 0.25 / 1.25 and spacings[2] == 0.25 / 1.5625
+    grids = [UniformGrid.from_box(run.numerics.box_lengths, h, "cpu") for h in spacings]
+    assert grids[0].n_points == GOLDEN["ladder.h2plus_R2.h0.25.n_points"]
+    assert grids[1].n_points == GOLDEN["ladder.h2plus_R2.h0.2.n_points"]
+    # 14 / 0.16 rounds to 88 cells (round half to even of the quotient 87.5)
+    assert grids[2].shape == (89, 89, 89)
+    for g in grids:
+        # no anchor: the coordinate origin is a lattice point, so both protons (x = +-1) are too
+        # whenever 1 / h is whole
+        idx = [-o / g.spacing for o in g.origin]
+        assert all(abs(i - round(i)) < 1e-9 for i in idx)
+    assert gates.grown_box(run) == (18.0, 18.0, 18.0)
+    big = UniformGrid.from_box(gates.grown_box(run), run.numerics.spacing, "cpu")
+    assert big.shape == (73, 73, 73) and big.spacing == 0.25
+
+
+def test_h2_ladder_rungs(pkg):
+    from cdft_amd import gates
+    from cdft_amd.geometry import UniformGrid
+
+    run = _run(pkg, "h2_R1.4")
+    h0 = run.numerics.spacing
+    assert h0 == pytest.approx(1.4 / 6)
+    shapes = [UniformGrid.from_box(run.numerics.box_lengths, h, "cpu").shape for h in gates.ladder_spacings(run)]
+    assert shapes[0] == (59, 59, 59)
+    # 58 cells at h0: 72.5 cells at h0 / 1.25 round half to even to 72, 90.625 at h0 / 1.5625 to 91
+    assert shapes[1] == (73, 73, 73) and shapes[2] == (92, 92, 92)
+
+
+def test_interacting_ladders_skip_and_order12_is_reported(pkg):
+    from cdft_amd import gates
+
+    run = _run(pkg, "he_atom_lda")
+    for fn in (gates.spacing_ladder, gates.box_growth, gates.stencil_order):
+        r = fn(run)
+        assert r.verdict == gates.SKIPPED and r.reason
+    bare = _run(pkg, "h2plus_R2")
+    r = gates.stencil_order(bare)
+    assert r.verdict == gates.SKIPPED and "order 12" in r.reason
+    assert gates.initial_guesses(bare).verdict == gates.SKIPPED
+    with pytest.raises(ValueError):
+        gates.resolve_gates(bare, ("G9.9",))
+
+
+@pytest.mark.parametrize("kind", ["hydrogenic", "diffuse", "compact"])
+def test_oracle_starting_densities_hold_the_electrons(kind):
+    from oracle.physics import INITIAL_GUESSES, KohnShamStep, Numerics, Scenario, build_geometry, initial_density
+
+    sc = Scenario((2.0,), ((0.0, 0.0, 0.0),), 2.0, "lda")
+    num = Numerics(box_lengths=(12.0, 12.0, 12.0), spacing=0.5)
+    step = KohnShamStep(sc, num, *build_geometry(sc, num))
+    n = initial_density(step, kind)
+    assert n.shape == (1, step.grid.n_points)
+    charge = float((step.q.mass_weights * (n[0] / step.f2)).sum())
+    assert charge == pytest.approx(2.0, rel=1e-12)
+    # the shape follows the exponent: the density at the nucleus over the density 1 bohr out is
+    # exp(2 zeta) for a single centre (up to the lattice point nearest 1 bohr)
+    pts = step.grid.points()
+    r = np.sqrt((pts**2).sum(-1))
+    centre, shell = int(np.argmin(r)), int(np.argmin(np.abs(r - 1.0)))
+    zeta = 2.0 * INITIAL_GUESSES[kind]
+    assert n[0, centre] / n[0, shell] == pytest.approx(math.exp(2.0 * zeta * (r[shell] - r[centre])), rel=1e-12)
+    with pytest.raises(ValueError):
+        initial_density(step, "uniform")

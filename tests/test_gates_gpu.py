@@ -1,0 +1,147 @@
+"""The re-solve gates on the device: the spacing ladder (G3.1) against the reference's pinned gate
+values and ladder eigenvalues, the box ladder (G3.2), the initial-guess gate (G3.4) and the strict
+determinism pair (G5.2).
+
+Pins and tolerances:
+- G3.1 on ``h2plus_R2`` and ``h2_R1.4``: the reference's ``closed.*.G3.1`` values
+  (``tests/golden/values.json`` of the reference), |measured - pin| <= 1e-10 Ha/atom, the same 1e-10
+  Ha the fingerprint comparison allows an eigenvalue (two float64 solves of one discretisation differ
+  by summation order only); the H2+ rungs at h = 0.25 and h = 0.2 against the reference's ladder
+  eigenvalues (``docs/figures/convergence_ladders.json``) to 1e-11 Ha, as the existing h = 0.5 pin;
+- the batched rungs against their own fixed-potential solves: iteration counts equal, eigenvalues
+  to 1e-14 relative (the batched launch runs the single-grid per-block code);
+- G3.4: the starting densities against the oracle's to 1e-12 relative, the SCF from the diffuse guess
+  against the oracle's SCF from it (iterations equal, total energy to 1e-10 Ha), and the gate's spread
+  under its 1e-7 Ha threshold;
+- G5.2: the pair bitwise identical (deterministic kernels: fixed-order reductions, no atomics).
+"""
+
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import ROOT
+from oracle import physics as ophys
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.loads((ROOT / "tests" / "golden" / "reference_values.json").read_text())
+
+
+@pytest.mark.parametrize("sid", ["h2plus_R2", "h2_R1.4"])
+def test_spacing_and_box_ladders(pkg, device, sid):
+    from cdft_amd import fingerprint, gates, scf
+
+    print(f"[gates] {sid}: production solve", flush=True)
+    run = scf.solve_scenario(fingerprint.REGISTRY[sid], device=device)
+    assert run.converged
+    results, wall = gates.resolve_gates(run, ("G3.1", "G3.2", "G3.5"))
+    g31, g32, g35 = results
+    print(f"[gates] {sid}: G3.1 {g31.measured!r} pin {GOLDEN[f'gate.{sid}.G3.1']!r} energies "
+          f"{g31.detail['energies']} shapes {g31.detail['grid_shapes']} G3.2 {g32.measured!r} wall {wall}", flush=True)
+    assert g31.verdict == gates.PASS and g32.verdict == gates.PASS
+    assert abs(g31.measured - GOLDEN[f"gate.{sid}.G3.1"]) <= 1e-10
+    assert g35.verdict == gates.SKIPPED
+    # one batch: the three spacing rungs and the grown box; the small box is the production rung
+    assert len(wall["rung_batches"]) == 1 and wall["rung_batches"][0]["rungs"] == 4
+    assert g32.detail["energy_small_box"] == g31.detail["energies"][0]
+    assert g31.detail["monotone"] in (True, False) and len(g31.detail["successive_differences"]) == 2
+    if sid == "h2plus_R2":
+        e = g31.detail["energies"]
+        assert abs(e[0] - GOLDEN["ladder.h2plus_R2.h0.25.eigenvalue"]) < 1e-11
+        assert abs(e[1] - GOLDEN["ladder.h2plus_R2.h0.2.eigenvalue"]) < 1e-11
+        # the production rung is the production solve: both protons are lattice points either way
+        assert abs(e[0] - float(run.eigenvalues[0])) < 1e-12
+        # the reference's status notes quote the box gate at 2.1e-7 (docs/02_STATUS.md:287-288, "the
+        # box (flat from 12 to 24 bohr, G3.2 2.1e-7)"): two significant figures
+        assert abs(g32.measured - 2.1e-7) < 0.05e-7
+
+
+def test_batched_rungs_are_their_own_solves(pkg, device):
+    from cdft_amd import fingerprint, gates, scf
+
+    sc = fingerprint.REGISTRY["h2plus_R2"]
+    num, anchor = fingerprint.derived_numerics(sc)
+    num = num.with_grid(0.5, 14.0)  # a coarse ladder: 29^3, 36^3, 45^3 and the grown 37^3
+    run = scf.Run(sc, num, anchor, device, np.zeros(1), 0.0, True, 0, "hydrogenic", 0.0, {})
+    rungs = gates.Rungs(run)
+    wanted = [(h, num.box_lengths) for h in gates.ladder_spacings(run)] + [(0.5, gates.grown_box(run))]
+    rungs.solve(wanted)
+    assert len(rungs.batches) == 1
+    for spacing, box in wanted:
+        solo = scf.solve_scenario(sc, num.with_grid(spacing, tuple(box)), device, derive_grid=False, n_states=1)
+        got = rungs[(spacing, box)]
+        print(f"[gates] rung h={spacing} box={box}: batched {got['eigenvalue']!r} ({got['n_iterations']}) "
+              f"solo {float(solo.eigenvalues[0])!r} ({solo.n_iterations})", flush=True)
+        assert got["n_iterations"] == solo.n_iterations
+        assert abs(got["eigenvalue"] - solo.eigenvalues[0]) <= 1e-14 * abs(solo.eigenvalues[0])
+
+
+@pytest.mark.parametrize("kind", ["diffuse", "compact"])
+def test_starting_densities_match_the_oracle(pkg, device, kind):
+    from cdft_amd import scf
+
+    sc = ophys.Scenario((1.0, 2.0), ((-0.7, 0.0, 0.0), (0.8, 0.1, 0.0)), 3.0, "lda")
+    num = ophys.Numerics(box_lengths=(9.8, 8.4, 8.4), spacing=0.35)
+    step = scf.KohnShamStep(sc, num, *scf.build_geometry(sc, num, device))
+    ostep = ophys.KohnShamStep(sc, num, *ophys.build_geometry(sc, num))
+    got = scf.initial_density(step, kind).cpu().numpy()
+    want = ophys.initial_density(ostep, kind)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-300)
+    hydrogenic = scf.initial_density(step).cpu().numpy()
+    assert not np.allclose(got, hydrogenic, rtol=1e-3)
+
+
+def test_scf_from_the_diffuse_guess_matches_the_oracle(pkg, device):
+    from cdft_amd import scf
+
+    sc = ophys.Scenario((2.0,), ((0.0, 0.0, 0.0),), 2.0, "lda")
+    num = ophys.Numerics(box_lengths=(12.0, 12.0, 12.0), spacing=0.4)
+    r = scf.solve_self_consistent(sc, num, device, anchor=(0.0, 0.0, 0.0), initial_guess="diffuse")
+    o = ophys.solve_self_consistent(sc, num, anchor=(0.0, 0.0, 0.0), initial_guess="diffuse")
+    print(f"[gates] diffuse SCF: {r['energies']['total']!r} ({r['scf_iterations']}) oracle "
+          f"{float(o['energies']['total'])!r} ({o['scf_iterations']})", flush=True)
+    assert r["scf_iterations"] == o["scf_iterations"]
+    assert abs(r["energies"]["total"] - float(o["energies"]["total"])) < 1e-10
+    with pytest.raises(ValueError):
+        scf.solve_self_consistent(sc, num, device, initial_guess="uniform")
+
+
+def test_initial_guess_gate_and_determinism_pair_interacting(pkg, device):
+    from cdft_amd import fingerprint, gates, scf
+
+    run = scf.solve_scenario(fingerprint.REGISTRY["he_atom_lda"], device=device)
+    results, wall = gates.resolve_gates(run)
+    by_id = {r.gate_id: r for r in results}
+    print(f"[gates] he_atom_lda: " + ", ".join(f"{g} {r.verdict} {r.measured!r}" for g, r in by_id.items())
+          + f" wall {wall}", flush=True)
+    for g in ("G3.1", "G3.2", "G3.5"):
+        assert by_id[g].verdict == gates.SKIPPED
+    g34 = by_id["G3.4"]
+    assert g34.verdict == gates.PASS and set(g34.detail["energies"]) == set(scf.INITIAL_GUESSES)
+    assert abs(run.total - GOLDEN["fingerprint.he_atom_lda.total"]) < 1e-10
+    g52 = by_id["G5.2"]
+    assert g52.verdict == gates.PASS and g52.detail["bitwise_identical"] and g52.measured == 0.0
+    assert g52.detail["primary_vs_strict"] == 0.0
+    assert g34.detail["concurrent"] and g52.detail["concurrent"]
+    # the re-solves run at once on side streams give bitwise what they give one after the other
+    seq34 = gates.initial_guesses(run, concurrent=False)
+    assert seq34.detail["energies"] == g34.detail["energies"]
+    assert seq34.detail["iterations"] == g34.detail["iterations"]
+    seq52 = gates.determinism_pair(run, concurrent=False)
+    assert seq52.detail["first_run"] == g52.detail["first_run"] == g52.detail["repeat_run"]
+
+
+def test_determinism_pair_fixed_potential(pkg, device):
+    from cdft_amd import fingerprint, gates, scf
+
+    run = scf.solve_scenario(fingerprint.REGISTRY["h2plus_R2"], device=device)
+    for concurrent in (True, False):
+        g52 = gates.determinism_pair(run, concurrent)
+        assert g52.verdict == gates.PASS and g52.detail["bitwise_identical"] and g52.measured == 0.0
+        assert g52.detail["primary_vs_strict"] == 0.0
+    assert torch.cuda.is_available()
