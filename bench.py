@@ -213,12 +213,14 @@ def kernel_only(pkg, device, k=5, filters=3):
             "timed_dispatches": 20 * filters}
 
 
-def solves_only(pkg, device, n):
-    """One warm-up He-LDA 65^3 solve, a 1 GiB device copy (a marker in a kernel trace), then ``n`` timed
-    solves: a trace's dispatches after the copy are the timed solves' alone."""
+def solves_only(pkg, device, n, spacing=0.25, mixed=False):
+    """One warm-up He-LDA solve on the 16 bohr box at ``spacing`` (65^3 at 0.25, 81^3 at 0.2; float64,
+    or with ``mixed`` the float32 filter), a 1 GiB device copy (a marker in a kernel trace), then ``n``
+    timed solves: a trace's dispatches after the copy are the timed solves' alone."""
     import torch
 
-    run_solve(pkg, device, torch.float64)
+    dtype = torch.float32 if mixed else torch.float64
+    run_solve(pkg, device, dtype, spacing=spacing)
     a = torch.ones(2**30 // 8, dtype=torch.float64, device=device)
     b = torch.empty_like(a)
     b.copy_(a)
@@ -226,10 +228,14 @@ def solves_only(pkg, device, n):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(n):
-        r = run_solve(pkg, device, torch.float64)
+        r = run_solve(pkg, device, dtype, spacing=spacing)
     torch.cuda.synchronize()
-    return {"solves_only": n, "s_per_solve": (time.perf_counter() - t0) / n, "energy_total": r["energies"]["total"],
-            "scf_iterations": r["scf_iterations"], "launch": pkg.ops.hdiv_vec_shape((65, 65, 65), 5, torch.float64)}
+    shape = tuple(r["step"].grid.shape)
+    return {"solves_only": n, "spacing": spacing, "grid": list(shape), "mixed": mixed,
+            "s_per_solve": (time.perf_counter() - t0) / n, "energy_total": r["energies"]["total"],
+            "scf_iterations": r["scf_iterations"], "filter_precisions": {
+                p: r["filter_precisions"].count(p) for p in ("f32", "f64")},
+            "launch": pkg.ops.hdiv_vec_shape(shape, 5, torch.float64)}
 
 
 def apply_only(pkg, device, k=8, reps=60):
@@ -473,11 +479,12 @@ def grid_points(grid):
     return np.stack(np.broadcast_arrays(i, j, k), axis=-1).reshape(-1, 3)
 
 
-def run_solve(pkg, device, filter_dtype, timer=None):
+def run_solve(pkg, device, filter_dtype, timer=None, spacing=0.25):
     import torch
     from cdft_amd import scf
 
-    r = scf.solve_self_consistent(he_lda_scenario(), numerics(), device, filter_dtype=filter_dtype, filter_hook=timer)
+    r = scf.solve_self_consistent(he_lda_scenario(), numerics(spacing=spacing), device, filter_dtype=filter_dtype,
+                                  filter_hook=timer)
     torch.cuda.synchronize()
     return r
 
@@ -505,6 +512,9 @@ def main(argv=None):
     ap.add_argument("--solves-only", type=int, default=0, metavar="N",
                     help="one warm-up He 65^3 solve, a 1 GiB device copy (the marker after which a kernel trace "
                          "holds only the timed solves), then N timed solves; for rocprofv3 runs")
+    ap.add_argument("--spacing", type=float, default=0.25,
+                    help="grid spacing of the --solves-only solves (0.25: 65^3, 0.2: 81^3)")
+    ap.add_argument("--mixed", action="store_true", help="--solves-only with the float32 filter")
     ap.add_argument("--launch", default=None, metavar="TY:TZ:SEG",
                     help="vector-kernel launch (tile rows, tile width, planes per segment) for the He 65^3 grid's "
                          "blocks, in place of the timing sweep (--kernel-only, --apply-only, --solves-only), so "
@@ -530,7 +540,7 @@ def main(argv=None):
         for k in (1, 5, 8):
             pkg.ops.hdiv_vec_set((65, 65, 65), k, torch.float64, *setting)
     if args.solves_only:
-        print(json.dumps(solves_only(pkg, device, args.solves_only)), flush=True)
+        print(json.dumps(solves_only(pkg, device, args.solves_only, args.spacing, args.mixed)), flush=True)
         return 0
     if args.kernel_only:
         print(json.dumps(kernel_only(pkg, device)), flush=True)
