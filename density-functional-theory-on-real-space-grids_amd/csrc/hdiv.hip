@@ -28,8 +28,12 @@
 namespace cdft {
 
 constexpr int TMAX = 16;   // largest tile extent along y and z
-constexpr int MAXH = 2;    // halo values staged per thread per plane
+constexpr int MAXH = 2;    // halo values staged per thread per plane (orders up to 8)
 constexpr int NTMAX = 256; // largest block
+
+// Halo values a column-kernel thread stages per plane: the 2R (TY + TZ) halo values of a tile over
+// at most NTMAX threads (order 12, reach 11: 704 values at 16 x 16 tiles take three slots).
+constexpr int halo_slots(int reach) { return reach <= 7 ? MAXH : 3; }
 
 template <int P>
 struct Taps {
@@ -113,9 +117,9 @@ struct March {
 template <typename T, int P, int TZ>
 struct StencilLds {
   static constexpr int R = 2 * P - 1;
-  static constexpr int LZ = TZ + 16;  // plane-tile pitch (>= TZ + 2R)
-  static constexpr int YP = TZ;       // y-face pitch
-  static constexpr int ZP = TZ + 16;  // z-face pitch (>= TZ + R)
+  static constexpr int LZ = TZ + (2 * R <= 16 ? 16 : 32);  // plane-tile pitch (>= TZ + 2R)
+  static constexpr int YP = TZ;                            // y-face pitch
+  static constexpr int ZP = TZ + 16;                       // z-face pitch (>= TZ + R)
   T phi[(TMAX + 2 * R) * LZ];
   T yf[(TMAX + R) * YP];
   T zf[TMAX * ZP];
@@ -131,10 +135,11 @@ __device__ __forceinline__ void stencil_block(
     const Taps<P>& taps, const double* __restrict__ table, int step, const March& md, unsigned tile, int v,
     StencilLds<T, P, TZ>& lds) {
   constexpr int R = 2 * P - 1;  // stencil reach: 2P-1 points each side
-  static_assert(2 * R <= 16, "halo must fit the 16-double pitch pad");
-  constexpr int LZ = TZ + 16;   // plane-tile pitch (>= TZ + 2R)
-  constexpr int YP = TZ;        // y-face pitch
-  constexpr int ZP = TZ + 16;   // z-face pitch (>= TZ + R)
+  constexpr int LZ = StencilLds<T, P, TZ>::LZ;  // plane-tile pitch (>= TZ + 2R)
+  constexpr int YP = StencilLds<T, P, TZ>::YP;  // y-face pitch
+  constexpr int ZP = StencilLds<T, P, TZ>::ZP;  // z-face pitch (>= TZ + R)
+  constexpr int NH = halo_slots(R);             // halo slots per thread
+  static_assert(LZ >= TZ + 2 * R && ZP >= TZ + R && (LZ - TZ) % 16 == 0, "LDS pitches");
   T* s_phi = lds.phi;
   T* s_yf = lds.yf;
   T* s_zf = lds.zf;
@@ -191,9 +196,9 @@ __device__ __forceinline__ void stencil_block(
   // Halo slots of this thread (the same in every plane): in-plane source byte offset (NONE =
   // outside the grid, staged as zero) and LDS destination (-1 = no slot).
   const int nhy = 2 * R * TZ, nhalo = nhy + 2 * R * TY;
-  int hsrc[MAXH], hdst[MAXH];
+  int hsrc[NH], hdst[NH];
 #pragma unroll
-  for (int s = 0; s < MAXH; ++s) {
+  for (int s = 0; s < NH; ++s) {
     const int h = t + s * nt;
     hsrc[s] = NONE;
     hdst[s] = -1;
@@ -246,7 +251,7 @@ __device__ __forceinline__ void stencil_block(
   // overlaps the LDS work of a plane).
   struct Plane {
     T wxf;        // w0 at the newest x-face of the plane (extended index m + 2P - 1)
-    T hal[MAXH];  // y/z halo values
+    T hal[NH];  // y/z halo values
     T wy0, wz0, iw;
   };
   // Operands first used after the plane's first barrier: loaded at the start of the plane's own
@@ -261,7 +266,7 @@ __device__ __forceinline__ void stencil_block(
     bload_rec<T>(r_gp, rec, m * n1f * n2f * 4 * S, L.wxf, L.wy0, L.wz0, L.iw);
     if (MODE == 3) L.iw = T(0);
 #pragma unroll
-    for (int s = 0; s < MAXH; ++s) L.hal[s] = bload<T>(r_phi, hsrc[s], base);
+    for (int s = 0; s < NH; ++s) L.hal[s] = bload<T>(r_phi, hsrc[s], base);
   };
   auto load_late = [&](int m, Late& L) {
     const int base = m * plane * S;
@@ -297,7 +302,7 @@ __device__ __forceinline__ void stencil_block(
     // stage plane m
     if (act) s_phi[(ty + R) * LZ + tz + R] = qphi[QI(0)];
 #pragma unroll
-    for (int s = 0; s < MAXH; ++s)
+    for (int s = 0; s < NH; ++s)
       if (hdst[s] >= 0) s_phi[hdst[s]] = cur.hal[s];
     __syncthreads();
 
@@ -374,7 +379,7 @@ __device__ __forceinline__ void stencil_block(
   // operands out of the double buffer holds the float64 order-8 kernels at or below 128 VGPRs (four
   // waves per SIMD instead of three). The loop is unrolled over the x queues' period 2P, so the
   // queues rotate by renaming (the body's compile-time `rot`) instead of 4P-2 register moves per plane.
-  static_assert(2 * P <= 8, "the unrolled plane loop covers queue periods up to 8");
+  static_assert(2 * P <= 12, "the unrolled plane loop covers queue periods up to 12");
   Plane pa, pb;
   load_plane(X0, pa);
   for (int m = X0; m < X1; m += 2 * P) {
@@ -432,6 +437,34 @@ __device__ __forceinline__ void stencil_block(
       load_late(m + 7, late);
       load_plane(m + 8 < X1 ? m + 8 : X1 - 1, pa);
       body(std::integral_constant<int, 7>(), m + 7, pb, late);
+    }
+    if constexpr (8 < 2 * P) {
+      if (m + 8 >= X1) break;
+      Late late;
+      load_late(m + 8, late);
+      load_plane(m + 9 < X1 ? m + 9 : X1 - 1, pb);
+      body(std::integral_constant<int, 8>(), m + 8, pa, late);
+    }
+    if constexpr (9 < 2 * P) {
+      if (m + 9 >= X1) break;
+      Late late;
+      load_late(m + 9, late);
+      load_plane(m + 10 < X1 ? m + 10 : X1 - 1, pa);
+      body(std::integral_constant<int, 9>(), m + 9, pb, late);
+    }
+    if constexpr (10 < 2 * P) {
+      if (m + 10 >= X1) break;
+      Late late;
+      load_late(m + 10, late);
+      load_plane(m + 11 < X1 ? m + 11 : X1 - 1, pb);
+      body(std::integral_constant<int, 10>(), m + 10, pa, late);
+    }
+    if constexpr (11 < 2 * P) {
+      if (m + 11 >= X1) break;
+      Late late;
+      load_late(m + 11, late);
+      load_plane(m + 12 < X1 ? m + 12 : X1 - 1, pa);
+      body(std::integral_constant<int, 11>(), m + 11, pb, late);
     }
   }
   if constexpr (MODE == 4) {
@@ -938,7 +971,8 @@ static int pack_impl(const T* w0, const T* w1, const T* w2, const T* invw, int n
                      hipStream_t s) {
   if (!w0 || !w1 || !w2 || !invw || !gp || n0 < 1 || n1 < 1 || n2 < 1)
     return fail(CDFT_E_ARG, "hdiv_pack: bad arguments");
-  if (order != 2 && order != 4 && order != 6 && order != 8) return fail(CDFT_E_ARG, "hdiv_pack: bad order %d", order);
+  if (order != 2 && order != 4 && order != 6 && order != 8 && order != 12)
+    return fail(CDFT_E_ARG, "hdiv_pack: bad order %d", order);
   const long total = (long)n0 * (n1 + order) * (n2 + order);
   hipLaunchKernelGGL(pack_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, w0, w1, w2, invw, n0, n1,
                      n2, order, gp);
@@ -1018,7 +1052,7 @@ static March march_dims(int n0, int n1, int n2, int k, int reach, int tz = 0) {
   d.tiles_y = ceil_div(n1, g_tile_y ? g_tile_y : TMAX);
   d.ty = ceil_div(n1, d.tiles_y);
   const int cover = d.ty * d.tz;
-  const int halo = ceil_div(2 * reach * (d.ty + d.tz), MAXH);  // every halo value has a slot
+  const int halo = ceil_div(2 * reach * (d.ty + d.tz), halo_slots(reach));  // every halo value has a slot
   int th = cover > halo ? cover : halo;
   d.threads = ceil_div(th, 64) * 64;
   const int per_plane = d.tiles_y * d.tiles_z;
@@ -1187,7 +1221,12 @@ static int launch(const T* phi, const T* prev, T* out, const T* w0, const T* w1,
     case 4: return launch_p<T, 2, MODE>(phi, prev, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, table, step, stream);
     case 6: return launch_p<T, 3, MODE>(phi, prev, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, table, step, stream);
     case 8: return launch_p<T, 4, MODE>(phi, prev, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, table, step, stream);
-    default: return fail(CDFT_E_ARG, "unsupported stencil order %d (2, 4, 6 or 8)", order);
+    case 12:
+      if constexpr (std::is_same_v<T, double>)
+        return launch_p<T, 6, MODE>(phi, prev, out, w0, w1, w2, invw, pot, gp, k, n0, n1, n2, coef, table, step, stream);
+      else
+        return fail(CDFT_E_ARG, "stencil order 12 runs in float64 only");
+    default: return fail(CDFT_E_ARG, "unsupported stencil order %d (2, 4, 6, 8 or 12)", order);
   }
 }
 
@@ -1199,8 +1238,8 @@ static int validate(const void* phi, const void* out, const void* w0, const void
   if (k < 1 || n0 < 1 || n1 < 1 || n2 < 1)
     return fail(CDFT_E_ARG, "hdiv: sizes must be positive (k=%d, n=%d,%d,%d)", k, n0, n1, n2);
   if ((long double)n0 * n1 * n2 * k > 9.0e18L || k > 65535) return fail(CDFT_E_ARG, "hdiv: block too large");
-  if (order != 2 && order != 4 && order != 6 && order != 8)
-    return fail(CDFT_E_ARG, "hdiv: unsupported stencil order %d (2, 4, 6 or 8)", order);
+  if (order != 2 && order != 4 && order != 6 && order != 8 && order != 12)
+    return fail(CDFT_E_ARG, "hdiv: unsupported stencil order %d (2, 4, 6, 8 or 12)", order);
   if (phi == out) return fail(CDFT_E_ARG, "hdiv: out must not alias phi");
   return CDFT_OK;
 }
@@ -1465,7 +1504,7 @@ extern "C" int cdft_hdiv_set_tile(int tile_y, int tile_z) {
 
 extern "C" int cdft_hdiv_launch_shape(int n0, int n1, int n2, int k, int order, int* shape6) {
   if (!shape6 || n0 < 1 || n1 < 1 || n2 < 1 || k < 1) return fail(CDFT_E_ARG, "hdiv_launch_shape: bad arguments");
-  if (order != 2 && order != 4 && order != 6 && order != 8)
+  if (order != 2 && order != 4 && order != 6 && order != 8 && order != 12)
     return fail(CDFT_E_ARG, "hdiv_launch_shape: unsupported stencil order %d", order);
   const March d = march_dims(n0, n1, n2, k, order - 1);
   shape6[0] = d.ty;

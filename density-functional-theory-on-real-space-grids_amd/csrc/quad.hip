@@ -730,6 +730,7 @@ extern "C" int cdft_gradient_f64(const double* field, int nf, const int* shape, 
     case 4: hipLaunchKernelGGL(gradient_kernel<2>, dim3(blocks), dim3(256), 0, s, field, nf, g, coef, out); break;
     case 6: hipLaunchKernelGGL(gradient_kernel<3>, dim3(blocks), dim3(256), 0, s, field, nf, g, coef, out); break;
     case 8: hipLaunchKernelGGL(gradient_kernel<4>, dim3(blocks), dim3(256), 0, s, field, nf, g, coef, out); break;
+    case 12: hipLaunchKernelGGL(gradient_kernel<6>, dim3(blocks), dim3(256), 0, s, field, nf, g, coef, out); break;
     default: return fail(CDFT_E_ARG, "gradient: unsupported order %d", order);
   }
   return check_launch("gradient");

@@ -10,8 +10,8 @@ Each gate re-solves the scenario of a completed run and thresholds a difference:
   ``|E_large - E_small| / atoms`` (``tier3.py:183-287``);
 - G3.4 initial-guess independence: the SCF from every other starting density, the spread of the
   total energies (``tier3.py:409-460``);
-- G3.5 stencil-order independence: order 12 against the run's order (``tier3.py:290-352``). The
-  stencil kernels here run orders 2 to 8, so the gate is reported SKIPPED with that reason;
+- G3.5 stencil-order independence: the ground eigenvalue at order 12 against the run's order at the
+  run's spacing (``tier3.py:290-352``);
 - G5.2 determinism: two strict re-solves of the run's configuration, geometry rebuilt each time,
   ``max |eps_A - eps_B|`` (``tier5.py:236-363``).
 
@@ -20,8 +20,9 @@ the coordinate origin on the lattice (the reference's ``derive_grid=False``) and
 scenario, as the reference does. Their rungs are independent fixed-potential eigenproblems of
 different sizes: ``resolve_gates`` solves every distinct rung of the gates it runs as ONE lockstep
 batch (``batch.BatchedCheFSI``: one stencil launch per filter step for all rungs, each rung's
-eigenvalue and iteration count bitwise those of its own solve), and G3.1's production rung and
-G3.2's small box, one problem, are solved once. The re-solves of G3.4 and of the G5.2 pair are
+eigenvalue and iteration count bitwise those of its own solve), and G3.1's production rung, G3.2's
+small box and G3.5's run's-order solve, one problem, are solved once; G3.5's order-12 solve runs on
+its own (a batch runs one stencil order). The re-solves of G3.4 and of the G5.2 pair are
 independent too: they run at once, one host thread and one stream each (``at_once``), every scratch
 buffer, cached solver and captured graph being keyed by the stream.
 """
@@ -93,10 +94,8 @@ LADDER_RATIO = 1.25
 LADDER_RUNGS = 3
 #: Box growth in bohr of the domain-size gate (``tier3.py:201-202``).
 BOX_GROWTH = 4.0
-#: Comparison order of the stencil-order gate (``tier3.py:304-305``) and the highest order the
-#: stencil kernels here run.
+#: Comparison order of the stencil-order gate (``tier3.py:304-305``).
 HIGH_ORDER = 12
-KERNEL_MAX_ORDER = 8
 
 INTERACTING_LADDER_REASON = (
     "self-consistent scenario: the ladder gates re-solve the fixed-potential problem only; the "
@@ -330,16 +329,40 @@ def box_growth(run: scf.Run, rungs: Rungs | None = None) -> GateResult:
     })
 
 
-def stencil_order(run: scf.Run) -> GateResult:
-    """G3.5 (``tier3.py:290-352``): needs an order-12 solve, beyond the stencil kernels here."""
+def stencil_order(run: scf.Run, rungs: Rungs | None = None) -> GateResult:
+    """G3.5 (``tier3.py:290-352``): the ground eigenvalue at the run's order against order 12, both on
+    the run's spacing and box with the origin on the lattice. The run's-order solve is the production
+    rung of the other ladders (taken from ``rungs``); the order-12 one is a fixed-potential solve of
+    its own (a batch runs one stencil order)."""
     spec = SPECS["G3.5"]
     if run.scenario.interacting:
         return skipped(spec, INTERACTING_LADDER_REASON)
-    order = int(run.numerics.fd_order)
-    if order >= HIGH_ORDER:
-        return skipped(spec, f"the run's order {order} is not below the comparison order {HIGH_ORDER}")
-    return skipped(spec, f"the comparison solve needs order {HIGH_ORDER}; the stencil kernels here run "
-                         f"orders 2 to {KERNEL_MAX_ORDER}")
+    low_order = int(run.numerics.fd_order)
+    if low_order >= HIGH_ORDER:
+        return skipped(spec, f"the run's order {low_order} is not below the comparison order {HIGH_ORDER}")
+    rungs = Rungs(run) if rungs is None else rungs
+    h = float(run.numerics.spacing)
+    box = tuple(float(b) for b in run.numerics.box_lengths)
+    low = rungs[(h, box)]
+    high_numerics = dataclasses.replace(run.numerics, spacing=h, box_lengths=box, fd_order=HIGH_ORDER,
+                                        eigen=dataclasses.replace(run.numerics.eigen, cross_check=False))
+    started = time.perf_counter()
+    high = scf.solve_scenario(run.scenario, high_numerics, run.device, derive_grid=False, n_states=1)
+    wall = time.perf_counter() - started
+    if not (low["converged"] and high.converged):
+        return skipped(spec, "a solve of the order comparison did not converge")
+    e_low, e_high = low["eigenvalue"], float(high.eigenvalues[0])
+    difference = abs(e_high - e_low) / _n_atoms(run)
+    return make_result(spec, difference, {
+        "order_low": low_order,
+        "order_high": HIGH_ORDER,
+        "energy_low_order": e_low,
+        "energy_high_order": e_high,
+        "difference_per_atom": difference,
+        "spacing": h,
+        "n_iterations": [low["n_iterations"], high.n_iterations],
+        "high_order_wall_s": round(wall, 4),
+    })
 
 
 def initial_guesses(run: scf.Run, concurrent: bool = True) -> GateResult:
@@ -435,6 +458,8 @@ def resolve_gates(run: scf.Run, gate_ids=GATES, concurrent: bool = True) -> tupl
             wanted += [(h, box) for h in ladder_spacings(run)]
         if "G3.2" in gate_ids:
             wanted += [(float(run.numerics.spacing), box), (float(run.numerics.spacing), grown_box(run))]
+        if "G3.5" in gate_ids:
+            wanted += [(float(run.numerics.spacing), box)]
         started = time.perf_counter()
         rungs.solve(wanted)
         wall["rungs"] = time.perf_counter() - started
@@ -446,7 +471,7 @@ def resolve_gates(run: scf.Run, gate_ids=GATES, concurrent: bool = True) -> tupl
         elif gate_id == "G3.2":
             result = box_growth(run, rungs)
         elif gate_id == "G3.5":
-            result = stencil_order(run)
+            result = stencil_order(run, rungs)
         elif gate_id == "G3.4":
             result = initial_guesses(run, concurrent)
         else:

@@ -45,7 +45,8 @@ int cdft_hdiv_pack_f32(const float* w0, const float* w1, const float* w2, const 
 /* K1: out = 0.5 * invw * sum_d D_d^T [w_d D_d phi] + pot * phi for a block of k vectors.
  * Replaces CuspFactoredHamiltonian.apply / apply_with_potential / kinetic_apply
  * (src/cdft/operators/hamiltonian.py:251-318) and weighted_divergence
- * (src/cdft/operators/divergence.py:153-197, device tier fused.py:353-380). order in {2,4,6,8}. */
+ * (src/cdft/operators/divergence.py:153-197, device tier fused.py:353-380). order in {2,4,6,8}, and 12
+ * in float64 (the comparison order of the stencil-order gate, src/cdft/gates/tier3.py:290-352). */
 int cdft_hdiv_apply_f64(const double* phi, const double* w0, const double* w1, const double* w2,
                         const double* invw, const double* pot, const double* gp, double* out, int k,
                         int n0, int n1, int n2, const double* coef, int order, void* stream);

@@ -92,8 +92,11 @@ def test_interacting_ladders_skip_and_order12_is_reported(pkg):
         r = fn(run)
         assert r.verdict == gates.SKIPPED and r.reason
     bare = _run(pkg, "h2plus_R2")
-    r = gates.stencil_order(bare)
-    assert r.verdict == gates.SKIPPED and "order 12" in r.reason
+    import dataclasses
+
+    at12 = dataclasses.replace(bare, numerics=dataclasses.replace(bare.numerics, fd_order=12))
+    r = gates.stencil_order(at12)
+    assert r.verdict == gates.SKIPPED and "comparison order 12" in r.reason
     assert gates.initial_guesses(bare).verdict == gates.SKIPPED
     with pytest.raises(ValueError):
         gates.resolve_gates(bare, ("G9.9",))

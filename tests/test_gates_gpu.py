@@ -13,6 +13,8 @@ Pins and tolerances:
 - G3.4: the starting densities against the oracle's to 1e-12 relative, the SCF from the diffuse guess
   against the oracle's SCF from it (iterations equal, total energy to 1e-10 Ha), and the gate's spread
   under its 1e-7 Ha threshold;
+- G3.5: the order-12 solve against the oracle's order-12 solve (eigenvalue 1e-11 Ha, iterations
+  equal, kinetic energy 1e-10 Ha); its order-8 solve is bitwise the ladders' production rung;
 - G5.2: the pair bitwise identical (deterministic kernels: fixed-order reductions, no atomics).
 """
 
@@ -42,10 +44,13 @@ def test_spacing_and_box_ladders(pkg, device, sid):
     results, wall = gates.resolve_gates(run, ("G3.1", "G3.2", "G3.5"))
     g31, g32, g35 = results
     print(f"[gates] {sid}: G3.1 {g31.measured!r} pin {GOLDEN[f'gate.{sid}.G3.1']!r} energies "
-          f"{g31.detail['energies']} shapes {g31.detail['grid_shapes']} G3.2 {g32.measured!r} wall {wall}", flush=True)
-    assert g31.verdict == gates.PASS and g32.verdict == gates.PASS
+          f"{g31.detail['energies']} shapes {g31.detail['grid_shapes']} G3.2 {g32.measured!r} G3.5 {g35.measured!r} "
+          f"({g35.detail['energy_high_order']!r}) wall {wall}", flush=True)
+    assert g31.verdict == gates.PASS and g32.verdict == gates.PASS and g35.verdict == gates.PASS
     assert abs(g31.measured - GOLDEN[f"gate.{sid}.G3.1"]) <= 1e-10
-    assert g35.verdict == gates.SKIPPED
+    # G3.5's order-8 solve is the production rung of the ladders
+    assert g35.detail["energy_low_order"] == g31.detail["energies"][0]
+    assert g35.detail["order_high"] == 12 and g35.measured < 1e-6
     # one batch: the three spacing rungs and the grown box; the small box is the production rung
     assert len(wall["rung_batches"]) == 1 and wall["rung_batches"][0]["rungs"] == 4
     assert g32.detail["energy_small_box"] == g31.detail["energies"][0]
@@ -59,6 +64,23 @@ def test_spacing_and_box_ladders(pkg, device, sid):
         # the reference's status notes quote the box gate at 2.1e-7 (docs/02_STATUS.md:287-288, "the
         # box (flat from 12 to 24 bohr, G3.2 2.1e-7)"): two significant figures
         assert abs(g32.measured - 2.1e-7) < 0.05e-7
+
+
+def test_order12_solve_matches_the_oracle(pkg, device):
+    """The fixed-potential solve at stencil order 12 (G3.5's comparison solve) against the oracle's
+    order-12 solve of the same grid: eigenvalue to 1e-11 Ha, iteration count and kinetic energy (which
+    takes the order-12 gradient) equal."""
+    from cdft_amd import scf
+
+    sc = ophys.Scenario((1.0, 1.0), ((0.0, 0.0, -1.0), (0.0, 0.0, 1.0)), 1.0)
+    num = ophys.Numerics(box_lengths=(12.0, 12.0, 12.0), spacing=0.5, fd_order=12)
+    r = scf.solve_noninteracting(sc, num, device)
+    o = ophys.solve_noninteracting(sc, num)
+    print(f"[gates] order 12: {float(r['eigenvalues'][0])!r} ({r['n_iterations']}) oracle "
+          f"{float(o['eigenvalues'][0])!r} ({o['n_iterations']})", flush=True)
+    assert r["n_iterations"] == o["n_iterations"]
+    assert abs(float(r["eigenvalues"][0]) - float(o["eigenvalues"][0])) < 1e-11
+    assert abs(r["kinetic"] - float(o["kinetic"])) < 1e-10
 
 
 def test_batched_rungs_are_their_own_solves(pkg, device):
@@ -120,7 +142,7 @@ def test_initial_guess_gate_and_determinism_pair_interacting(pkg, device):
     print(f"[gates] he_atom_lda: " + ", ".join(f"{g} {r.verdict} {r.measured!r}" for g, r in by_id.items())
           + f" wall {wall}", flush=True)
     for g in ("G3.1", "G3.2", "G3.5"):
-        assert by_id[g].verdict == gates.SKIPPED
+        assert by_id[g].verdict == gates.SKIPPED  # the ladders re-solve the fixed-potential path only
     g34 = by_id["G3.4"]
     assert g34.verdict == gates.PASS and set(g34.detail["energies"]) == set(scf.INITIAL_GUESSES)
     assert abs(run.total - GOLDEN["fingerprint.he_atom_lda.total"]) < 1e-10

@@ -48,12 +48,14 @@ def test_apply_matches_oracle_f64(pkg, device, shape, spacing, charges, position
     assert (np.abs(out - expect) / scale).max() < 1e-13
 
 
-@pytest.mark.parametrize("order", [2, 4, 6, 8])
-def test_orders_match_oracle_with_random_weights(pkg, device, order):
+@pytest.mark.parametrize("order,shape", [(2, (11, 18, 23)), (4, (11, 18, 23)), (6, (11, 18, 23)), (8, (11, 18, 23)),
+                                         (12, (11, 18, 23)), (12, (3, 5, 30)), (12, (20, 4, 9)), (12, (14, 17, 2)),
+                                         (12, (29, 33, 40))])
+def test_orders_match_oracle_with_random_weights(pkg, device, order, shape):
     """Random positive face weights, ``invw`` and ``V``: every point is well conditioned, so the
-    comparison is per point."""
-    rng = np.random.default_rng(order)
-    shape = (11, 18, 23)
+    comparison is per point. Order 12 (reach 11, three halo slots a thread, the 12-plane loop) also
+    on grids thinner than its reach along each axis and on a grid of several tiles and segments."""
+    rng = np.random.default_rng(order + sum(shape))
     n = int(np.prod(shape))
     faces = [rng.uniform(0.5, 1.5, size=tuple(s + (order if d == a else 0) for d, s in enumerate(shape))) for a in range(3)]
     invw = rng.uniform(0.5, 2.0, size=n)
@@ -100,6 +102,31 @@ def test_filter_matches_oracle(pkg, device, degree):
     assert np.abs(out - expect).max() / np.abs(expect).max() < 1e-12
 
 
+def test_order12_filter_and_gradient_match_oracle(pkg, device):
+    """The order-12 cusp operator (the comparison order of the stencil-order gate) on an off-lattice
+    two-centre geometry: a degree-20 filter against the oracle's, and the order-12 central gradient
+    against the oracle's; order 12 is refused in float32."""
+    from oracle.core import CuspFactor as OracleFactor, CuspHamiltonian as OracleHamiltonian, Grid as OracleGrid
+    from cdft_amd.geometry import UniformGrid
+
+    grid = OracleGrid.from_box((8.0, 7.0, 9.0), 0.5, anchor=(0.1, -0.2, 0.3), fd_order=12)
+    ref = OracleHamiltonian(grid, OracleFactor(grid, (1.0, 2.0), [[0.1, -0.2, 0.3], [0.1, 0.4, -1.2]]))
+    geom = pkg.geometry_from_arrays(grid.shape, 12, grid.spacing, ref.face, ref.inverse_weight, ref.potential, device)
+    gen = torch.Generator().manual_seed(12)
+    block = torch.rand((3, grid.n_points), generator=gen, dtype=torch.float64) * 2 - 1
+    out = pkg.chebyshev_filter_device(block.to(device), geom, 20, -0.3, 90.0, -2.1).cpu().numpy()
+    expect = chebyshev_filter(ref, block.numpy(), 20, -0.3, 90.0, -2.1)
+    assert np.abs(out - expect).max() / np.abs(expect).max() < 1e-12
+    mine = UniformGrid(grid.shape, grid.spacing, grid.origin, device, fd_order=12, gradient_order=12)
+    got = mine.gradient(block.to(device)).cpu().numpy()
+    want = grid.gradient(block.numpy()).reshape(3, 3, -1)
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-13
+    geom32 = pkg.geometry_from_arrays(grid.shape, 12, grid.spacing, ref.face, ref.inverse_weight, ref.potential,
+                                      device, dtype=torch.float32)
+    with pytest.raises(pkg._lib.HipLibraryError, match="float64 only"):
+        pkg.ops.hdiv_apply(block.to(device=device, dtype=torch.float32), geom32)
+
+
 def test_invalid_arguments_raise(pkg, device):
     grid, ref = _case((9, 9, 9), 0.5, (1.0,), [[0.0, 0.0, 0.0]])
     geom = pkg.geometry_from_arrays(grid.shape, 8, grid.spacing, ref.face, ref.inverse_weight, ref.potential, device)
@@ -113,7 +140,8 @@ def test_invalid_arguments_raise(pkg, device):
 
 @pytest.mark.parametrize("order,shape,k,segment", [(8, (65, 65, 65), 5, 0), (8, (29, 25, 25), 3, 0),
                                                    (6, (17, 9, 40), 2, 4), (8, (6, 5, 7), 5, 0),
-                                                   (4, (30, 31, 32), 4, 11)])
+                                                   (4, (30, 31, 32), 4, 11), (12, (23, 19, 27), 3, 0),
+                                                   (12, (9, 11, 13), 2, 5)])
 def test_fused_residual_norms_match_apply_then_norm(pkg, device, order, shape, k, segment):
     """K1 with the residual-norm epilogue (nothing stored, block partials combined in fixed order)
     against the apply followed by the weighted norm of ``A x - e x``, with the oracle's numpy norm

@@ -37,14 +37,18 @@ def test_staggered_weights_closed_forms():
     assert tuple(round(c, 10) for c in staggered_coefficients(4)) == (0.0416666667, -1.125, 1.125, -0.0416666667)
     assert fd_coefficients(2, 2) == (1.0, -2.0, 1.0)
     assert fd_coefficients(1, 2) == (-0.5, 0.0, 0.5)
-    for order in (2, 4, 6, 8):
+    for order in (2, 4, 6, 8, 12):
         c = np.array(staggered_coefficients(order))
         k = np.arange(-order // 2, order // 2) + 0.5
         for m in range(order):
-            assert abs((c * k**m).sum() - (1.0 if m == 1 else 0.0)) < 1e-12
+            # exact for x^m, m < order; the bound scales with the terms (5.5^11 ~ 1e8 at order 12)
+            scale = max(1.0, float(np.abs(c * k**m).sum()))
+            assert abs((c * k**m).sum() - (1.0 if m == 1 else 0.0)) < 1e-14 * scale
+        # and not for x^(order+1) (x^order holds by the antisymmetry of c): the order is exactly this
+        assert abs((c * k ** (order + 1)).sum()) > 1e-6
 
 
-@pytest.mark.parametrize("order", [2, 4, 6, 8])
+@pytest.mark.parametrize("order", [2, 4, 6, 8, 12])
 def test_transpose_is_exact_and_composite_is_spd(order):
     """Assemble ``D^T w D`` column by column on a 1-D line: symmetric to round-off, PSD, and for
     ``order = 2``, ``w = 1`` the 3-point Laplacian."""
