@@ -64,6 +64,7 @@ SIGNATURES: dict[str, list] = {
     "cdft_far_factor_f64": [P, D, P, I, P, P, P],
     "cdft_node_stencils_f64": [P, L, P, D, P, P, P, P, P],
     "cdft_interpolate_f64": [P, I, P, P, L, P, D, P, P, P],
+    "cdft_interpolate_chunks_f64": [P, I, P, I, P, P, L, P, D, P, P, P],
     "cdft_lift_f64": [P, I, L, P, P, P, D, P, P, I, P, P, P, P, P, L, P, P, L, P, P],
     "cdft_gradient_f64": [P, I, P, D, P, P, I, P, P],
     "cdft_far_fine_f64": [I, P, P, P, P, P, P, D, P, P],

@@ -281,6 +281,57 @@ __global__ __launch_bounds__(256) void interpolate_kernel(const double* __restri
   }
 }
 
+// The same interpolant with the stencil block shared: one wavefront per chunk of at most
+// INTERP_CHUNK consecutive nodes that share one base cell (nodes are numbered in base-cell order, so a
+// cell's nodes are consecutive). Lane 8 j1 + j2 loads its (y, z) column of the cell's 8^3 block for
+// every field once; then, for each node of the chunk, it forms exactly interpolate_kernel's sum over
+// the 8 x offsets and the same butterfly, so every output is bitwise interpolate_kernel's. A node's
+// 8^3 loads (8 rows of 64 bytes per field) are made once per chunk instead of once per node.
+constexpr int INTERP_CHUNK = 16;
+
+template <int NF>
+__global__ __launch_bounds__(256) void interpolate_chunks_kernel(const double* __restrict__ field, long nfield,
+                                                                 const int* __restrict__ chunk_node, int n_chunks,
+                                                                 const int* __restrict__ base,
+                                                                 const double* __restrict__ lw, long m, GridDesc g,
+                                                                 double* __restrict__ out) {
+  const int chunk = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);  // wave-uniform
+  if (chunk >= n_chunks) return;
+  const int lane = (int)(threadIdx.x & 63);
+  const int j1 = lane >> 3, j2 = lane & 7;
+  const int first = chunk_node[chunk], last = chunk_node[chunk + 1];
+  const long plane = (long)g.n1 * g.n2;
+  const long b0 = base[3 * first] - 3;
+  const long col = (long)(base[3 * first + 1] - 3 + j1) * g.n2 + (base[3 * first + 2] - 3 + j2);
+  double v[NF][8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const long idx = (b0 + a) * plane + col;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) v[f][a] = field[f * nfield + idx];
+  }
+  for (int node = first; node < last; ++node) {
+    const double* w = lw + (long)node * 24;
+    const double wyz = w[8 + j1] * w[16 + j2];
+    double acc[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) acc[f] = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const double wa = w[a] * wyz;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) acc[f] += wa * v[f][a];
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      double x = acc[f];
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+      if (lane == 0) out[f * m + node] = x;
+    }
+  }
+}
+
 struct CellBox {
   int lo0, lo1, lo2;  // smallest base cell
   int c0, c1, c2;     // cell-box extents
@@ -661,6 +712,24 @@ extern "C" int cdft_interpolate_f64(const double* field, int nf, const int* base
   hipLaunchKernelGGL(interpolate_kernel, dim3(blocks_for(m * 64, 256)), dim3(256), 0, (hipStream_t)stream, field,
                      nf, n, base, lw, m, g, out);
   return check_launch("interpolate");
+}
+
+extern "C" int cdft_interpolate_chunks_f64(const double* field, int nf, const int* chunk_node, int n_chunks,
+                                           const int* base, const double* lw, long m, const int* shape, double h,
+                                           const double* origin, double* out, void* stream) {
+  if (!field || !chunk_node || !base || !lw || !out || nf < 1 || nf > 4 || m < 1 || n_chunks < 1)
+    return fail(CDFT_E_ARG, "interpolate_chunks: bad arguments (nf=%d, chunks=%d)", nf, n_chunks);
+  GridDesc g = make_grid(shape, h, origin);
+  const long n = (long)g.n0 * g.n1 * g.n2;
+  const dim3 grid(blocks_for(64L * n_chunks, 256)), block(256);
+  hipStream_t s = (hipStream_t)stream;
+  switch (nf) {
+    case 1: hipLaunchKernelGGL(interpolate_chunks_kernel<1>, grid, block, 0, s, field, n, chunk_node, n_chunks, base, lw, m, g, out); break;
+    case 2: hipLaunchKernelGGL(interpolate_chunks_kernel<2>, grid, block, 0, s, field, n, chunk_node, n_chunks, base, lw, m, g, out); break;
+    case 3: hipLaunchKernelGGL(interpolate_chunks_kernel<3>, grid, block, 0, s, field, n, chunk_node, n_chunks, base, lw, m, g, out); break;
+    default: hipLaunchKernelGGL(interpolate_chunks_kernel<4>, grid, block, 0, s, field, n, chunk_node, n_chunks, base, lw, m, g, out); break;
+  }
+  return check_launch("interpolate_chunks");
 }
 
 // cell_box (host): lo0, lo1, lo2, c0, c1, c2; the chunk tables (device, int32): chunk_node

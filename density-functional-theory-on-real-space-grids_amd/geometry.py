@@ -412,6 +412,7 @@ class CuspQuadrature:
         self.node_order = perm  # GPU node i is node perm[i] of the sphere-by-sphere enumeration
         self._cell_box = _ints([lo[0], lo[1], lo[2], ext[0], ext[1], ext[2]])
         self._build_lift_chunks(self._base_rows[2], start, lo, ext, as_dev)
+        self._build_interp_chunks(counts, start, as_dev)
         # Far part of the rule.
         self.far = grid.empty()
         _lib.call("cdft_far_factor_f64", grid.c_shape, h, grid.c_origin, n_atoms, ctypes.addressof(self._spheres_c),
@@ -469,13 +470,36 @@ class CuspQuadrature:
         self._col_chunk = as_dev(col_chunk.astype(np.int32), torch.int32)
         self._lift_per_field = int(off[-1])
 
-    def interpolate(self, field: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
-        """Degree-7 interpolant at the nodes, ``(nf, m)`` for ``(nf, N)`` (``nf <= 4``)."""
+    #: Nodes per chunk of the interpolation (nodes of one base cell that share one load of its 8^3
+    #: stencil block; the kernel's own bound).
+    interp_chunk = 16
+
+    def _build_interp_chunks(self, counts, cell_start, as_dev) -> None:
+        """Split every non-empty cell's nodes (consecutive in node order) into chunks of at most
+        ``interp_chunk``: the chunk boundaries as node offsets, ending with the node count."""
+        cells = np.flatnonzero(counts)
+        first = np.asarray(cell_start, dtype=np.int64)[cells]
+        per_cell = -(-np.asarray(counts, dtype=np.int64)[cells] // self.interp_chunk)
+        total = int(per_cell.sum())
+        before = np.repeat(np.cumsum(per_cell) - per_cell, per_cell)
+        starts = np.repeat(first, per_cell) + (np.arange(total) - before) * self.interp_chunk
+        self._interp_chunks = total
+        self._interp_node = as_dev(np.append(starts, self.n_nodes).astype(np.int32), torch.int32)
+
+    def interpolate(self, field: torch.Tensor, out: torch.Tensor | None = None, per_node: bool = False) -> torch.Tensor:
+        """Degree-7 interpolant at the nodes, ``(nf, m)`` for ``(nf, N)`` (``nf <= 4``): one load of
+        each base cell's stencil block per chunk of its nodes (``per_node``: one per node, the same
+        values bitwise)."""
         f = field.reshape(-1, self.grid.n_points)
         nf = f.shape[0]
         out = torch.empty((nf, self.n_nodes), dtype=torch.float64, device=self.grid.device) if out is None else out
-        _lib.call("cdft_interpolate_f64", f.data_ptr(), nf, self.base.data_ptr(), self.lw.data_ptr(), self.n_nodes,
-                  self.grid.c_shape, self.grid.spacing, self.grid.c_origin, out.data_ptr(), stream_of(f))
+        if per_node:
+            _lib.call("cdft_interpolate_f64", f.data_ptr(), nf, self.base.data_ptr(), self.lw.data_ptr(), self.n_nodes,
+                      self.grid.c_shape, self.grid.spacing, self.grid.c_origin, out.data_ptr(), stream_of(f))
+        else:
+            _lib.call("cdft_interpolate_chunks_f64", f.data_ptr(), nf, self._interp_node.data_ptr(), self._interp_chunks,
+                      self.base.data_ptr(), self.lw.data_ptr(), self.n_nodes, self.grid.c_shape, self.grid.spacing,
+                      self.grid.c_origin, out.data_ptr(), stream_of(f))
         return out.reshape(-1) if field.dim() == 1 else out
 
     def lift(self, node_values: torch.Tensor, add: torch.Tensor | None = None, out: torch.Tensor | None = None):

@@ -315,6 +315,14 @@ int cdft_interpolate_f64(const double* field, int nf, const int* base, const dou
                          const int* shape, double h, const double* origin, double* out,
                          void* stream);
 
+/* cdft_interpolate_f64 at every node of a quadrature whose nodes are numbered in base-cell order,
+ * bitwise its result: chunk_node (device, int32, n_chunks + 1 node offsets, the last = m) splits the
+ * nodes into chunks of at most 16 consecutive nodes that share one base cell; each chunk's 8^3 stencil
+ * block is loaded once. Replaces CuspQuadrature.interpolate (src/cdft/operators/quadrature.py:613-620). */
+int cdft_interpolate_chunks_f64(const double* field, int nf, const int* chunk_node, int n_chunks,
+                                const int* base, const double* lw, long m, const int* shape, double h,
+                                const double* origin, double* out, void* stream);
+
 /* The transpose of interpolate: out[f] = add[f] + L^T x[f] (add may be NULL), deterministic and
  * atomic-free. Nodes must be numbered in base-cell order over the cell box (host: lo0, lo1, lo2,
  * c0, c1, c2). The walk is split into chunks of at most 64 consecutive nodes of one z-column; the

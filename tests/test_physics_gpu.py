@@ -114,6 +114,27 @@ def test_interpolate_and_lift_are_transposes(he_case, device):
     assert abs(lhs - rhs) < 1e-11 * abs(lhs)
 
 
+@pytest.mark.parametrize("nf", [1, 2, 3, 4])
+def test_chunked_interpolation_is_the_per_node_one_bitwise(he_case, h2_case, device, nf):
+    """One load of each base cell's 8^3 block per chunk of its nodes gives bitwise the per-node
+    interpolant, on the single-centre rule (a dense cluster of nodes at the nucleus) and on two
+    overlapping off-lattice spheres; every chunk holds nodes of one base cell."""
+    from cdft_amd import geometry as g
+
+    h2_grid, h2_factor = h2_case[2], h2_case[3]
+    for quad in (he_case[5], g.CuspQuadrature(h2_grid, h2_factor)):
+        bounds = quad._interp_node.cpu().numpy()
+        assert bounds[0] == 0 and bounds[-1] == quad.n_nodes and np.all(np.diff(bounds) > 0)
+        assert np.diff(bounds).max() <= quad.interp_chunk and quad._interp_chunks == bounds.size - 1
+        base = quad._base_host
+        for a, b in zip(bounds[:-1], bounds[1:]):
+            assert (base[a:b] == base[a]).all()
+        rng = np.random.default_rng(nf)
+        field = torch.as_tensor(rng.standard_normal((nf, quad.grid.n_points))).to(device)
+        chunked = quad.interpolate(field)
+        assert torch.equal(chunked, quad.interpolate(field, per_node=True))
+
+
 def test_poisson_matches_oracle_and_gaussian(he_case, device):
     from cdft_amd.poisson import CoulombCutoffPoisson
 
