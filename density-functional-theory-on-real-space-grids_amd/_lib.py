@@ -89,6 +89,7 @@ SIGNATURES: dict[str, list] = {
     "cdft_poisson_destroy": [P],
     "cdft_poisson_kernel_f64": [P, D, D, P, P],
     "cdft_poisson_solve_f64": [P, P, P, P, P, P, P, P, P],
+    "cdft_poisson_set_x_path": [I],
     "cdft_tpa_create": [I, I, I, P],
     "cdft_tpa_destroy": [P],
     "cdft_tpa_filter_f64": [P, D, P, P, P],

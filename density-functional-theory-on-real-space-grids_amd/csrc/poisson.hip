@@ -15,11 +15,33 @@
 // The zero rows y >= n1 and columns z >= n2 inside those slabs stay in the (y, z) transforms. The
 // result is the full 3-D transform's up to rounding. Buffers are owned by the caller: `padded_in` (n0
 // p1 p2 real, zero outside the box, never written by the transforms), `slabs`, `spectrum`, `padded_out`.
+#include <algorithm>
+
 #include "common.h"
 
 #include <hipfft/hipfft.h>
 
 namespace cdft {
+
+// The x transforms, the multiply by the kernel and the inverse x transforms of one solve as one
+// launch: each block takes `cols` adjacent (ky, kz) columns into LDS (their first n0 x entries; the
+// rest are the zero padding), runs a Stockham autosort FFT of length p0 over them (radix 4, 2, 3 and 5
+// stages, twiddles exp(-2 pi i j / p0) from sincospi), multiplies by K, runs the inverse FFT and
+// writes the first n0 x entries (the only ones the inverse (y, z) transforms read). Adjacent columns
+// are adjacent in memory, so the loads and stores of a block are coalesced across its columns; the
+// strided batched 1-D transforms it replaces read one column a transform.
+constexpr int XFFT_MAX_STAGES = 24;
+constexpr int XFFT_LDS_BYTES = 61440;
+
+struct XFFTPlan {
+  int n;       // transform length p0
+  int n0;      // occupied (and kept) x entries of a column
+  int ncols;   // columns: p1 (p2/2+1)
+  int cols;    // columns per block
+  int pitch;   // LDS doubles2 per column (n + 1: adjacent columns start in different banks)
+  int stages;
+  int radix[XFFT_MAX_STAGES];
+};
 
 struct PoissonPlan {
   hipfftHandle yz_forward;  // 2-D D2Z over (p1, p2), n0 slabs
@@ -28,7 +50,164 @@ struct PoissonPlan {
   hipfftHandle x_inverse;   // 1-D Z2Z along x, in place
   int n0, n1, n2;           // box
   int p0, p1, p2;           // padded
+  bool fused;               // the x transforms and the multiply run as one column kernel (xfft)
+  XFFTPlan xfft;
 };
+
+// Which x path cdft_poisson_solve_f64 takes: 0 = the column kernel where the plan has one (default),
+// 1 = the rocFFT strided transforms with a separate multiply (set by cdft_poisson_set_x_path).
+static int g_x_path = 0;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double f) { return make_double2(a.x * f, a.y * f); }
+// i^DIR z: DIR = -1 gives -i z, DIR = +1 gives i z
+template <int DIR>
+__device__ __forceinline__ double2 rot90(double2 z) {
+  return DIR < 0 ? make_double2(z.y, -z.x) : make_double2(-z.y, z.x);
+}
+
+// One Stockham stage of radix R over `cols` columns: with n = N / s the current length and m = n / R,
+// y[q + s (R p + k)] = w^(p k) sum_j x[q + s (p + j m)] exp(DIR 2 pi i j k / R), w = exp(DIR 2 pi i / n),
+// for p < m, q < s; w^(p k) = tw[p k s] (conjugated for DIR = +1), p k s < N.
+template <int DIR, int R>
+__device__ void xfft_stage_r(const double2* __restrict__ src, double2* __restrict__ dst,
+                             const double2* __restrict__ tw, int N, int pitch, int cols, int s) {
+  const int m = N / s / R;
+  const int per = N / R;
+  for (int b = threadIdx.x; b < cols * per; b += blockDim.x) {
+    const int c = b / per, t = b - c * per;
+    const int p = t / s, q = t - p * s;
+    const double2* x = src + c * pitch + q + s * p;
+    double2* y = dst + c * pitch + q + s * R * p;
+    double2 A[R];
+    if constexpr (R == 4) {
+      const double2 a0 = x[0], a1 = x[s * m], a2 = x[2 * s * m], a3 = x[3 * s * m];
+      const double2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = csub(a1, a3);
+      A[0] = cadd(t0, t2);
+      A[2] = csub(t0, t2);
+      A[1] = cadd(t1, rot90<DIR>(t3));
+      A[3] = csub(t1, rot90<DIR>(t3));
+    } else if constexpr (R == 2) {
+      const double2 a0 = x[0], a1 = x[s * m];
+      A[0] = cadd(a0, a1);
+      A[1] = csub(a0, a1);
+    } else if constexpr (R == 3) {
+      const double2 a0 = x[0], a1 = x[s * m], a2 = x[2 * s * m];
+      const double2 sm = cadd(a1, a2), df = csub(a1, a2);
+      const double2 mid = csub(a0, cscale(sm, 0.5));
+      const double2 e = rot90<DIR>(cscale(df, 0.86602540378443864676));  // sin(2 pi / 3)
+      A[0] = cadd(a0, sm);
+      A[1] = cadd(mid, e);
+      A[2] = csub(mid, e);
+    } else {
+      static_assert(R == 5, "radices 2, 3, 4 and 5");
+      const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;  // cos(2 pi/5), cos(4 pi/5)
+      const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;   // sin(2 pi/5), sin(4 pi/5)
+      const double2 a0 = x[0], a1 = x[s * m], a2 = x[2 * s * m], a3 = x[3 * s * m], a4 = x[4 * s * m];
+      const double2 s14 = cadd(a1, a4), d14 = csub(a1, a4), s23 = cadd(a2, a3), d23 = csub(a2, a3);
+      A[0] = cadd(a0, cadd(s14, s23));
+      const double2 b1 = cadd(a0, cadd(cscale(s14, c1), cscale(s23, c2)));
+      const double2 b2 = cadd(a0, cadd(cscale(s14, c2), cscale(s23, c1)));
+      const double2 e1 = rot90<DIR>(cadd(cscale(d14, s1), cscale(d23, s2)));
+      const double2 e2 = rot90<DIR>(csub(cscale(d14, s2), cscale(d23, s1)));
+      A[1] = cadd(b1, e1);
+      A[4] = csub(b1, e1);
+      A[2] = cadd(b2, e2);
+      A[3] = csub(b2, e2);
+    }
+    y[0] = A[0];
+#pragma unroll
+    for (int k = 1; k < R; ++k) {
+      double2 w = tw[p * k * s];
+      if (DIR > 0) w.y = -w.y;
+      y[s * k] = cmul(A[k], w);
+    }
+  }
+}
+
+template <int DIR>
+__device__ void xfft_stage(const double2* src, double2* dst, const double2* tw, int N, int pitch, int cols, int r,
+                           int s) {
+  switch (r) {
+    case 4: xfft_stage_r<DIR, 4>(src, dst, tw, N, pitch, cols, s); break;
+    case 2: xfft_stage_r<DIR, 2>(src, dst, tw, N, pitch, cols, s); break;
+    case 3: xfft_stage_r<DIR, 3>(src, dst, tw, N, pitch, cols, s); break;
+    default: xfft_stage_r<DIR, 5>(src, dst, tw, N, pitch, cols, s); break;
+  }
+}
+
+template <int DIR>
+__device__ double2* xfft_run(double2* a, double2* b, const double2* tw, const XFFTPlan& pl) {
+  int s = 1;
+  for (int st = 0; st < pl.stages; ++st) {
+    const int r = pl.radix[st];
+    xfft_stage<DIR>(a, b, tw, pl.n, pl.pitch, pl.cols, r, s);
+    __syncthreads();
+    double2* t = a;
+    a = b;
+    b = t;
+    s *= r;
+  }
+  return a;  // the buffer holding the result
+}
+
+__global__ __launch_bounds__(256) void xfft_convolve_kernel(const double2* __restrict__ slabs,
+                                                            const double* __restrict__ kernel,
+                                                            double2* __restrict__ spectrum, XFFTPlan pl) {
+  extern __shared__ double2 lds[];
+  const int N = pl.n, cols = pl.cols, pitch = pl.pitch;
+  double2* tw = lds;
+  double2* bufa = lds + N;
+  double2* bufb = bufa + cols * pitch;
+  const long col0 = (long)blockIdx.x * cols;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    double sn, cs;
+    sincospi(-2.0 * (double)j / (double)N, &sn, &cs);
+    tw[j] = make_double2(cs, sn);
+  }
+  // the first n0 entries of each column; the zero padding above them
+  for (int e = threadIdx.x; e < cols * N; e += blockDim.x) {
+    const int x = e / cols, c = e - x * cols;
+    const long col = col0 + c;
+    double2 v = make_double2(0.0, 0.0);
+    if (x < pl.n0 && col < pl.ncols) v = slabs[(long)x * pl.ncols + col];
+    bufa[c * pitch + x] = v;
+  }
+  __syncthreads();
+  double2* f = xfft_run<-1>(bufa, bufb, tw, pl);
+  double2* g = f == bufa ? bufb : bufa;
+  for (int e = threadIdx.x; e < cols * N; e += blockDim.x) {
+    const int x = e / cols, c = e - x * cols;
+    const long col = col0 + c;
+    if (col < pl.ncols) f[c * pitch + x] = cscale(f[c * pitch + x], kernel[(long)x * pl.ncols + col]);
+  }
+  __syncthreads();
+  double2* h = xfft_run<1>(f, g, tw, pl);
+  for (int e = threadIdx.x; e < cols * pl.n0; e += blockDim.x) {
+    const int x = e / cols, c = e - x * cols;
+    const long col = col0 + c;
+    if (col < pl.ncols) spectrum[(long)x * pl.ncols + col] = h[c * pitch + x];
+  }
+}
+
+// The radix sequence of a transform length (4s first, then 2, 3 and 5); false when another prime
+// divides it or it needs more stages than the plan holds.
+static bool xfft_factor(int n, XFFTPlan& pl) {
+  pl.stages = 0;
+  int r = n;
+  for (int f : {4, 2, 3, 5}) {
+    while (r % f == 0) {
+      if (pl.stages == XFFT_MAX_STAGES) return false;
+      pl.radix[pl.stages++] = f;
+      r /= f;
+    }
+  }
+  return r == 1;
+}
 
 __global__ void green_kernel(int p0, int p1, int p2, double h, double radius, double norm,
                              double* __restrict__ kernel) {
@@ -128,7 +307,24 @@ extern "C" int cdft_poisson_create(int n0, int n1, int n2, int p0, int p1, int p
     delete plan;
     return fail(CDFT_E_LIBRARY, "hipfftPlanMany for the (%d, %d, %d) padded box failed (plan %d)", p0, p1, p2, made);
   }
+  // the column kernel, where the length factors into radices 2-5 and one column fits the LDS budget;
+  // four columns a block (a sweep of 1-16 columns and 64-256 threads on the 180^3, 216^3 and
+  // 256 x 216^2 boxes: 114, 164 and 171 us a solve at four, against 133, 189 and 183 at sixteen)
+  XFFTPlan& x_plan = plan->xfft;
+  x_plan.n = p0;
+  x_plan.n0 = n0;
+  x_plan.ncols = column;
+  x_plan.pitch = p0 + 1;
+  const int budget = XFFT_LDS_BYTES / (int)sizeof(double2) - p0;  // after the twiddles
+  x_plan.cols = budget >= 2 * x_plan.pitch ? std::min(4, budget / (2 * x_plan.pitch)) : 0;
+  plan->fused = x_plan.cols >= 1 && xfft_factor(p0, x_plan);
   *handle = plan;
+  return CDFT_OK;
+}
+
+extern "C" int cdft_poisson_set_x_path(int which) {
+  if (which != 0 && which != 1) return fail(CDFT_E_ARG, "poisson_set_x_path: 0 (column kernel) or 1 (rocFFT)");
+  g_x_path = which;
   return CDFT_OK;
 }
 
@@ -173,13 +369,23 @@ extern "C" int cdft_poisson_solve_f64(void* handle, const double* density, doubl
   hipfftDoubleComplex* sp = (hipfftDoubleComplex*)spectrum;
   if (hipfftExecD2Z(pl->yz_forward, padded_in, sl) != HIPFFT_SUCCESS)
     return fail(CDFT_E_LIBRARY, "hipfftExecD2Z (y, z slabs) failed");
-  if (hipfftExecZ2Z(pl->x_forward, sl, sp, HIPFFT_FORWARD) != HIPFFT_SUCCESS)
-    return fail(CDFT_E_LIBRARY, "hipfftExecZ2Z (x, forward) failed");
-  hipLaunchKernelGGL(spectrum_multiply_kernel, dim3(nb(total)), dim3(256), 0, s, (double2*)spectrum, kernel, total);
-  rc = check_launch("poisson spectrum_multiply");
-  if (rc) return rc;
-  if (hipfftExecZ2Z(pl->x_inverse, sp, sp, HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
-    return fail(CDFT_E_LIBRARY, "hipfftExecZ2Z (x, inverse) failed");
+  if (pl->fused && g_x_path == 0) {
+    const XFFTPlan& xp = pl->xfft;
+    const unsigned blocks = (unsigned)((xp.ncols + xp.cols - 1) / xp.cols);
+    const size_t lds = (size_t)(xp.n + 2 * xp.cols * xp.pitch) * sizeof(double2);
+    hipLaunchKernelGGL(xfft_convolve_kernel, dim3(blocks), dim3(256), lds, s, (const double2*)sl, kernel, (double2*)sp,
+                       xp);
+    rc = check_launch("poisson xfft_convolve");
+    if (rc) return rc;
+  } else {
+    if (hipfftExecZ2Z(pl->x_forward, sl, sp, HIPFFT_FORWARD) != HIPFFT_SUCCESS)
+      return fail(CDFT_E_LIBRARY, "hipfftExecZ2Z (x, forward) failed");
+    hipLaunchKernelGGL(spectrum_multiply_kernel, dim3(nb(total)), dim3(256), 0, s, (double2*)spectrum, kernel, total);
+    rc = check_launch("poisson spectrum_multiply");
+    if (rc) return rc;
+    if (hipfftExecZ2Z(pl->x_inverse, sp, sp, HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
+      return fail(CDFT_E_LIBRARY, "hipfftExecZ2Z (x, inverse) failed");
+  }
   if (hipfftExecZ2D(pl->yz_inverse, sp, padded_out) != HIPFFT_SUCCESS)
     return fail(CDFT_E_LIBRARY, "hipfftExecZ2D (y, z slabs) failed");
   hipLaunchKernelGGL(gather_kernel, dim3(nb(n)), dim3(256), 0, s, padded_out, pl->n0, pl->n1, pl->n2, pl->p1, pl->p2,

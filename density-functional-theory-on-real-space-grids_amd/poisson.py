@@ -4,8 +4,8 @@ Mirrors ``CoulombCutoffPoisson`` (reference ``src/cdft/operators/poisson.py:27-1
 ``fft_friendly(ceil(pad * n))`` per axis, cutoff radius ``R = min(P_d h)/2``, kernel
 ``K(G) = 4 pi (1 - cos G R)/G^2`` with ``K(0) = 2 pi R^2``. The plan and the Green's-function table
 are built once per grid and spacing (and shared by later solvers of the same grid); a solve is
-pad-scatter, (y, z) D2Z on the occupied x-slabs, x transform, multiply, x transform, (y, z) Z2D on the
-kept x-slabs, gather (K5).
+pad-scatter, (y, z) D2Z on the occupied x-slabs, x transform, multiply, x transform (the three as one
+hand-written column kernel), (y, z) Z2D on the kept x-slabs, gather (K5).
 """
 
 from __future__ import annotations
@@ -30,6 +30,12 @@ def fft_friendly(n: int) -> int:
         if r == 1:
             return c
         c += 1
+
+
+def set_x_path(which: int) -> None:
+    """0: the x transforms and the multiply as one column kernel (default); 1: rocFFT's strided
+    transforms around a separate multiply. Both give the same potential to rounding."""
+    _lib.call("cdft_poisson_set_x_path", int(which))
 
 
 class _Plan:

@@ -474,9 +474,16 @@ int cdft_poisson_kernel_f64(void* handle, double h, double radius, double* kerne
  * transforms on the n0 occupied x-slabs only, the x transform on the whole padded spectrum. Caller
  * workspace: padded_in and padded_out (n0 p1 p2 doubles; padded_in zero outside the box), slabs and
  * spectrum (p0 p1 (p2/2+1) complex doubles; slabs zero beyond its first n0 x-slabs); the zero parts
- * are never written. Replaces CoulombCutoffPoisson.solve (src/cdft/operators/poisson.py:66-84). */
+ * are never written. The forward x transform, the multiply by kernel and the inverse x transform run
+ * as one hand-written column kernel (Stockham radix-2/3/4/5 FFTs in LDS) when p0 factors into 2, 3
+ * and 5 and a column fits its LDS budget, else as rocFFT strided transforms around a multiply.
+ * Replaces CoulombCutoffPoisson.solve (src/cdft/operators/poisson.py:66-84). */
 int cdft_poisson_solve_f64(void* handle, const double* density, double* padded_in, void* slabs, void* spectrum,
                            double* padded_out, const double* kernel, double* potential, void* stream);
+/* Which x path cdft_poisson_solve_f64 takes, for every later call in the process: 0 = the column
+ * kernel where the plan has one (default), 1 = the rocFFT transforms (same results to rounding).
+ * No counterpart in the reference: a switch for comparing the two paths. */
+int cdft_poisson_set_x_path(int which);
 
 /* ---- the LOBPCG cross-check's kinetic preconditioner (rocFFT through hipFFT) ---- */
 
