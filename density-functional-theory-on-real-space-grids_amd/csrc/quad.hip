@@ -489,13 +489,22 @@ __global__ void lift_combine_kernel(const double* __restrict__ zcol, CellBox cb,
     for (int j0 = 0; j0 < 8; ++j0) {
       const int b0 = i0 - j0 + 3 - cb.lo0;
       if (b0 < 0 || b0 >= cb.c0) continue;
+      // the row's eight loads issued together, then summed in j1 order; a column outside the box
+      // adds +0.0, which leaves the sum's bits as skipping it would (the sum starts at +0.0, so it
+      // is never -0.0)
+      double v[8][NF];
+#pragma unroll
       for (int j1 = 0; j1 < 8; ++j1) {
         const int b1 = i1 - j1 + 3 - cb.lo1;
-        if (b1 < 0 || b1 >= cb.c1) continue;
-        const long idx = (((long)b0 * cb.c1 + b1) * 64 + j0 * 8 + j1) * ez + iz;
+        const bool in = b1 >= 0 && b1 < cb.c1;
+        const long idx = in ? (((long)b0 * cb.c1 + b1) * 64 + j0 * 8 + j1) * ez + iz : 0L;
 #pragma unroll
-        for (int f = 0; f < NF; ++f) acc[f] += zcol[f * ncols * 64 * ez + idx];
+        for (int f = 0; f < NF; ++f) v[j1][f] = in ? zcol[f * ncols * 64 * ez + idx] : 0.0;
       }
+#pragma unroll
+      for (int j1 = 0; j1 < 8; ++j1)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) acc[f] += v[j1][f];
     }
   }
 #pragma unroll
