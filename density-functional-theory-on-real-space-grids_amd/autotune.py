@@ -27,9 +27,10 @@ from . import ops
 #: Planes per segment tried (0 = the built-in heuristic).
 CANDIDATES = (0, 4, 6, 8, 11, 16, 22, 33)
 
-#: (largest tile rows along y, tile width along z) tried at the best segment length ((0, 0) = the
-#: built-in choice: 16 rows at most, 13 or 16 columns by the fewer idle ones).
-TILES = ((0, 0), (16, 16), (16, 13), (13, 13), (12, 16), (8, 16))
+#: (largest tile rows along y, tile width along z) tried at the best segment length: (0, 0) is the
+#: built-in choice (for order 8 the vector kernel at the width with the fewest idle columns over the
+#: batch, at most 17 rows); widths 16-28 are the vector kernel's, 13 the column kernel's.
+TILES = ((0, 0), (16, 16), (12, 16), (0, 20), (0, 22), (11, 22), (0, 24), (0, 28), (9, 28), (16, 13), (13, 13))
 
 _cache: dict[tuple, dict] = {}
 

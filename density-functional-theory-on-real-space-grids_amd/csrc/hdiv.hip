@@ -942,6 +942,39 @@ __global__ __launch_bounds__(NTMAX) void hdiv_batched_kernel(const BatchGrid<T>*
                                 g.md, tile, v, lds);
 }
 
+// The order-8 batch on the vector kernel: the same grid search and buffer rotation, then exactly the
+// single-grid vector kernel's block (bitwise its steps, and so the column kernel's).
+template <typename T, int MODE, int TZ>
+__global__ __launch_bounds__(NTMAX) void hdiv_batched_vec_kernel(const BatchGrid<T>* __restrict__ grids, int n_grids,
+                                                                 int step, int degree) {
+  __shared__ VecLds<T, TZ> lds;
+  const unsigned lin = xcd_tile(blockIdx.x, gridDim.x);
+  int lo = 0, hi = n_grids - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (grids[mid].block_start <= lin) lo = mid; else hi = mid - 1;
+  }
+  const BatchGrid<T>& g = grids[lo];
+  const unsigned local = lin - g.block_start;
+  const unsigned tile = local / (unsigned)g.k;
+  const int v = (int)(local - tile * (unsigned)g.k);
+  Taps<4> taps;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) taps.c[i] = g.taps[i];
+  const T* y = g.x;
+  const T* prev = nullptr;
+  T* dst = g.out;
+  if (MODE != 0) {
+    auto buf = [&](int i) { return i == 0 ? g.out : (i == 1 ? g.work1 : g.work2); };
+    const int offset = ((-(degree - 1)) % 3 + 3) % 3;
+    dst = buf((step + offset) % 3);
+    y = step == 0 ? g.x : buf((step - 1 + offset) % 3);
+    prev = step <= 1 ? g.x : buf((step - 2 + offset) % 3);
+  }
+  stencil_block_vec<T, MODE, TZ>(y, prev, dst, g.w0, g.w1, g.w2, g.invw, g.pot, g.n0, g.n1, g.n2, taps, g.table, step,
+                                 g.md, tile, v, lds);
+}
+
 // The per-point geometry records the stencil kernels read each plane: for plane m and column (y, z) of
 // the face-extended rows (y < n1 + 2P, z < n2 + 2P), (w0 at extended plane m + 2P - 1, w1 at (m, y, z),
 // w2 at (m, y, z), invw), each part zero where the separate array has no such entry for this column.
@@ -1047,7 +1080,8 @@ static int column_tile_width(int n2) {
 // tz: the tile width (0 = the override of cdft_hdiv_set_tile, else column_tile_width).
 static March march_dims(int n0, int n1, int n2, int k, int reach, int tz = 0) {
   March d;
-  d.tz = tz ? tz : g_tile_z ? g_tile_z : column_tile_width(n2);
+  // (a vector-only width in the override is for batch plans: the column kernel takes its own choice)
+  d.tz = tz ? tz : (g_tile_z == 13 || g_tile_z == 16) ? g_tile_z : column_tile_width(n2);
   d.tiles_z = ceil_div(n2, d.tz);
   d.tiles_y = ceil_div(n1, g_tile_y ? g_tile_y : TMAX);
   d.ty = ceil_div(n1, d.tiles_y);
@@ -1104,28 +1138,14 @@ static bool vec_lookup(int n0, int n1, int n2, int k, int elem, VecSetting& out)
 // The vector kernel's launch: by default the tile width with the fewest idle columns (ties: the
 // wider), the largest tile rows up to VEC_TMAX_Y that split n1 evenly, and the segment count of the
 // CU-fill rule at the kernel's occupancy (three waves per SIMD in float64, two in float32).
-static March vec_dims(int n0, int n1, int n2, int k, int elem) {
-  VecSetting set{};
-  const bool tuned = vec_lookup(n0, n1, n2, k, elem, set);
+// The vector kernel's launch for tile width tz, tiles of at most ty_max rows and segments of seg
+// planes (0: the CU-fill rule).
+static March vec_dims_with(int n0, int n1, int n2, int k, int elem, int tz, int ty_max, int seg) {
   const int cz = 16 / elem;
   March d;
-  if (tuned) {
-    d.tz = set.tz;
-  } else {
-    int best = 0, idle = 1 << 30;
-    const int* w = elem == 8 ? VEC_TZ_F64 : VEC_TZ_F32;
-    const int nw = elem == 8 ? (int)std::size(VEC_TZ_F64) : (int)std::size(VEC_TZ_F32);
-    for (int i = 0; i < nw; ++i) {
-      const int id = ceil_div(n2, w[i]) * w[i] - n2;
-      if (id <= idle) {
-        idle = id;
-        best = w[i];
-      }
-    }
-    d.tz = best;
-  }
+  d.tz = tz;
   d.tiles_z = ceil_div(n2, d.tz);
-  d.tiles_y = ceil_div(n1, tuned ? set.ty : VEC_TMAX_Y);
+  d.tiles_y = ceil_div(n1, ty_max);
   d.ty = ceil_div(n1, d.tiles_y);
   const int tzv = d.tz / cz;
   const int reach = 7;
@@ -1133,14 +1153,35 @@ static March vec_dims(int n0, int n1, int n2, int k, int elem) {
   const int cover = d.ty * tzv;
   d.threads = ceil_div(std::max(cover, halo), 64) * 64;
   const int per_plane = d.tiles_y * d.tiles_z;
-  if (tuned && set.seg > 0)
-    d.nseg = ceil_div(n0, set.seg);
+  if (seg > 0)
+    d.nseg = ceil_div(n0, seg);
   else
     d.nseg = fill_segments(n0, (long)per_plane * k, d.threads, elem == 8 ? 12 : 8);
   d.nseg = std::max(1, std::min(d.nseg, n0));
   d.seg = ceil_div(n0, d.nseg);
   d.tiles = (unsigned)per_plane * d.nseg;
   return d;
+}
+
+// The vector width with the fewest idle columns over n2 (ties: the wider).
+static int vec_best_width(int n2, int elem) {
+  int best = 0, idle = 1 << 30;
+  const int* w = elem == 8 ? VEC_TZ_F64 : VEC_TZ_F32;
+  const int nw = elem == 8 ? (int)std::size(VEC_TZ_F64) : (int)std::size(VEC_TZ_F32);
+  for (int i = 0; i < nw; ++i) {
+    const int id = ceil_div(n2, w[i]) * w[i] - n2;
+    if (id <= idle) {
+      idle = id;
+      best = w[i];
+    }
+  }
+  return best;
+}
+
+static March vec_dims(int n0, int n1, int n2, int k, int elem) {
+  VecSetting set{};
+  if (vec_lookup(n0, n1, n2, k, elem, set)) return vec_dims_with(n0, n1, n2, k, elem, set.tz, set.ty, set.seg);
+  return vec_dims_with(n0, n1, n2, k, elem, vec_best_width(n2, elem), VEC_TMAX_Y, 0);
 }
 
 template <typename T, int MODE, int TZ>
@@ -1495,8 +1536,8 @@ extern "C" int cdft_hdiv_set_segment(int planes) {
 
 extern "C" int cdft_hdiv_set_tile(int tile_y, int tile_z) {
   if (tile_y < 0 || tile_y > TMAX) return fail(CDFT_E_ARG, "hdiv_set_tile: tile_y must be 0 (automatic) or 1..%d", TMAX);
-  if (tile_z != 0 && tile_z != 13 && tile_z != 16)
-    return fail(CDFT_E_ARG, "hdiv_set_tile: tile_z must be 0 (automatic), 13 or 16");
+  if (tile_z != 0 && tile_z != 13 && tile_z != 16 && !vec_width_ok(tile_z, 8))
+    return fail(CDFT_E_ARG, "hdiv_set_tile: tile_z must be 0 (automatic), 13, 16, 20, 22, 24 or 28");
   g_tile_y = tile_y;
   g_tile_z = tile_z;
   return CDFT_OK;
@@ -1579,6 +1620,28 @@ template <typename T, int P, int MODE>
 static int launch_batched_p(const void* desc, int n_grids, const int* launch, int step, int degree, hipStream_t s) {
   const dim3 grid((unsigned)launch[0]), block((unsigned)launch[1]);
   const BatchGrid<T>* g = (const BatchGrid<T>*)desc;
+  if constexpr (P == 4 && MODE <= 2) {
+    if (launch[2] < 0) {  // the vector kernel, tile width -launch[2]
+#define CDFT_BVEC(W)                                                                                               \
+  case W:                                                                                                          \
+    hipLaunchKernelGGL((hdiv_batched_vec_kernel<T, MODE, W>), grid, block, 0, s, g, n_grids, step, degree); \
+    return check_launch("hdiv_batched_vec_kernel");
+      switch (-launch[2]) {
+        CDFT_BVEC(16)
+        CDFT_BVEC(20)
+        CDFT_BVEC(24)
+        CDFT_BVEC(28)
+        case 22:
+          if constexpr (sizeof(T) == 8) {
+            hipLaunchKernelGGL((hdiv_batched_vec_kernel<T, MODE, 22>), grid, block, 0, s, g, n_grids, step, degree);
+            return check_launch("hdiv_batched_vec_kernel");
+          }
+          break;
+      }
+#undef CDFT_BVEC
+      return fail(CDFT_E_ARG, "batched: no vector kernel of tile width %d", -launch[2]);
+    }
+  }
   if (launch[2] == 13)
     hipLaunchKernelGGL((hdiv_batched_kernel<T, P, MODE, 13>), grid, block, 0, s, g, n_grids, step, degree);
   else
@@ -1600,7 +1663,8 @@ static int launch_batched(const void* desc, int n_grids, const int* launch, int 
 
 static int check_batch_launch(const void* desc, int n_grids, const int* launch, int order) {
   if (!desc || !launch || n_grids < 1) return fail(CDFT_E_ARG, "batched: null descriptor or no grids");
-  if (launch[0] < 1 || launch[1] < 64 || launch[1] > NTMAX || (launch[2] != 13 && launch[2] != 16))
+  const bool vec = launch[2] < 0 && order == 8 && vec_width_ok(-launch[2], 8);  // float32: 16, 20, 24, 28 too
+  if (launch[0] < 1 || launch[1] < 64 || launch[1] > NTMAX || (launch[2] != 13 && launch[2] != 16 && !vec))
     return fail(CDFT_E_ARG, "batched: launch {%d, %d, %d} is not one cdft_batch_plan wrote", launch[0], launch[1],
                 launch[2]);
   if (order != 2 && order != 4 && order != 6 && order != 8) return fail(CDFT_E_ARG, "batched: bad order %d", order);
@@ -1629,11 +1693,34 @@ static int batch_plan(int n_grids, const int* shapes, int k, int order, const do
   BatchGrid<T>* g = (BatchGrid<T>*)desc;
   unsigned long start = 0;
   int threads = 64;
-  // One launch runs one tile width: the override of cdft_hdiv_set_tile, else the width that leaves the
-  // fewest idle points over the whole batch (grids of different z extents may each prefer another;
-  // the outputs do not depend on the tiling).
-  int tz = g_tile_z;
-  if (tz == 0) {
+  // Order 8 runs on the vector kernel (unless cdft_hdiv_set_kernel(1) or a column-only tile width is
+  // in force): one vector width for the batch, the override of cdft_hdiv_set_tile or the width that
+  // leaves the fewest idle points over the whole batch; tile rows and segment lengths likewise.
+  const int elem = (int)sizeof(T);
+  const bool vec = order == 8 && g_kernel.load(std::memory_order_relaxed) == 0 &&
+                   (g_tile_z == 0 || vec_width_ok(g_tile_z, elem));
+  int vtz = g_tile_z;
+  if (vec && vtz == 0) {
+    const int* w = elem == 8 ? VEC_TZ_F64 : VEC_TZ_F32;
+    const int nw = elem == 8 ? (int)std::size(VEC_TZ_F64) : (int)std::size(VEC_TZ_F32);
+    long best_idle = -1;
+    for (int c = 0; c < nw; ++c) {
+      long idle = 0;
+      for (int i = 0; i < n_grids; ++i) {
+        const long n0 = shapes[3 * i], n1 = shapes[3 * i + 1], n2 = shapes[3 * i + 2];
+        idle += n0 * n1 * (ceil_div((int)n2, w[c]) * w[c] - n2);
+      }
+      if (best_idle < 0 || idle <= best_idle) {
+        best_idle = idle;
+        vtz = w[c];
+      }
+    }
+  }
+  // The column kernel: one launch runs one tile width, the override of cdft_hdiv_set_tile, else the
+  // width that leaves the fewest idle points over the whole batch (grids of different z extents may
+  // each prefer another; the outputs do not depend on the tiling).
+  int tz = (g_tile_z == 13 || g_tile_z == 16) ? g_tile_z : 0;
+  if (!vec && tz == 0) {
     long idle[2] = {0, 0};
     for (int i = 0; i < n_grids; ++i) {
       const long n0 = shapes[3 * i], n1 = shapes[3 * i + 1], n2 = shapes[3 * i + 2];
@@ -1647,8 +1734,10 @@ static int batch_plan(int n_grids, const int* shapes, int k, int order, const do
     if (n0 < 1 || n1 < 1 || n2 < 1) return fail(CDFT_E_ARG, "batch_plan: grid %d has a bad shape", i);
     if ((long)(n0 + order) * (n1 + order) * (n2 + order) * 4 * (long)sizeof(T) >= 0x7fffffffL)
       return fail(CDFT_E_ARG, "batch_plan: grid %d too large for 32-bit offsets", i);
-    const March d = march_dims(n0, n1, n2, k, order - 1, tz);
-    if (d.threads > NTMAX) return fail(CDFT_E_ARG, "batch_plan: grid %d needs %d threads", i, d.threads);
+    const March d = vec ? vec_dims_with(n0, n1, n2, k, elem, vtz, g_tile_y ? g_tile_y : VEC_TMAX_Y, g_segment)
+                        : march_dims(n0, n1, n2, k, order - 1, tz);
+    if (d.threads > NTMAX || (vec && d.ty > VEC_TMAX_Y))
+      return fail(CDFT_E_ARG, "batch_plan: grid %d needs %d threads", i, d.threads);
     threads = d.threads > threads ? d.threads : threads;
     const void* const* p = ptrs + 11 * i;
     for (int j = 0; j < 11; ++j)
@@ -1677,7 +1766,7 @@ static int batch_plan(int n_grids, const int* shapes, int k, int order, const do
   }
   launch[0] = (int)start;
   launch[1] = threads;
-  launch[2] = tz;
+  launch[2] = vec ? -vtz : tz;
   return CDFT_OK;
 }
 

@@ -286,7 +286,8 @@ def hdiv_set_segment(planes: int) -> None:
 
 def hdiv_set_tile(tile_y: int = 0, tile_z: int = 0) -> None:
     """Tile extents of the marching stencil kernels: the largest along y (1..16) and the width along
-    z (13 or 16); 0 restores the automatic choice. Outputs do not depend on them."""
+    z (13 or 16 for the column kernel; 16, 20, 22, 24 or 28 select the vector kernel of an order-8
+    batch plan); 0 restores the automatic choice. Outputs do not depend on them."""
     _lib.call("cdft_hdiv_set_tile", int(tile_y), int(tile_z))
 
 

@@ -125,7 +125,9 @@ long cdft_batch_desc_bytes(int n_grids);
  * device unchanged) for n_grids grids of one stencil order and block width k. shapes (host,
  * n_grids x 3), coefs (host, n_grids x order staggered weights / h), ptrs (host, n_grids x 11 device
  * pointers: x, out, work1, work2, w0, w1, w2, invw, pot, geometry records, Chebyshev table (degree x 3)). launch (host,
- * 3 ints) receives {blocks, threads, tile width}; every grid must get the same tile width. */
+ * 3 ints) receives {blocks, threads, tile}: tile > 0 is the column kernel's width (13 or 16), tile < 0
+ * the vector kernel's (-16, -20, -22, -24 or -28), one for every grid of the batch. Order 8 runs on
+ * the vector kernel unless cdft_hdiv_set_kernel(1) or a column-only width (13) is in force. */
 int cdft_batch_plan_f64(int n_grids, const int* shapes, int k, int order, const double* coefs,
                         const void* const* ptrs, void* desc, long desc_bytes, int* launch);
 /* K2 for every grid of a batch: each grid's degree-`degree` filter of x into out through work1 /
@@ -223,8 +225,9 @@ int cdft_cheb_table_f64(const double* evals, int index, const double* bounds, in
 int cdft_hdiv_set_segment(int planes);
 
 /* Tile extents of the marching stencil kernels for later launches and batch plans (0 = automatic):
- * the largest tile extent along y, 1..16 (the tiles then split n1 evenly), and the tile width along z,
- * 13 or 16. Outputs are bitwise independent of the tiling (and of the segment length); a launch
+ * the largest tile extent along y, 1..16 (the tiles then split n1 evenly), and the tile width along z:
+ * 13 or 16 for the column kernel, and for an order-8 batch plan also a vector-kernel width (16, 20,
+ * 22, 24 or 28). Outputs are bitwise independent of the tiling (and of the segment length); a launch
  * tuning knob with no counterpart in the reference, whose stencil is torch slicing
  * (src/cdft/operators/divergence.py:84-197). Process-wide, like cdft_hdiv_set_segment. */
 int cdft_hdiv_set_tile(int tile_y, int tile_z);

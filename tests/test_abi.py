@@ -84,3 +84,47 @@ def test_vector_launch_shape_and_settings_are_host_only(pkg):
         ops.hdiv_vec_set((65, 65, 65), 5, torch.float32, 11, 22, 0)
     with pytest.raises(pkg._lib.HipLibraryError):
         ops.hdiv_set_kernel(2)
+
+
+def test_batch_plan_picks_the_vector_kernel_for_order_8(pkg):
+    """A batch plan is host-only (it writes descriptors and reads no device memory, so placeholder
+    addresses do): order 8 runs on the vector kernel (launch tile < 0) at the width with the fewest
+    idle columns over the batch, a column-only width or the column-kernel switch selects the column
+    kernel, other orders stay on it, and a vector-only width never reaches a column launch."""
+    import ctypes
+
+    from cdft_amd import _lib
+
+    ops = pkg.ops
+    lib = _lib.load()
+    shapes_list = [(61, 57, 57), (71, 57, 57), (81, 57, 57)]
+
+    def plan(order=8):
+        n = len(shapes_list)
+        shapes = (ctypes.c_int * (3 * n))(*[v for s in shapes_list for v in s])
+        coefs = (ctypes.c_double * (order * n))(*([0.5] * (order * n)))
+        ptrs = (ctypes.c_void_p * (11 * n))(*([0x100000] * (11 * n)))
+        nbytes = int(lib.cdft_batch_desc_bytes(n))
+        host = (ctypes.c_uint8 * nbytes)()
+        launch = (ctypes.c_int * 3)()
+        _lib.call("cdft_batch_plan_f64", n, ctypes.addressof(shapes), 5, order, ctypes.addressof(coefs),
+                  ctypes.addressof(ptrs), ctypes.addressof(host), nbytes, ctypes.addressof(launch))
+        return list(launch)
+
+    blocks, threads, tile = plan()
+    assert tile == -20 and threads % 64 == 0 and blocks > 0  # 57 = 3 x 20 - 3: the fewest idle columns
+    assert plan(order=6)[2] in (13, 16)
+    try:
+        ops.hdiv_set_tile(0, 13)
+        assert plan()[2] == 13
+        ops.hdiv_set_tile(0, 28)
+        assert plan()[2] == -28
+        assert ops.hdiv_launch_shape((61, 57, 57), 5)["tile_z"] in (13, 16)  # the column kernel's own choice
+        ops.hdiv_set_tile(0, 0)
+        ops.hdiv_set_kernel(1)
+        assert plan()[2] in (13, 16)
+    finally:
+        ops.hdiv_set_kernel(0)
+        ops.hdiv_set_tile(0, 0)
+    with pytest.raises(pkg._lib.HipLibraryError):
+        ops.hdiv_set_tile(0, 21)
