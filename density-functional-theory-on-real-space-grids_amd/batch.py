@@ -19,9 +19,7 @@ stopped, while the batch keeps iterating until every grid has.
 
 from __future__ import annotations
 
-import concurrent.futures
 import ctypes
-import os
 import time
 
 import numpy as np
@@ -30,7 +28,7 @@ import torch
 from . import _lib, ops
 from .eigen import BOUND_REUSE_RITZ_FRACTION, EigenConfig, RitzWork, _fraction  # noqa: F401
 from .chefsi import scaled_chebyshev_table
-from .hamiltonian import lanczos_finish, lanczos_launch
+from .hamiltonian import lanczos_finish, lanczos_launch, seeded_starts
 from .ops import stream_of
 
 
@@ -83,15 +81,6 @@ class BatchPlan:
     def apply(self, stream_tensor: torch.Tensor) -> None:
         _lib.call(f"cdft_hdiv_apply_batched_{self.suffix}", self.desc.data_ptr(), self.n,
                   ctypes.addressof(self.launch), self.order, stream_of(stream_tensor))
-
-
-def _seeded_start(seed: int, k: int, n: int) -> tuple[torch.Tensor, torch.Tensor]:
-    """The draws a single-grid solve makes from ``torch.Generator().manual_seed(seed)``: the (k, n)
-    start block, then the (n,) Lanczos start (reference ``precision.py:246-263``)."""
-    gen = torch.Generator(device="cpu").manual_seed(seed)
-    block = torch.randn((k, n), generator=gen, dtype=torch.float64)
-    start = torch.randn((n,), generator=gen, dtype=torch.float64)
-    return block, start
 
 
 class _Instance:
@@ -181,19 +170,19 @@ class BatchedCheFSI:
             work.flag = flags[i: i + 1]
             work.table = tables[i]
             insts.append(_Instance(h, work, self.n_states))
-        # Start: the reference's seeded draws per grid (block, then the Lanczos start, from a fresh
-        # CPU generator each), made on host threads in parallel; every grid's Lanczos run enqueued
-        # without a read, then one transfer for all of them; the first Rayleigh-Ritz per grid.
+        # Start: the reference's seeded draws per grid (the (k, n) block, then the Lanczos start, from
+        # a fresh CPU generator each, precision.py:246-263; the blocks' common prefix drawn once,
+        # hamiltonian.seeded_starts); every grid's Lanczos run enqueued without a read, then one
+        # transfer for all of them; the first Rayleigh-Ritz per grid.
         ta = time.perf_counter()
         sizes = [it.h.grid.n_points for it in insts]
-        with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1))) as pool:
-            draws = list(pool.map(lambda npts: _seeded_start(self.seed, k, npts), sizes))
+        draws = seeded_starts(self.seed, k, sizes, dev)
         t_draw = time.perf_counter() - ta
         tb = time.perf_counter()
         lanczos = []
-        for it, (block_h, start_h) in zip(insts, draws):
-            it.block = block_h.to(dev).contiguous()
-            lanczos.append(lanczos_launch(it.h, start_h.to(dev), cfg.lanczos_steps))
+        for it, (block, start) in zip(insts, draws):
+            it.block = block
+            lanczos.append(lanczos_launch(it.h, start, cfg.lanczos_steps))
         values = torch.stack(lanczos).cpu().numpy()
         for it, vals in zip(insts, values):
             it.smin, it.smax = lanczos_finish(vals, cfg.lanczos_steps, cfg.bound_safety)

@@ -133,34 +133,30 @@ def _layout_verified() -> bool:
     return _state_layout_ok
 
 
-def parallel_randn(seed: int, sizes) -> list[torch.Tensor]:
-    """``[torch.randn(s, generator=g, dtype=float64) for s in sizes]`` with ``g`` freshly seeded
-    ``seed``, bitwise, drawn in chunks on host threads into pinned memory.
+def randn_at(seed: int, jobs) -> list[torch.Tensor]:
+    """For each job ``(u0, n)`` (``n >= 16``): the ``n`` float64 normals ``torch.randn(n)`` gives from a
+    generator seeded ``seed`` whose next uniform is the ``u0``-th of its stream, bitwise, drawn in
+    chunks on host threads into pinned memory.
 
     torch fills ``n >= 16`` float64 normals by drawing ``n`` uniforms (two 32-bit outputs each),
     transforming every full group of 16 on its own, and, when 16 does not divide ``n``, drawing 16
     more uniforms for the last 16 values. A chunk of whole groups drawn from the generator state at
     its first uniform is therefore bitwise that part of the serial draw; the states come from
-    ``cdft_mt19937_states``. Sizes below 16 (a different torch path) or a generator layout this
-    code does not recognise fall back to the serial draw."""
-    sizes = [tuple(s) if not isinstance(s, int) else (s,) for s in sizes]
-    counts = [int(np.prod(s)) for s in sizes]
-    pinned = torch.cuda.is_available()
-    if min(counts, default=16) < 16 or not _layout_verified():
-        gen = torch.Generator().manual_seed(int(seed))
-        return [torch.randn(s, generator=gen, dtype=torch.float64) for s in sizes]
+    ``cdft_mt19937_states``. The caller checks :func:`_layout_verified` first."""
     _, chunk_pool = _pools()
-    plan = []  # (draw, lo, hi, uniform offset of lo) per chunk, then the tails
+    pinned = torch.cuda.is_available()
+    plan = []  # (job, lo, hi, uniform offset of lo) per chunk, then the tails
     tails = []
-    uniform = 0
-    for d, n in enumerate(counts):
+    counts = [int(n) for _, n in jobs]
+    for d, (u0, n) in enumerate(jobs):
+        if n < 16:
+            raise ValueError("randn_at draws 16 values or more a job")
         full = n - n % 16
         for lo in range(0, full, _CHUNK_VALUES):
             hi = min(full, lo + _CHUNK_VALUES)
-            plan.append((d, lo, hi, uniform + lo))
+            plan.append((d, lo, hi, int(u0) + lo))
         if n % 16:
-            tails.append((d, uniform + n))
-        uniform += n + (16 if n % 16 else 0)
+            tails.append((d, int(u0) + n))
     positions = sorted({2 * u for *_, u in plan} | {2 * u for _, u in tails})
     key, index = _states(seed, positions)
     where = {p: i for i, p in enumerate(positions)}
@@ -181,7 +177,74 @@ def parallel_randn(seed: int, sizes) -> list[torch.Tensor]:
         f.result()
     for d, values in tail_values:
         outs[d][counts[d] - 16:] = values
-    return [o.view(s) for o, s in zip(outs, sizes)]
+    return outs
+
+
+def _consumed(n: int) -> int:
+    """Uniforms a ``torch.randn(n)`` float64 draw (``n >= 16``) takes from the generator."""
+    return n + (16 if n % 16 else 0)
+
+
+def parallel_randn(seed: int, sizes) -> list[torch.Tensor]:
+    """``[torch.randn(s, generator=g, dtype=float64) for s in sizes]`` with ``g`` freshly seeded
+    ``seed``, bitwise, drawn in chunks on host threads into pinned memory (:func:`randn_at`). Sizes
+    below 16 (a different torch path) or a generator layout this code does not recognise fall back
+    to the serial draw."""
+    sizes = [tuple(s) if not isinstance(s, int) else (s,) for s in sizes]
+    counts = [int(np.prod(s)) for s in sizes]
+    if min(counts, default=16) < 16 or not _layout_verified():
+        gen = torch.Generator().manual_seed(int(seed))
+        return [torch.randn(s, generator=gen, dtype=torch.float64) for s in sizes]
+    jobs, uniform = [], 0
+    for n in counts:
+        jobs.append((uniform, n))
+        uniform += _consumed(n)
+    return [o.view(s) for o, s in zip(randn_at(seed, jobs), sizes)]
+
+
+def seeded_starts(seed: int, k: int, sizes, device) -> list[tuple[torch.Tensor, torch.Tensor]]:
+    """For each grid size ``n`` of ``sizes``: the ``(k, n)`` start block and the ``(n,)`` Lanczos start a
+    solve draws from a fresh generator seeded ``seed`` (``torch.randn((k, n))`` then
+    ``torch.randn(n)``), on ``device``, bitwise.
+
+    Every grid's block is a draw from the start of the same stream, so all blocks share its full
+    groups of 16: that common prefix (as long as the longest block's full groups) is drawn and
+    uploaded once, and each grid's block copies its part on the device and adds its own tail group.
+    Only the Lanczos starts, which begin at a position that depends on the size, are drawn per
+    grid. Sizes whose draws fall below 16 values take the serial draw."""
+    sizes = [int(n) for n in sizes]
+    if not sizes:
+        return []
+    if min(sizes) < 16 or not _layout_verified():
+        out = []
+        for n in sizes:
+            gen = torch.Generator().manual_seed(int(seed))
+            block = torch.randn((k, n), generator=gen, dtype=torch.float64)
+            start = torch.randn((n,), generator=gen, dtype=torch.float64)
+            out.append((block.to(device), start.to(device)))
+        return out
+    blocks = [k * n for n in sizes]
+    prefix = max(b - b % 16 for b in blocks)
+    jobs = [(0, prefix)] if prefix >= 16 else []
+    tail_of, start_of = {}, {}
+    for i, (n, b) in enumerate(zip(sizes, blocks)):
+        if b % 16:
+            tail_of[i] = len(jobs)
+            jobs.append((b, 16))
+        start_of[i] = len(jobs)
+        jobs.append((_consumed(b), n))
+    drawn = randn_at(seed, jobs)
+    common = drawn[0].to(device, non_blocking=True) if prefix >= 16 else None
+    out = []
+    for i, (n, b) in enumerate(zip(sizes, blocks)):
+        block = torch.empty(b, dtype=torch.float64, device=device)
+        full = b - b % 16
+        if full:
+            block[:full].copy_(common[:full])
+        if i in tail_of:
+            block[b - 16:].copy_(drawn[tail_of[i]].to(device, non_blocking=True))
+        out.append((block.view(k, n), drawn[start_of[i]].to(device, non_blocking=True)))
+    return out
 
 
 def prefetch_draws(store: dict, seed: int, sequences) -> None:

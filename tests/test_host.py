@@ -43,6 +43,25 @@ def test_chunked_draws_are_the_serial_draws(pkg, monkeypatch):
     assert all(torch.equal(a, b) for a, b in zip(with_small, _torch_sequence(3, [(40,), (7,)])))
 
 
+def test_batch_starts_share_the_block_prefix_bitwise(pkg, monkeypatch):
+    """The batched solver's start vectors: for every grid size the (k, n) block and the (n,) Lanczos
+    start equal a fresh generator's serial draws, although the blocks' common prefix is drawn once
+    (sizes with and without a partial last group, k = 1 and 5, chunks that straddle twists)."""
+    from cdft_amd import hamiltonian as hm
+
+    monkeypatch.setattr(hm, "_CHUNK_VALUES", 64)
+    for seed, k, sizes in ((0, 5, [17, 35, 100, 1000, 3111]), (2**32 + 3, 1, [16, 17, 31, 33, 1025]),
+                           (9, 3, [250, 343])):
+        got = hm.seeded_starts(seed, k, sizes, torch.device("cpu"))
+        for n, (block, start) in zip(sizes, got):
+            expect_block, expect_start = _torch_sequence(seed, [(k, n), (n,)])
+            assert torch.equal(block, expect_block) and torch.equal(start, expect_start), (seed, k, n)
+    small = hm.seeded_starts(1, 1, [7, 40], torch.device("cpu"))  # below 16: the serial draw
+    for n, (block, start) in zip([7, 40], small):
+        expect_block, expect_start = _torch_sequence(1, [(1, n), (n,)])
+        assert torch.equal(block, expect_block) and torch.equal(start, expect_start)
+
+
 def test_mersenne_states_match_the_stream(pkg):
     """The states the library reports continue the stream exactly: numbers drawn from a generator
     placed at position p equal the numbers after p outputs of the seeded generator."""
