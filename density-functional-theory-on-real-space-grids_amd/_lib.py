@@ -58,6 +58,9 @@ SIGNATURES: dict[str, list] = {
     "cdft_rotate_gram_f64": [P, I, P, P, P, L, P, L, P, P],
     "cdft_hdiv_resnorm_f64": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, L, P, P],
     "cdft_hdiv_resnorm_tail_f64": [P, P, P, P, P, P, P, P, P, I, I, I, I, P, I, P, L, P, I, P, I, P, P, P, P],
+    "cdft_local_apply_f64": [P, P, P, I, I, I, I, P, I, I, P],
+    "cdft_local_filter_f64": [P, P, P, P, P, I, I, I, I, P, I, I, P, I, P],
+    "cdft_local_resnorm_f64": [P, P, P, P, I, I, I, I, P, I, I, P, L, P, I, P, I, P, P, P, P],
     "cdft_cusp_grid_fields_f64": [I, P, P, P, D, P, D, P, P, D, P, P, P, P, P],
     "cdft_face_weights_f64": [I, P, P, P, D, P, I, P, P, P, P, P],
     "cdft_node_fields_f64": [P, P, P, L, I, P, P, P, D, P, P, P, P, P, P],
@@ -102,6 +105,7 @@ VALUED = {
     "cdft_reduce_workspace_doubles": (ctypes.c_long, [L]),
     "cdft_batch_desc_bytes": (ctypes.c_long, [I]),
     "cdft_hdiv_pack_len": (ctypes.c_long, [I, I, I, I]),
+    "cdft_local_partials_len": (ctypes.c_long, [I, I, I, I]),
 }
 
 

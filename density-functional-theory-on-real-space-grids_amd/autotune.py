@@ -190,6 +190,8 @@ def tune_for_solve(geom, k: int) -> bool:
     """Count a solve of ``(geom.shape, k, geom.dtype)`` and tune it (:func:`tune_stencil`) on the
     :data:`TUNE_ON_SOLVE`-th. True when this call changed the launch settings in force: graphs
     captured before it still hold the old launches and should be recaptured."""
+    if not hasattr(geom, "packed"):
+        return False  # the model systems' plain operator has one launch shape
     signature = (tuple(geom.shape), k, geom.dtype, geom.device)
     if not ENABLED or signature in _stencil_cache:
         return False

@@ -1,5 +1,5 @@
 // Device bodies of the small kernels that end a Rayleigh-Ritz pass, shared by their own launches
-// (blas.hip) and by the fused residual-norm tail (hdiv.hip), so both give bitwise the same values.
+// (blas.hip) and by the fused residual-norm tails (hdiv.hip, local.hip), so all give bitwise the same values.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -252,6 +252,22 @@ class ChebyshevFilteredSubspace:
         The last bound arrays stay referenced, so their addresses cannot be handed to new arrays."""
         geom = hamiltonian.geometry
         q = hamiltonian.measure.q
+        if isinstance(geom, ops.LocalGeometry):
+            # the plain operator of the model systems: its weights are kernel arguments (captured by
+            # value), its potential and measure device arrays; float64 filters only
+            if self.mixed:
+                raise ValueError("the plain model-system operator filters in float64 only")
+            layout = ("local", geom.shape, geom.order, geom.boundary, tuple(geom.coef), geom.device)
+            if layout != self._layout:
+                self._layout = layout
+                self._graphs.clear()
+                self._geom64 = geom.with_potential(self._pot)
+                self._measure = DeviceMeasure(self.grid, q.clone())
+            elif self._sources is None or self._sources[0] is not q:
+                self._measure.q.copy_(q)
+            self._sources = (q,)
+            self._pot.copy_(hamiltonian.potential)
+            return
         sources = (geom.w0, geom.w1, geom.w2, geom.invw, geom.packed, q)
         if self._sources is None or any(a is not b for a, b in zip(sources, self._sources)):
             layout = (geom.shape, geom.order, tuple(geom.taps), geom.device, geom.dtype)

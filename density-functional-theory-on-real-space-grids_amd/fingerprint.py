@@ -15,7 +15,7 @@ float64 solves of one discretisation differ by summation order only), an occupat
 judged. ``--exact`` fails on any difference at all.
 
 The analytic model systems of the reference's registry (``harmonic_w1``, ``box_L10``) have no nuclei:
-this path solves the cusp-factorised Coulomb operator only, and records them as unsupported.
+they are solved with the plain operator ``-1/2 lap + V`` (``model.py``) and recorded the same way.
 """
 
 from __future__ import annotations
@@ -65,9 +65,10 @@ REGISTRY: dict[str, Scenario] = {s.scenario_id: s for s in (
     _atom("h_atom_N0.5_lda", 1, 0.5, "lda"),
 )}
 
-#: Registered in the reference, not solvable on this path (no nuclei, no cusp factor).
-UNSUPPORTED = {"harmonic_w1": "harmonic well: a model potential without nuclei",
-               "box_L10": "infinite well: a hard-wall model system without nuclei"}
+#: The model systems of the reference's registry (no nuclei: the plain operator of ``model.py``).
+MODEL_IDS = ("harmonic_w1", "box_L10")
+#: Registered in the reference and not solvable here (none at present).
+UNSUPPORTED: dict[str, str] = {}
 
 # The reference's sizing constants (``grid.py:61-87``).
 SINGLE_CENTRE_HALF_BOX_PER_Z = 16.0
@@ -203,6 +204,38 @@ def record(scenario: Scenario, device: torch.device, numerics: Numerics | None =
     }
 
 
+def model_record(scenario_id: str, device: torch.device) -> dict:
+    """Solve one model system (``model.solve_model``) and return its fingerprint entry."""
+    from . import model
+
+    started = time.perf_counter()
+    r = model.solve_model(scenario_id, device)
+    system, grid = r["system"], r["grid"]
+    vectors = r["eigenvectors"]
+    n_states = int(vectors.shape[0])
+    occ = np.asarray(r["occupations"], dtype=np.float64)[:n_states]
+    density = (torch.as_tensor(occ, dtype=torch.float64, device=vectors.device)[:, None] * vectors * vectors).sum(0)
+    energies = dict.fromkeys(ENERGY_KEYS, 0.0)
+    energies.update({"kinetic": r["kinetic"], "external": r["external"], "total": r["total"],
+                     "harris_foulkes": r["band"]})
+    rec = grid_record(grid)
+    rec["boundary"] = "odd_reflection" if system.boundary == 1 else "zero"
+    peak = torch.cuda.max_memory_allocated(device) if device.type == "cuda" else None
+    return {
+        "status": "valid" if r["converged"] else "unconverged",
+        "energies": {k: repr(float(energies[k])) for k in ENERGY_KEYS},
+        "eigenvalues": [repr(float(x)) for x in r["eigenvalues"]],
+        "occupations": [repr(float(x)) for x in occ],
+        "density_sha256": _digest(density.reshape(1, -1)),
+        "orbitals_sha256": _digest(vectors.reshape(1, n_states, -1)),
+        "grid": rec,
+        "n_iterations": int(r["n_iterations"]),
+        "n_states": n_states,
+        "wall_time_s": round(time.perf_counter() - started, 3),
+        "gpu_peak_bytes": int(peak) if peak is not None else None,
+    }
+
+
 def _write(target: pathlib.Path, data: dict) -> None:
     target.parent.mkdir(parents=True, exist_ok=True)
     partial = target.with_suffix(target.suffix + ".partial")
@@ -216,8 +249,8 @@ def fingerprint(device: str | torch.device = "cuda", scenario_ids=None, target: 
     dev = torch.device(device)
     if dev.type != "cuda" or not torch.cuda.is_available():
         raise RuntimeError("the fingerprint solves on the GPU (the HIP kernels); no GPU is visible")
-    names = list(REGISTRY) + list(UNSUPPORTED) if scenario_ids is None else list(scenario_ids)
-    unknown = [s for s in names if s not in REGISTRY and s not in UNSUPPORTED]
+    names = list(MODEL_IDS) + list(REGISTRY) + list(UNSUPPORTED) if scenario_ids is None else list(scenario_ids)
+    unknown = [s for s in names if s not in REGISTRY and s not in UNSUPPORTED and s not in MODEL_IDS]
     if unknown:
         raise SystemExit(f"unknown scenario id(s): {', '.join(unknown)}")
     out: dict = {"_meta": {"device_requested": str(device), "device": str(dev),
@@ -229,6 +262,8 @@ def fingerprint(device: str | torch.device = "cuda", scenario_ids=None, target: 
         print(f"[fingerprint] {sid} on {dev} ...", flush=True)
         if sid in UNSUPPORTED:
             out[sid] = {"status": "unsupported", "error": UNSUPPORTED[sid]}
+        elif sid in MODEL_IDS:
+            out[sid] = model_record(sid, dev)
         else:
             out[sid] = record(REGISTRY[sid], dev)
         print(f"[fingerprint] {sid}: {out[sid]['status']}", flush=True)

@@ -222,6 +222,58 @@ class StencilGeometry:
                                cast(self.invw), cast(self.potential), self.taps)
 
 
+@dataclass
+class LocalGeometry:
+    """The plain operator ``-1/2 lap + V`` of the analytic model systems (reference
+    ``operators/hamiltonian.py:92-125``): central second-derivative weights ``c_j / h^2`` of order
+    ``order`` as host doubles (``coef``), the halo's ``boundary`` (0 = zero, 1 = odd reflection about
+    walls one spacing outside the outermost points) and the diagonal ``potential``. Float64 only; the
+    solver's calls that take a :class:`StencilGeometry` take this one too."""
+
+    shape: tuple[int, int, int]
+    order: int
+    boundary: int
+    coef: ctypes.Array
+    potential: torch.Tensor
+
+    dtype = torch.float64
+
+    @property
+    def device(self) -> torch.device:
+        return self.potential.device
+
+    @property
+    def n_points(self) -> int:
+        n0, n1, n2 = self.shape
+        return n0 * n1 * n2
+
+    def with_potential(self, potential: torch.Tensor) -> "LocalGeometry":
+        return LocalGeometry(self.shape, self.order, self.boundary, self.coef, potential)
+
+
+def local_coefficients(order: int, spacing: float) -> ctypes.Array:
+    """``c_j * (1 / h^2)`` for the order-``order`` central second derivative (the reference scales each
+    weight by ``1 / (h h)`` the same way, ``laplacian.py:96-104``)."""
+    from .stencil import central_coefficients
+
+    scale = 1.0 / (spacing * spacing)
+    taps = central_coefficients(2, order)
+    return (ctypes.c_double * len(taps))(*[c * scale for c in taps])
+
+
+def _local_partials(geom: LocalGeometry, k: int) -> torch.Tensor:
+    n0, n1, n2 = geom.shape
+    need = int(_lib.load().cdft_local_partials_len(n0, n1, n2, k))
+    key = ("local",) + workspace_key(geom.device)
+    ws = _stencil_partials.get(key)
+    if ws is None or ws.numel() < need:
+        if ws is not None:
+            _retired.append(ws)  # a captured graph may still address it
+        ws = torch.empty(max(need, 4096), dtype=torch.float64, device=geom.device)
+        _stencil_partials[key] = ws
+    return ws
+
+
 def pack_records(geom: "StencilGeometry") -> torch.Tensor:
     """The per-point geometry records (w0 at the plane's newest x-face, w1, w2, invw) over the
     face-extended (y, z) rows that the stencil kernels read each plane (``cdft_hdiv_pack_*``)."""
@@ -254,6 +306,10 @@ def hdiv_apply(phi: torch.Tensor, geom: StencilGeometry, out: torch.Tensor | Non
         out = torch.empty_like(block)
     require(out, (k, geom.n_points), geom.dtype, geom.device, "out")
     n0, n1, n2 = geom.shape
+    if isinstance(geom, LocalGeometry):
+        _lib.call("cdft_local_apply_f64", block.data_ptr(), geom.potential.data_ptr(), out.data_ptr(), k, n0, n1, n2,
+                  ctypes.addressof(geom.coef), geom.order, geom.boundary, stream_of(block))
+        return out.reshape(-1) if squeeze else out
     _lib.call(f"cdft_hdiv_apply_{_suffix(geom.dtype)}", block.data_ptr(), geom.w0.data_ptr(), geom.w1.data_ptr(),
               geom.w2.data_ptr(), geom.invw.data_ptr(), geom.potential.data_ptr(), geom.packed.data_ptr(), out.data_ptr(), k, n0, n1, n2,
               ctypes.addressof(geom.taps), geom.order, stream_of(block))
@@ -347,6 +403,8 @@ def hdiv_resnorm(x: torch.Tensor, geom: StencilGeometry, evals: torch.Tensor, q:
     require(out, (k,), torch.float64, geom.device, "norms")
     if evals.dtype != torch.float64 or evals.numel() < k or not evals.is_contiguous():
         raise ValueError("hdiv_resnorm: evals must hold k contiguous float64 values")
+    if isinstance(geom, LocalGeometry):
+        return _local_resnorm(x, geom, evals, q, out, tail)
     blocks = hdiv_launch_shape(geom.shape, k, geom.order)["blocks"]
     key = workspace_key(geom.device)
     ws = _stencil_partials.get(key)
@@ -374,6 +432,26 @@ def hdiv_resnorm(x: torch.Tensor, geom: StencilGeometry, evals: torch.Tensor, q:
     return out
 
 
+def _local_resnorm(x, geom: LocalGeometry, evals, q, out, tail):
+    k = x.shape[0]
+    ws = _local_partials(geom, k)
+    n0, n1, n2 = geom.shape
+    if tail is None:
+        index, bounds, degree, table, gram, orth = -1, None, 0, None, None, None
+    else:
+        index, bounds, degree, table, gram, orth = tail
+        require(bounds, (2,), torch.float64, geom.device, "bounds")
+        require(table, (degree, 3), torch.float64, geom.device, "table")
+        require(gram, (k, k), torch.float64, geom.device, "gram")
+        require(orth, (1,), torch.float64, geom.device, "orth")
+        if not 0 <= index < k:
+            raise ValueError("hdiv_resnorm: the table's eigenvalue index must be below k")
+    _lib.call("cdft_local_resnorm_f64", x.data_ptr(), geom.potential.data_ptr(), evals.data_ptr(), q.data_ptr(), k,
+              n0, n1, n2, ctypes.addressof(geom.coef), geom.order, geom.boundary, ws.data_ptr(), ws.numel(),
+              out.data_ptr(), int(index), _ptr(bounds), int(degree), _ptr(table), _ptr(gram), _ptr(orth), stream_of(x))
+    return out
+
+
 def cheb_step(y: torch.Tensor, prev: torch.Tensor | None, geom: StencilGeometry, table: torch.Tensor, step: int,
               out: torch.Tensor) -> torch.Tensor:
     """One fused recurrence step ``(A y - c y) a - b prev`` with ``(c, a, b) = table[step]``."""
@@ -396,6 +474,11 @@ def cheb_filter(x: torch.Tensor, geom: StencilGeometry, table: torch.Tensor, deg
     if work is None:
         work = (torch.empty_like(x), torch.empty_like(x))
     n0, n1, n2 = geom.shape
+    if isinstance(geom, LocalGeometry):
+        _lib.call("cdft_local_filter_f64", x.data_ptr(), out.data_ptr(), work[0].data_ptr(), work[1].data_ptr(),
+                  geom.potential.data_ptr(), k, n0, n1, n2, ctypes.addressof(geom.coef), geom.order, geom.boundary,
+                  table.data_ptr(), int(degree), stream_of(x))
+        return out
     _lib.call(f"cdft_cheb_filter_{_suffix(geom.dtype)}", x.data_ptr(), out.data_ptr(), work[0].data_ptr(),
               work[1].data_ptr(), geom.w0.data_ptr(), geom.w1.data_ptr(), geom.w2.data_ptr(), geom.invw.data_ptr(),
               geom.potential.data_ptr(), geom.packed.data_ptr(), k, n0, n1, n2, ctypes.addressof(geom.taps), geom.order, table.data_ptr(),

@@ -115,6 +115,30 @@ int cdft_cheb_filter_f32(const float* x, float* out, float* work1, float* work2,
                          const float* gp, int k, int n0, int n1, int n2, const double* coef, int order,
                          const double* table, int degree, void* stream);
 
+/* ---- the plain operator of the analytic model systems: H = -1/2 lap + V ----
+ * Replaces LocalHamiltonian.apply / apply_with_potential (src/cdft/operators/hamiltonian.py:92-125)
+ * with the grid's Laplacian (src/cdft/grid.py:714-740, src/cdft/operators/laplacian.py:85-105):
+ * coef = the 2R + 1 central second-derivative weights c_j / h^2 (j = -R..R, order = 2R in 2..12),
+ * accumulated per axis 0, 1, 2 from -R to +R; boundary 0 = zero halo, 1 = odd reflection about walls
+ * one spacing outside the outermost points (grid.py:808-827; every axis needs >= R points). Blocks
+ * (k, N) float64 on the device, C order. */
+int cdft_local_apply_f64(const double* phi, const double* pot, double* out, int k, int n0, int n1, int n2,
+                         const double* coef, int order, int boundary, void* stream);
+/* The degree-`degree` scaled Chebyshev filter of x with this operator (as cdft_cheb_filter_f64;
+ * src/cdft/eigen/chefsi.py:81-160). */
+int cdft_local_filter_f64(const double* x, double* out, double* work1, double* work2, const double* pot, int k,
+                          int n0, int n1, int n2, const double* coef, int order, int boundary, const double* table,
+                          int degree, void* stream);
+/* The doubles `partials` must hold for cdft_local_resnorm_f64 (-1 for bad sizes). */
+long cdft_local_partials_len(int n0, int n1, int n2, int k);
+/* norms[v] = ||H phi_v - evals[v] phi_v||_q (residual_norms, src/cdft/eigen/rayleigh_ritz.py); with
+ * index >= 0 also the next filter's table and max |gram - I| in the combine's launch, as
+ * cdft_hdiv_resnorm_tail_f64 (index < 0: bounds, table, gram and orth unused). */
+int cdft_local_resnorm_f64(const double* phi, const double* pot, const double* evals, const double* q, int k, int n0,
+                           int n1, int n2, const double* coef, int order, int boundary, double* partials,
+                           long partials_len, double* norms, int index, const double* bounds, int degree,
+                           double* table, const double* gram, double* orth, void* stream);
+
 /* ---- batched grids: many independent operators in one launch ----
  * The batched scan of independent solves that the reference runs as separate processes or streams
  * (scripts/scan_throughput.py:212-313); no single reference function corresponds. */

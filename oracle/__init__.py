@@ -14,6 +14,9 @@ How each part is pinned (anything not listed is covered by a test named in the m
   assembled matrices; the checkerboard Rayleigh quotients and self-adjointness order pinned in the
   reference's ``tests/golden/values.json``; ``W == -Z^2/2`` for one nucleus; identity factor gives
   the plain Laplacian.
+* ``model``: the plain Laplacian exact on a cubic away from the zero halo; the sampled sine an
+  eigenvector of the odd-reflection operator with its closed-form eigenvalue, which is the
+  reference's recorded ``box_L10`` eigenvalue to 1.3e-14 (``tests/test_model.py``).
 * ``eigen``: bare H at 33^3 against the reference golden eigenvalue ``-0.4999999999997764`` and
   kinetic energy ``0.5000000000415459``; Chebyshev recurrence against the explicit polynomial.
 * ``quadrature``: ``integral f^2 = pi/Z^3`` (and the two-centre closed form), Lagrange reproduction

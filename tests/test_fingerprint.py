@@ -20,7 +20,7 @@ def test_derived_grids_are_the_reference_grids(pkg):
     from cdft_amd import fingerprint as fp
     from cdft_amd.geometry import UniformGrid
 
-    assert set(fp.REGISTRY) | set(fp.UNSUPPORTED) == {k for k in REFERENCE if not k.startswith("_")}
+    assert set(fp.REGISTRY) | set(fp.MODEL_IDS) == {k for k in REFERENCE if not k.startswith("_")} and not fp.UNSUPPORTED
     for sid, scenario in fp.REGISTRY.items():
         num, anchor = fp.derived_numerics(scenario)
         grid = UniformGrid.from_box(num.box_lengths, num.spacing, torch.device("cpu"), anchor=anchor,
