@@ -79,3 +79,14 @@ def test_model_systems_match_the_reference_fingerprint(pkg, device, sid):
     assert got["occupations"] == ref["occupations"]
     failing = fp.diff({sid: ref}, {sid: got}, out=lambda *_: None)
     assert failing == 0
+
+
+def test_harmonic_spectrum_against_the_analytic_levels(pkg, device):
+    """The reference's G1.2 check (``gates/tier1.py:46-90``): the lowest harmonic-well eigenvalues against
+    ``(n + 3/2) omega`` with their degeneracies (1, then the three-fold 5/2 shell) to 1e-6 Ha."""
+    from cdft_amd import model
+
+    r = model.solve_model("harmonic_w1", device, n_states=4)
+    print(f"[model] harmonic spectrum {r['eigenvalues'].tolist()} ({r['n_iterations']})", flush=True)
+    assert r["converged"]
+    np.testing.assert_allclose(r["eigenvalues"], [1.5, 2.5, 2.5, 2.5], rtol=0, atol=1e-6)
